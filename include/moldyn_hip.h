@@ -1,0 +1,201 @@
+/*
+ * moldyn_hip.h -- C ABI of the MI355X (gfx950) molecular-dynamics step engine.
+ *
+ * Drop-in for the per-step solver path of moldyn (AndrewChe7/moldyn): the
+ * integrator, Lennard-Jones potential, Berendsen / Nose-Hoover thermostat,
+ * Berendsen barostat and macro-parameter functions of the `moldyn_solver`
+ * crate.  The reference is a Rust crate without an FFI layer; every entry
+ * point below names the Rust item whose behaviour it reproduces, so that a
+ * Rust `extern "C"` block (or ctypes, or cgo) can bind it unchanged.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Positions/velocities/forces are N x 3
+ *    double arrays (x, y, z per particle), particles ordered type-major as in
+ *    State{particles: Vec<Vec<Particle>>} (core/src/particle.rs:26-32);
+ *    for a batch of replicas, replica-major, then type-major.
+ *  - Units are the reference's: nm, ps, 1e-27 kg, zJ, MPa, K (K_B below).
+ *  - Every call returns 0 on success or a negative MD_ERR_* code; the text of
+ *    the last error is md_last_error().  Nothing aborts the process (the
+ *    reference panics on the same conditions).
+ *  - A context owns its device memory and one HIP stream; one host thread
+ *    drives one context at a time.  Contexts are independent.
+ */
+#ifndef MOLDYN_HIP_H
+#define MOLDYN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MD_K_B 1.380648528 /* core/src/lib.rs:15 */
+
+enum {
+    MD_OK = 0,
+    MD_ERR_ARG = -1,          /* bad argument (null pointer, size, id) */
+    MD_ERR_HIP = -2,          /* HIP runtime failure */
+    MD_ERR_NOMEM = -3,        /* device allocation failed */
+    MD_ERR_UNIMPLEMENTED = -4,/* Custom variants: the reference's todo!() */
+    MD_ERR_BAROSTAT = -5,     /* mu^3 <= 0 or non-finite T / P */
+    MD_ERR_OVERFLOW = -6,     /* neighbour list or cell capacity could not be grown */
+    MD_ERR_NO_DEVICE = -7     /* no usable gfx950 device */
+};
+
+/* Force evaluation paths (md_options.force_mode). */
+enum {
+    MD_FORCE_FP64_ALLPAIRS = 0, /* O(N^2), reference criterion and sum order: bitwise update_force */
+    MD_FORCE_FP64_LIST = 1,     /* cell grid + Verlet list, FP64: same pair set and sum order */
+    MD_FORCE_FP32_LIST = 2      /* cell grid + Verlet list, FP32 pair forces, FP64 reductions */
+};
+
+/* Thermostat, initializer/thermostat.rs:4-20 (enum Thermostat) plus the
+ * target temperature the reference passes beside it (integrator.rs:15).
+ * kind: 0 none, 1 Berendsen{tau, lambda}, 2 NoseHoover{tau, psi, lambda},
+ * 3 Custom (rejected with MD_ERR_UNIMPLEMENTED, as the reference's todo!()).
+ * lambda and psi are in/out, mutated like the Rust enum fields. */
+typedef struct {
+    int kind;
+    double tau;
+    double target_t; /* K */
+    double lambda;
+    double psi;
+} md_thermostat;
+
+/* Barostat, initializer/barostat.rs:4-16 (enum Barostat) plus target pressure.
+ * kind: 0 none, 1 Berendsen{beta, tau, myu}, 2 Custom (MD_ERR_UNIMPLEMENTED).
+ * myu is in/out. */
+typedef struct {
+    int kind;
+    double beta;
+    double tau;
+    double target_p; /* program units (MPa) */
+    double myu;
+} md_barostat;
+
+/* Engine options (no reference counterpart: the reference has one O(N^2) path). */
+typedef struct {
+    int force_mode;        /* MD_FORCE_*; default MD_FORCE_FP32_LIST */
+    double skin;           /* Verlet skin, nm; default 0.3 * sigma(0,0) */
+    int block_size;        /* threads per block of the per-atom kernels: 64, 128, 256 or 512; default 256 */
+    int list_capacity;     /* initial neighbours per atom; 0 = automatic; grown on overflow */
+    int cell_capacity;     /* initial atoms per cell slot array; 0 = automatic; grown on overflow */
+    double cells_per_atom; /* cell-grid budget per replica, cells per atom; default 2 */
+    int graph_chunk;       /* steps per captured hipGraph; 0 = default 32, 1 = no graphs */
+} md_options;
+
+typedef struct md_ctx md_ctx;
+
+/* Lennard-Jones scalar, potential.rs:57-75 Potential::get_potential_and_force:
+ * returns U(r) - u_cut and dU/dr, both 0 when r > r_cut.  Host-side. */
+void md_lj_potential_and_force(double sigma, double eps, double r_cut, double u_cut, double r,
+                               double* u, double* dudr);
+/* potential.rs:27-55 Potential::new_lennard_jones: r_cut = 2.5 sigma, u_cut = U(r_cut). */
+void md_new_lennard_jones(double sigma, double eps, double* r_cut, double* u_cut);
+
+/* Context for one system: initializer/position.rs:116-137 initialize_particles
+ * (counts per type) + the masses the ParticleDatabase holds
+ * (core/src/particles_database.rs:67-79) + State::boundary_box. */
+int md_create(md_ctx** out, int n_types, const int64_t* counts, const double* masses,
+              const double box[3]);
+/* Batch of independent replicas in one context: counts is n_replicas x n_types,
+ * boxes n_replicas x 3; masses per type shared.  No reference counterpart
+ * (the reference runs one State at a time). */
+int md_create_replicas(md_ctx** out, int n_replicas, int n_types, const int64_t* counts,
+                       const double* masses, const double* boxes);
+void md_destroy(md_ctx* ctx);
+const char* md_last_error(const md_ctx* ctx);
+/* Text for an error raised before a context exists (md_create failures). */
+const char* md_global_error(void);
+
+int md_get_options(const md_ctx* ctx, md_options* out);
+int md_set_options(md_ctx* ctx, const md_options* opt);
+
+/* PotentialsDatabase::set_potential(t0, t1, Potential::new_lennard_jones(sigma, eps))
+ * (potential.rs:141-144, 27-55).  Unset pairs use the default Ar LJ(0.3418, 1.712)
+ * (potential.rs:95-101). */
+int md_set_pair_potential(md_ctx* ctx, uint16_t t0, uint16_t t1, double sigma, double eps);
+/* Same with all four serialized fields, as read from potentials.json
+ * (potential.rs:104-139: Potential::LennardJones{sigma, eps, r_cut, u_cut}). */
+int md_set_pair_potential_lj(md_ctx* ctx, uint16_t t0, uint16_t t1, double sigma, double eps,
+                             double r_cut, double u_cut);
+
+/* Replace positions, velocities and box (save_data.rs:129-151 StateToSave -> State):
+ * forces, potentials and virials are zeroed like a freshly loaded State.
+ * box is 3 doubles per replica. */
+int md_upload_state(md_ctx* ctx, const double* x, const double* v, const double* box);
+/* Hand over the force-side fields of a State (Particle::force, ::potential,
+ * ::temp, core/src/particle.rs:11-16) after md_upload_state, so that a step
+ * continues from a state whose forces the caller already holds
+ * (integrator.rs:28-34 reads them in the first half kick).  Any pointer may
+ * be NULL (that field stays zero). */
+int md_upload_forces(md_ctx* ctx, const double* force, const double* pot, const double* virial);
+/* Copy back; any output pointer may be NULL.  F is N x 3, pot and virial are
+ * Particle::potential and Particle::temp of the last force evaluation. */
+int md_download_state(md_ctx* ctx, double* x, double* v, double* force, double* pot,
+                      double* virial, double* box);
+
+/* update_force(&PotentialsDatabase, &mut State), potential.rs:158-216. */
+int md_update_force(md_ctx* ctx);
+/* State::apply_boundary_conditions, core/src/particle.rs:120-141. */
+int md_apply_boundary_conditions(md_ctx* ctx);
+
+/* n_steps x Integrator::VerletMethod.calculate(db, state, dt, barostat, thermostat)
+ * (solver/integrator.rs:14-64).  th / ba may be NULL (Option::None); their
+ * lambda, psi and myu fields are updated in place as the Rust enums are. */
+int md_step(md_ctx* ctx, double dt, md_thermostat* th, md_barostat* ba, int64_t n_steps);
+/* Same for a replica batch: th and ba are arrays of n_replicas (or NULL). */
+int md_step_replicas(md_ctx* ctx, double dt, md_thermostat* th, md_barostat* ba,
+                     int64_t n_steps);
+
+/* Thermostat::calculate_lambda / update (initializer/thermostat.rs:24-72) and
+ * Barostat::calculate_myu / update (initializer/barostat.rs:21-56) on the
+ * device-resident state of replica 0, for callers that drive the integrator
+ * themselves.  dt, type_id and target have the Rust meaning. */
+int md_thermostat_calculate_lambda(md_ctx* ctx, md_thermostat* th, double dt, uint16_t type_id);
+int md_thermostat_update(md_ctx* ctx, md_thermostat* th, double dt, uint16_t type_id);
+int md_barostat_calculate_myu(md_ctx* ctx, md_barostat* ba, double dt, uint16_t type_id);
+int md_barostat_update(md_ctx* ctx, md_barostat* ba, uint16_t type_id);
+
+/* Macro parameters of (replica, type) for the current state (macro_parameters/):
+ * out[0] kinetic energy   (energy.rs:14-22)
+ * out[1] potential energy (energy.rs:40-49, sum / 2)
+ * out[2] thermal energy   (energy.rs:25-37)
+ * out[3] temperature K    (temperature.rs:4-7, per-type count)
+ * out[4] pressure         (pressure.rs:5-20)
+ * out[5..7] momentum      (mod.rs:28-34)
+ * out[8..10] centre-of-mass velocity (mod.rs:12-25) */
+int md_macro(md_ctx* ctx, int replica, uint16_t type_id, double out[11]);
+
+/* Lattice initialisers (host-side), initializer/position.rs:34-99:
+ * kind 0 UnitCell::U (grid[0]*grid[1]*grid[2] sites), 1 UnitCell::FCC (4 per cell). */
+int md_init_lattice(int kind, const double start[3], const int64_t grid[3], double a, double* pos);
+/* Seeded Maxwell-Boltzmann velocities with pair negation, structure of
+ * initializer/velocity.rs:6-29 (the reference's thread_rng is unseeded; this
+ * uses SplitMix64 + Box-Muller so inputs are reproducible). */
+int md_init_mb_velocities(double* vel, int64_t n, double temperature, double mass, uint64_t seed);
+
+/* Observability (no reference counterpart). */
+typedef struct {
+    int64_t steps;          /* real steps taken in the last md_step call */
+    int64_t launches;       /* step-kernel launches (includes list-refresh-only launches) */
+    int64_t rebuilds;       /* neighbour-list rebuilds in the last call */
+    int64_t refresh_steps;  /* launches that only refilled the cell grid */
+    int64_t retries;        /* capacity-overflow replays */
+    int32_t list_capacity;
+    int32_t cell_capacity;
+    double mean_list_len;   /* at the last rebuild */
+    int32_t max_list_len;
+    int32_t n_slots;        /* padded particle slots on the device */
+} md_stats;
+int md_get_stats(md_ctx* ctx, md_stats* out);
+/* Per-kernel device timing with hipEvents around every launch of the step
+ * kernel (kernel 0), the finalize kernel (1), the all-pairs force kernel (2)
+ * and the drift kernel (3).  Enabling it disables graph capture. */
+int md_set_profiling(md_ctx* ctx, int on);
+int md_kernel_time(md_ctx* ctx, int kernel, double* total_ms, int64_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1364 @@
+// md_api.hip -- host runtime behind include/moldyn_hip.h.
+//
+// A context owns the slot layout, device buffers, one HIP stream and a
+// captured hipGraph of `graph_chunk` step launches.  md_step runs in chunks:
+// snapshot the state (list paths), prime the controls on the device, replay
+// the graph, let replicas that spent launches on list refreshes catch up,
+// then check the device's overflow flags; on overflow the chunk is replayed
+// from the snapshot with larger capacities (results do not depend on when
+// lists are rebuilt, because every list is sorted and pairs outside r_cut
+// contribute nothing).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "md_device.h"
+#include "moldyn_hip.h"
+
+using namespace md;
+
+static thread_local std::string g_error;
+
+namespace {
+
+enum { K_STEP = 0, K_FINALIZE = 1, K_ALLPAIRS = 2, K_DRIFT = 3, K_NUM = 4 };
+
+struct EvPair {
+    hipEvent_t a, b;
+    int kernel;
+};
+
+}  // namespace
+
+struct md_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int R = 1, T = 1;
+    std::vector<int64_t> counts;  // R x T
+    std::vector<double> masses;   // T
+    std::vector<double> box0;     // R x 3 (box at creation)
+    md_options opt{};
+    bool skin_set = false;
+    std::map<std::pair<int, int>, std::pair<double, double>> pot_lj;  // sigma, eps
+    std::map<std::pair<int, int>, std::pair<double, double>> pot_cut; // r_cut, u_cut
+    std::vector<PairParam> pairs;
+    // layout
+    int bs = 256;
+    int64_t n_slots = 0, n_atoms = 0, total_cells = 0;
+    int n_blocks = 0;
+    std::vector<GroupInfo> groups;
+    std::vector<RepInfo> reps;
+    std::vector<int32_t> blk_group;
+    // device
+    DevState d{};
+    std::vector<void*> base_allocs;
+    std::vector<void*> list_allocs;
+    void* snap = nullptr;
+    size_t snap_bytes = 0;
+    RepCtrl* snap_ctrl = nullptr;
+    bool fp32 = true, list_mode = true;
+    bool sums_valid = false;
+    double last_dt = 0.002;
+    double rlist = 0.0, rc_max = 0.0;
+    // graphs
+    hipGraphExec_t gexec = nullptr;
+    int g_len = 0;
+    // profiling
+    bool prof = false;
+    std::vector<EvPair> ev_pool;
+    size_t ev_used = 0;
+    double ktime[K_NUM] = {0, 0, 0, 0};
+    int64_t kcount[K_NUM] = {0, 0, 0, 0};
+    md_stats stats{};
+    int64_t last_rebuilds = 0, last_refresh = 0;
+    std::string err;
+};
+
+// ------------------------------------------------------------------ helpers
+
+static int fail(md_ctx* c, int code, const std::string& msg) {
+    if (c)
+        c->err = msg;
+    else
+        g_error = msg;
+    return code;
+}
+
+#define HIPCHK(c, expr)                                                                     \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail((c), MD_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static double fold_mass(double m, int64_t n) {
+    // macro_parameters/mod.rs:17-22: sequential fold of 1.0 * m
+    double a = 0.0;
+    for (int64_t i = 0; i < n; ++i) a = a + 1.0 * m;
+    return a;
+}
+
+static void lj_scalar(double sigma, double eps, double r_cut, double u_cut, double r, double* u,
+                      double* f) {
+    // potential.rs:57-75
+    if (r > r_cut) {
+        *u = 0.0;
+        *f = 0.0;
+        return;
+    }
+    const double sr = sigma / r;
+    const double s2 = sr * sr;
+    const double s4 = s2 * s2;
+    const double s6 = s2 * s4;
+    const double s12 = s6 * s6;
+    *u = 4.0 * eps * (s12 - s6) - u_cut;
+    *f = (24.0 * eps / r) * (s6 - 2.0 * s12);
+}
+
+static double r2max_of(double r_cut) {
+    // largest double d with sqrt(d) <= r_cut (sqrt is correctly rounded, hence monotone)
+    double r2 = r_cut * r_cut;
+    while (r2 > 0.0 && sqrt(r2) > r_cut) r2 = nextafter(r2, 0.0);
+    while (sqrt(nextafter(r2, INFINITY)) <= r_cut) r2 = nextafter(r2, INFINITY);
+    return r2;
+}
+
+static void rebuild_pairs(md_ctx* c) {
+    const int T = c->T;
+    c->pairs.assign((size_t)T * T, PairParam{});
+    c->rc_max = 0.0;
+    for (int a = 0; a < T; ++a)
+        for (int b = 0; b < T; ++b) {
+            const std::pair<int, int> key(std::min(a, b), std::max(a, b));
+            double sigma = 0.3418, eps = 1.712, rc, uc;  // potential.rs:96
+            auto it = c->pot_lj.find(key);
+            if (it != c->pot_lj.end()) {
+                sigma = it->second.first;
+                eps = it->second.second;
+            }
+            auto ic = c->pot_cut.find(key);
+            if (ic != c->pot_cut.end()) {
+                rc = ic->second.first;
+                uc = ic->second.second;
+            } else {
+                md_new_lennard_jones(sigma, eps, &rc, &uc);
+            }
+            PairParam p;
+            p.sigma = sigma;
+            p.eps = eps;
+            p.r_cut = rc;
+            p.u_cut = uc;
+            p.r2max = r2max_of(rc);
+            p.sig2f = (float)(sigma * sigma);
+            p.eps24f = (float)(24.0 * eps);
+            p.eps4f = (float)(4.0 * eps);
+            p.ucutf = (float)uc;
+            c->pairs[(size_t)a * T + b] = p;
+            c->rc_max = std::max(c->rc_max, rc);
+        }
+    if (!c->skin_set) c->opt.skin = 0.3 * c->pairs[0].sigma;
+    c->rlist = c->rc_max + c->opt.skin;
+}
+
+static int dalloc(md_ctx* c, std::vector<void*>& owner, void** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, MD_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(bytes) +
+                                         "): " + hipGetErrorString(e));
+    }
+    owner.push_back(*p);
+    return MD_OK;
+}
+
+static void free_list(std::vector<void*>& v) {
+    for (void* p : v) (void)hipFree(p);
+    v.clear();
+}
+
+static void drop_graph(md_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->g_len = 0;
+}
+
+static void free_all(md_ctx* c) {
+    drop_graph(c);
+    free_list(c->list_allocs);
+    free_list(c->base_allocs);
+    if (c->snap) (void)hipFree(c->snap);
+    c->snap = nullptr;
+    c->snap_bytes = 0;
+    if (c->snap_ctrl) (void)hipFree(c->snap_ctrl);
+    c->snap_ctrl = nullptr;
+}
+
+static void make_layout(md_ctx* c) {
+    const int bs = c->bs;
+    c->groups.clear();
+    c->reps.clear();
+    c->blk_group.clear();
+    int64_t slot = 0, cell_base = 0;
+    int block = 0;
+    c->n_atoms = 0;
+    for (int r = 0; r < c->R; ++r) {
+        RepInfo R{};
+        R.start = slot;
+        R.first_group = (int32_t)c->groups.size();
+        R.n_types = c->T;
+        R.n_atoms = 0;
+        for (int t = 0; t < c->T; ++t) {
+            GroupInfo G{};
+            G.start = slot;
+            G.count = c->counts[(size_t)r * c->T + t];
+            G.nblocks = (int32_t)((G.count + bs - 1) / bs);
+            G.first_block = block;
+            G.rep = r;
+            G.type = t;
+            G.mass = c->masses[t];
+            G.sum_m = fold_mass(G.mass, G.count);
+            for (int b = 0; b < G.nblocks; ++b) c->blk_group.push_back((int32_t)c->groups.size());
+            slot += (int64_t)G.nblocks * bs;
+            block += G.nblocks;
+            R.n_atoms += G.count;
+            c->groups.push_back(G);
+        }
+        R.end = slot;
+        const double want = ceil(std::max(0.1, c->opt.cells_per_atom) * (double)R.n_atoms);
+        R.cell_budget = std::max<int64_t>(27, (int64_t)want);
+        R.cell_base = cell_base;
+        cell_base += R.cell_budget;
+        c->n_atoms += R.n_atoms;
+        c->reps.push_back(R);
+    }
+    c->n_slots = slot;
+    c->n_blocks = block;
+    c->total_cells = cell_base;
+}
+
+static int auto_capacities(md_ctx* c) {
+    // expected neighbours inside r_list and atoms per cell at the current density
+    int lc = 8, cc = 8;
+    for (int r = 0; r < c->R; ++r) {
+        const double* b = &c->box0[(size_t)r * 3];
+        const double vol = b[0] * b[1] * b[2];
+        const double n = (double)c->reps[r].n_atoms;
+        if (n <= 0 || !(vol > 0)) continue;
+        const double rho = n / vol;
+        const double nb = 4.0 / 3.0 * M_PI * c->rlist * c->rlist * c->rlist * rho;
+        lc = std::max(lc, (int)ceil(2.0 * nb + 8.0));
+        int gd[3];
+        grid_dims(b, c->rlist, c->reps[r].cell_budget, gd);
+        const double per = n / ((double)gd[0] * gd[1] * gd[2]);
+        cc = std::max(cc, (int)ceil(per + 6.0 * sqrt(per) + 6.0));
+    }
+    if (c->opt.list_capacity <= 0) c->d.list_cap = lc;
+    if (c->opt.cell_capacity <= 0) c->d.cell_cap = cc;
+    return MD_OK;
+}
+
+static int alloc_lists(md_ctx* c) {
+    free_list(c->list_allocs);
+    drop_graph(c);
+    void* p;
+    int rc;
+    if (!c->list_mode) {
+        c->d.nbr = nullptr;
+        c->d.ncount = nullptr;
+        c->d.cell_hdr[0] = c->d.cell_hdr[1] = nullptr;
+        c->d.cell_slots[0] = c->d.cell_slots[1] = nullptr;
+        return MD_OK;
+    }
+    const int64_t N = c->n_slots;
+    if ((rc = dalloc(c, c->list_allocs, &p, sizeof(int32_t) * (size_t)N * c->d.list_cap))) return rc;
+    c->d.nbr = (int32_t*)p;
+    if ((rc = dalloc(c, c->list_allocs, &p, sizeof(int32_t) * (size_t)N))) return rc;
+    c->d.ncount = (int32_t*)p;
+    HIPCHK(c, hipMemsetAsync(c->d.ncount, 0, sizeof(int32_t) * (size_t)N, c->st));
+    for (int k = 0; k < 2; ++k) {
+        if ((rc = dalloc(c, c->list_allocs, &p, sizeof(unsigned long long) * (size_t)c->total_cells)))
+            return rc;
+        c->d.cell_hdr[k] = (unsigned long long*)p;
+        HIPCHK(c, hipMemsetAsync(p, 0, sizeof(unsigned long long) * (size_t)c->total_cells, c->st));
+        if ((rc = dalloc(c, c->list_allocs, &p,
+                         sizeof(int32_t) * (size_t)c->total_cells * c->d.cell_cap)))
+            return rc;
+        c->d.cell_slots[k] = (int32_t*)p;
+    }
+    return MD_OK;
+}
+
+static int upload_pairs(md_ctx* c) {
+    HIPCHK(c, hipMemcpyAsync((void*)c->d.pairs, c->pairs.data(), sizeof(PairParam) * c->pairs.size(),
+                             hipMemcpyHostToDevice, c->st));
+    return MD_OK;
+}
+
+// Allocate everything for the current layout; state buffers are NaN / zero.
+static int alloc_base(md_ctx* c) {
+    free_all(c);
+    make_layout(c);
+    c->fp32 = c->opt.force_mode == MD_FORCE_FP32_LIST;
+    c->list_mode = c->opt.force_mode != MD_FORCE_FP64_ALLPAIRS;
+    DevState& d = c->d;
+    d = DevState{};
+    const int64_t N = c->n_slots;
+    const size_t nd = sizeof(double) * (size_t)std::max<int64_t>(N, 1);
+    void* p;
+    int rc;
+    double** dptrs[6] = {d.x, d.y, d.z, d.vx, d.vy, d.vz};
+    for (int k = 0; k < 6; ++k)
+        for (int b = 0; b < 2; ++b) {
+            if ((rc = dalloc(c, c->base_allocs, &p, nd))) return rc;
+            dptrs[k][b] = (double*)p;
+        }
+    void** fptrs[3] = {d.fx, d.fy, d.fz};
+    for (int k = 0; k < 3; ++k)
+        for (int b = 0; b < 2; ++b) {
+            if ((rc = dalloc(c, c->base_allocs, &p, nd))) return rc;
+            fptrs[k][b] = p;
+        }
+    if ((rc = dalloc(c, c->base_allocs, &p, nd))) return rc;
+    d.pot = (double*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, nd))) return rc;
+    d.vir = (double*)p;
+    std::vector<uint8_t> st((size_t)std::max<int64_t>(N, 1), 0);
+    for (const GroupInfo& G : c->groups)
+        for (int64_t i = 0; i < (int64_t)G.nblocks * c->bs; ++i) st[(size_t)(G.start + i)] = (uint8_t)G.type;
+    if ((rc = dalloc(c, c->base_allocs, &p, st.size()))) return rc;
+    d.slot_type = (const uint8_t*)p;
+    HIPCHK(c, hipMemcpy(p, st.data(), st.size(), hipMemcpyHostToDevice));
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(int32_t) * std::max<size_t>(c->blk_group.size(), 1))))
+        return rc;
+    d.blk_group = (const int32_t*)p;
+    if (!c->blk_group.empty())
+        HIPCHK(c, hipMemcpy(p, c->blk_group.data(), sizeof(int32_t) * c->blk_group.size(),
+                            hipMemcpyHostToDevice));
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(GroupInfo) * c->groups.size()))) return rc;
+    d.groups = (const GroupInfo*)p;
+    HIPCHK(c, hipMemcpy(p, c->groups.data(), sizeof(GroupInfo) * c->groups.size(),
+                        hipMemcpyHostToDevice));
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(RepInfo) * c->reps.size()))) return rc;
+    d.reps = (const RepInfo*)p;
+    HIPCHK(c, hipMemcpy(p, c->reps.data(), sizeof(RepInfo) * c->reps.size(), hipMemcpyHostToDevice));
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(RepCtrl) * c->R))) return rc;
+    d.ctrl = (RepCtrl*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(RepParams) * c->R))) return rc;
+    d.params = (const RepParams*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(RepStats)))) return rc;
+    d.stats = (RepStats*)p;
+    HIPCHK(c, hipMemset(p, 0, sizeof(RepStats)));
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(PairParam) * (size_t)c->T * c->T))) return rc;
+    d.pairs = (const PairParam*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * (size_t)std::max(c->n_blocks, 1))))
+        return rc;
+    d.partial = (double*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * c->groups.size()))) return rc;
+    d.sums = (double*)p;
+    HIPCHK(c, hipMemset(p, 0, sizeof(double) * NS * c->groups.size()));
+    for (int t = 0; t < c->T && t < kMaxTypes; ++t) d.mass_t[t] = c->masses[t];
+    d.n_types = c->T;
+    d.n_blocks = c->n_blocks;
+    d.n_slots = N;
+    d.list_cap = c->opt.list_capacity > 0 ? c->opt.list_capacity : 8;
+    d.cell_cap = c->opt.cell_capacity > 0 ? c->opt.cell_capacity : 8;
+    rebuild_pairs(c);
+    if ((rc = upload_pairs(c))) return rc;
+    auto_capacities(c);
+    if ((rc = alloc_lists(c))) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return MD_OK;
+}
+
+static int read_ctrl(md_ctx* c, std::vector<RepCtrl>& out) {
+    out.resize(c->R);
+    HIPCHK(c, hipMemcpyAsync(out.data(), c->d.ctrl, sizeof(RepCtrl) * c->R, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return MD_OK;
+}
+
+static int write_ctrl(md_ctx* c, const std::vector<RepCtrl>& in) {
+    HIPCHK(c, hipMemcpyAsync(c->d.ctrl, in.data(), sizeof(RepCtrl) * c->R, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return MD_OK;
+}
+
+static RepParams base_params(md_ctx* c) {
+    RepParams P{};
+    P.dt = c->last_dt;
+    P.list_mode = c->list_mode ? 1 : 0;
+    P.rlist = c->rlist;
+    P.rc_max = c->rc_max;
+    P.margin = 1e-9 * c->rlist;
+    P.insert_growth = 1.0;
+    P.target_step = 0;
+    return P;
+}
+
+static int write_params(md_ctx* c, const std::vector<RepParams>& P) {
+    HIPCHK(c, hipMemcpyAsync((void*)c->d.params, P.data(), sizeof(RepParams) * c->R,
+                             hipMemcpyHostToDevice, c->st));
+    return MD_OK;
+}
+
+static int ensure_sums(md_ctx* c) {
+    if (c->sums_valid) return MD_OK;
+    std::vector<RepParams> P((size_t)c->R, base_params(c));
+    int rc;
+    if ((rc = write_params(c, P))) return rc;
+    if (c->n_blocks > 0) HIPCHK(c, launch_reduce_state(c->d, c->fp32, c->bs, c->st));
+    HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->sums_valid = true;
+    return MD_OK;
+}
+
+static int read_sums(md_ctx* c, std::vector<double>& S) {
+    S.resize((size_t)NS * c->groups.size());
+    HIPCHK(c, hipMemcpyAsync(S.data(), c->d.sums, sizeof(double) * S.size(), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return MD_OK;
+}
+
+// ------------------------------------------------------------------ profiling
+
+static hipError_t ev_begin(md_ctx* c, int kernel, EvPair** out) {
+    *out = nullptr;
+    if (!c->prof) return hipSuccess;
+    if (c->ev_used == c->ev_pool.size()) {
+        EvPair e;
+        hipError_t r = hipEventCreate(&e.a);
+        if (r != hipSuccess) return r;
+        r = hipEventCreate(&e.b);
+        if (r != hipSuccess) return r;
+        c->ev_pool.push_back(e);
+    }
+    EvPair* e = &c->ev_pool[c->ev_used++];
+    e->kernel = kernel;
+    *out = e;
+    return hipEventRecord(e->a, c->st);
+}
+
+static hipError_t ev_end(md_ctx* c, EvPair* e) {
+    if (!e) return hipSuccess;
+    return hipEventRecord(e->b, c->st);
+}
+
+static void ev_collect(md_ctx* c) {
+    for (size_t i = 0; i < c->ev_used; ++i) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_pool[i].a, c->ev_pool[i].b) == hipSuccess) {
+            c->ktime[c->ev_pool[i].kernel] += ms;
+            c->kcount[c->ev_pool[i].kernel] += 1;
+        }
+    }
+    c->ev_used = 0;
+}
+
+// One step launch sequence: list paths k_step + k_finalize, all-pairs path
+// k_drift + k_allpairs + k_finalize.
+static hipError_t launch_once(md_ctx* c) {
+    hipError_t e;
+    EvPair* ev;
+    if (c->list_mode) {
+        if ((e = ev_begin(c, K_STEP, &ev))) return e;
+        if ((e = launch_step(c->d, c->fp32, c->bs, c->st))) return e;
+        if ((e = ev_end(c, ev))) return e;
+    } else {
+        if ((e = ev_begin(c, K_DRIFT, &ev))) return e;
+        if ((e = launch_drift(c->d, c->bs, c->st))) return e;
+        if ((e = ev_end(c, ev))) return e;
+        if ((e = ev_begin(c, K_ALLPAIRS, &ev))) return e;
+        if ((e = launch_allpairs(c->d, c->bs, false, c->st))) return e;
+        if ((e = ev_end(c, ev))) return e;
+    }
+    if ((e = ev_begin(c, K_FINALIZE, &ev))) return e;
+    if ((e = launch_finalize(c->d, M_STEP, c->R, c->st))) return e;
+    return ev_end(c, ev);
+}
+
+static int ensure_graph(md_ctx* c, int len) {
+    if (c->gexec && c->g_len == len) return MD_OK;
+    drop_graph(c);
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < len && e == hipSuccess; ++k) e = launch_once(c);
+    hipError_t e2 = hipStreamEndCapture(c->st, &g);
+    if (e != hipSuccess) return fail(c, MD_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(c, MD_ERR_HIP, std::string("end capture: ") + hipGetErrorString(e2));
+    e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(c, MD_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    }
+    c->g_len = len;
+    return MD_OK;
+}
+
+// ------------------------------------------------------------------ snapshot
+
+static size_t state_bytes(md_ctx* c) {
+    return sizeof(double) * (size_t)c->n_slots * 11;  // x y z vx vy vz fx fy fz pot vir
+}
+
+static int snapshot(md_ctx* c, bool save, int buf) {
+    const size_t n = sizeof(double) * (size_t)c->n_slots;
+    if (!c->snap) {
+        c->snap_bytes = std::max<size_t>(state_bytes(c), 256);
+        HIPCHK(c, hipMalloc(&c->snap, c->snap_bytes));
+        HIPCHK(c, hipMalloc((void**)&c->snap_ctrl, sizeof(RepCtrl) * c->R));
+    }
+    char* s = (char*)c->snap;
+    void* arrs[11] = {c->d.x[buf], c->d.y[buf], c->d.z[buf], c->d.vx[buf], c->d.vy[buf], c->d.vz[buf],
+                      c->d.fx[buf], c->d.fy[buf], c->d.fz[buf], c->d.pot, c->d.vir};
+    for (int k = 0; k < 11; ++k) {
+        if (save)
+            HIPCHK(c, hipMemcpyAsync(s + k * n, arrs[k], n, hipMemcpyDeviceToDevice, c->st));
+        else
+            HIPCHK(c, hipMemcpyAsync(arrs[k], s + k * n, n, hipMemcpyDeviceToDevice, c->st));
+    }
+    if (save)
+        HIPCHK(c, hipMemcpyAsync(c->snap_ctrl, c->d.ctrl, sizeof(RepCtrl) * c->R, hipMemcpyDeviceToDevice, c->st));
+    else
+        HIPCHK(c, hipMemcpyAsync(c->d.ctrl, c->snap_ctrl, sizeof(RepCtrl) * c->R, hipMemcpyDeviceToDevice, c->st));
+    return MD_OK;
+}
+
+static int reset_stats(md_ctx* c) {
+    HIPCHK(c, hipMemsetAsync(c->d.stats, 0, sizeof(RepStats), c->st));
+    return MD_OK;
+}
+
+static int read_stats(md_ctx* c, RepStats& s) {
+    HIPCHK(c, hipMemcpyAsync(&s, c->d.stats, sizeof(RepStats), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return MD_OK;
+}
+
+// Grow list / cell capacity after an overflow; returns 1 when grown.
+static int grow_capacity(md_ctx* c, const RepStats& s, int* grown) {
+    *grown = 0;
+    if (s.list_overflow) {
+        c->d.list_cap = std::max(c->d.list_cap * 2, (int)(s.max_list * 1.25) + 4);
+        *grown = 1;
+    }
+    if (s.cell_overflow) {
+        c->d.cell_cap = std::max(c->d.cell_cap * 2, (int)(s.max_cell * 1.25) + 4);
+        *grown = 1;
+    }
+    if (*grown) return alloc_lists(c);
+    return MD_OK;
+}
+
+// Build lists at the current positions and evaluate forces (list paths).
+static int list_force_eval(md_ctx* c) {
+    int rc;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        std::vector<RepCtrl> C;
+        if ((rc = read_ctrl(c, C))) return rc;
+        for (int r = 0; r < c->R; ++r) {
+            RepCtrl& X = C[r];
+            X.ins_epoch = X.fill_epoch + 1u;
+            const int b = X.ins_epoch & 1;
+            for (int k = 0; k < 3; ++k) X.gbox[b][k] = X.box[k];
+            grid_dims(X.gbox[b], c->rlist, c->reps[r].cell_budget, X.gdim[b]);
+            X.fill_epoch = X.ins_epoch;
+        }
+        if ((rc = write_ctrl(c, C))) return rc;
+        std::vector<RepParams> P((size_t)c->R, base_params(c));
+        if ((rc = write_params(c, P))) return rc;
+        if ((rc = reset_stats(c))) return rc;
+        if (c->n_blocks > 0) {
+            HIPCHK(c, launch_insert_all(c->d, c->bs, c->st));
+            HIPCHK(c, launch_force_eval(c->d, c->fp32, c->bs, c->st));
+        }
+        HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->st));
+        RepStats s;
+        if ((rc = read_stats(c, s))) return rc;
+        int grown = 0;
+        if ((rc = grow_capacity(c, s, &grown))) return rc;
+        if (grown) continue;
+        if ((rc = read_ctrl(c, C))) return rc;
+        for (int r = 0; r < c->R; ++r) {
+            C[r].flags = (C[r].flags & ~(F_REBUILD | F_INSERT | F_NULL | F_EXIT | F_WRITE_PV)) |
+                         F_HAVE_LIST | F_GRID_FRESH;
+            C[r].D = 0.0;
+            C[r].s = 1.0;
+        }
+        if ((rc = write_ctrl(c, C))) return rc;
+        c->sums_valid = true;
+        c->stats.list_capacity = c->d.list_cap;
+        c->stats.cell_capacity = c->d.cell_cap;
+        return MD_OK;
+    }
+    return fail(c, MD_ERR_OVERFLOW, "neighbour list / cell capacity kept overflowing");
+}
+
+// ------------------------------------------------------------------ step
+
+static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, int64_t n,
+                     bool per_rep) {
+    if (n < 0) return fail(c, MD_ERR_ARG, "n_steps < 0");
+    if (n == 0) return MD_OK;
+    for (int r = 0; r < c->R; ++r) {
+        const md_thermostat* t = th ? &th[per_rep ? r : 0] : nullptr;
+        const md_barostat* b = ba ? &ba[per_rep ? r : 0] : nullptr;
+        if (t && (t->kind == 3)) return fail(c, MD_ERR_UNIMPLEMENTED, "Thermostat::Custom is todo!() in the reference");
+        if (t && (t->kind < 0 || t->kind > 3)) return fail(c, MD_ERR_ARG, "bad thermostat kind");
+        if (b && (b->kind == 2)) return fail(c, MD_ERR_UNIMPLEMENTED, "Barostat::Custom is todo!() in the reference");
+        if (b && (b->kind < 0 || b->kind > 2)) return fail(c, MD_ERR_ARG, "bad barostat kind");
+    }
+    for (const GroupInfo& G : c->groups)
+        if (G.count == 0)
+            return fail(c, MD_ERR_ARG, "a particle type has no particles (the reference indexes particle_type[0])");
+    c->last_dt = dt;
+    int rc;
+    if ((rc = ensure_sums(c))) return rc;
+    std::vector<RepCtrl> C;
+    if ((rc = read_ctrl(c, C))) return rc;
+    const int64_t step0 = C[0].step;
+    std::vector<RepParams> P((size_t)c->R, base_params(c));
+    for (int r = 0; r < c->R; ++r) {
+        RepParams& p = P[r];
+        p.dt = dt;
+        const md_thermostat* t = th ? &th[per_rep ? r : 0] : nullptr;
+        const md_barostat* b = ba ? &ba[per_rep ? r : 0] : nullptr;
+        if (t && t->kind > 0) {
+            p.th_kind = t->kind;
+            p.th_tau = t->tau;
+            p.th_target = t->target_t;
+            p.psi_in = t->kind == 2 ? t->psi : 0.0;
+        }
+        if (b && b->kind > 0) {
+            p.ba_kind = b->kind;
+            p.ba_beta = b->beta;
+            p.ba_tau = b->tau;
+            p.ba_target = b->target_p;
+        }
+        // the box and positions scale inconsistently with several types under a
+        // barostat (barostat.rs:45 runs once per type): no skin can be trusted
+        p.rebuild_every_step = (c->list_mode && c->T > 1 && p.ba_kind == 1) ? 1 : 0;
+        if (C[r].step != step0) return fail(c, MD_ERR_ARG, "replicas out of step");
+    }
+    const int K = c->prof ? 1 : (c->opt.graph_chunk > 0 ? c->opt.graph_chunk : 32);
+    const int64_t chunk = c->list_mode ? std::max<int64_t>(512, 16 * (int64_t)K) : 64;
+    int64_t done = 0;
+    bool first = true, bad = false;
+    int retries = 0;
+    md_stats st{};
+    while (done < n) {
+        const int64_t cn = std::min(n - done, chunk);
+        int buf = C[0].cur;
+        if (c->list_mode) {
+            if ((rc = snapshot(c, true, buf))) return rc;
+        }
+        for (int r = 0; r < c->R; ++r) {
+            P[r].target_step = step0 + done + cn;
+            P[r].use_psi_in = first ? 1 : 0;
+        }
+        if ((rc = write_params(c, P))) return rc;
+        if ((rc = reset_stats(c))) return rc;
+        HIPCHK(c, launch_finalize(c->d, M_PRIME, c->R, c->st));
+        int64_t k = 0;
+        if (!c->prof && K > 1 && cn >= K) {
+            if ((rc = ensure_graph(c, K))) return rc;
+            for (; k + K <= cn; k += K) HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+        }
+        for (; k < cn; ++k) HIPCHK(c, launch_once(c));
+        int64_t launched = cn;
+        for (int iter = 0;; ++iter) {
+            if ((rc = read_ctrl(c, C))) return rc;
+            if (c->prof) ev_collect(c);
+            int64_t behind = 0;
+            for (int r = 0; r < c->R; ++r) behind = std::max(behind, P[r].target_step - C[r].step);
+            if (behind <= 0) break;
+            if (iter > 64) return fail(c, MD_ERR_HIP, "replicas did not reach the target step");
+            for (int64_t q = 0; q < behind; ++q) HIPCHK(c, launch_once(c));
+            launched += behind;
+        }
+        RepStats s;
+        if ((rc = read_stats(c, s))) return rc;
+        if (s.list_overflow || s.cell_overflow) {
+            int grown = 0;
+            if ((rc = grow_capacity(c, s, &grown))) return rc;
+            if (++retries > 8) return fail(c, MD_ERR_OVERFLOW, "capacity kept overflowing");
+            if ((rc = snapshot(c, false, buf))) return rc;
+            if ((rc = read_ctrl(c, C))) return rc;
+            for (int r = 0; r < c->R; ++r) C[r].flags &= ~(F_HAVE_LIST | F_GRID_FRESH);
+            if ((rc = write_ctrl(c, C))) return rc;
+            st.retries++;
+            continue;
+        }
+        for (int r = 0; r < c->R; ++r)
+            if (C[r].flags & F_BAD) bad = true;
+        st.launches += launched;
+        done += cn;
+        first = false;
+    }
+    for (int r = 0; r < c->R; ++r) {
+        md_thermostat* t = th ? &th[per_rep ? r : 0] : nullptr;
+        md_barostat* b = ba ? &ba[per_rep ? r : 0] : nullptr;
+        if (t && t->kind > 0 && (per_rep || r == 0)) {
+            t->lambda = C[r].lambda_used;
+            if (t->kind == 2) t->psi = C[r].psi_used;
+        }
+        if (b && b->kind > 0 && (per_rep || r == 0)) b->myu = C[r].myu_used;
+    }
+    st.steps = n;
+    int64_t rb = 0, nl = 0;
+    for (int r = 0; r < c->R; ++r) {
+        rb += C[r].rebuilds;
+        nl += C[r].nulls;
+    }
+    st.rebuilds = rb - c->stats.rebuilds;  // counters in ctrl are cumulative
+    st.refresh_steps = nl - c->stats.refresh_steps;
+    c->stats.rebuilds = rb;
+    c->stats.refresh_steps = nl;
+    c->stats.steps = st.steps;
+    c->stats.launches = st.launches;
+    c->stats.retries = st.retries;
+    c->stats.list_capacity = c->d.list_cap;
+    c->stats.cell_capacity = c->d.cell_cap;
+    c->stats.n_slots = (int32_t)c->n_slots;
+    c->last_rebuilds = st.rebuilds;
+    c->last_refresh = st.refresh_steps;
+    c->sums_valid = true;
+    if (bad) return fail(c, MD_ERR_BAROSTAT, "non-finite temperature/pressure or mu^3 <= 0 during the run");
+    return MD_OK;
+}
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+void md_lj_potential_and_force(double sigma, double eps, double r_cut, double u_cut, double r,
+                               double* u, double* dudr) {
+    double a, b;
+    lj_scalar(sigma, eps, r_cut, u_cut, r, &a, &b);
+    if (u) *u = a;
+    if (dudr) *dudr = b;
+}
+
+void md_new_lennard_jones(double sigma, double eps, double* r_cut, double* u_cut) {
+    // potential.rs:27-55: r_cut = sigma * 2.5, u_cut = U(r_cut) evaluated with u_cut = 0
+    const double rc = sigma * 2.5;
+    double u, f;
+    lj_scalar(sigma, eps, rc, 0.0, rc, &u, &f);
+    if (r_cut) *r_cut = rc;
+    if (u_cut) *u_cut = u;
+}
+
+const char* md_global_error(void) { return g_error.c_str(); }
+
+const char* md_last_error(const md_ctx* c) { return c ? c->err.c_str() : g_error.c_str(); }
+
+static int check_device(int* dev) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return fail(nullptr, MD_ERR_NO_DEVICE, "no HIP device visible");
+    if (hipGetDevice(dev) != hipSuccess) return fail(nullptr, MD_ERR_NO_DEVICE, "hipGetDevice failed");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, *dev) != hipSuccess)
+        return fail(nullptr, MD_ERR_NO_DEVICE, "hipGetDeviceProperties failed");
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(nullptr, MD_ERR_NO_DEVICE,
+                    std::string("kernels are built for gfx950, device is ") + prop.gcnArchName);
+    return MD_OK;
+}
+
+int md_create_replicas(md_ctx** out, int n_replicas, int n_types, const int64_t* counts,
+                       const double* masses, const double* boxes) {
+    if (!out) return fail(nullptr, MD_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (n_replicas < 1 || n_types < 1 || n_types > kMaxTypes || !counts || !masses || !boxes)
+        return fail(nullptr, MD_ERR_ARG, "bad replica/type counts or NULL arrays (1..8 types)");
+    int64_t total = 0;
+    for (int64_t i = 0; i < (int64_t)n_replicas * n_types; ++i) {
+        if (counts[i] < 0) return fail(nullptr, MD_ERR_ARG, "negative particle count");
+        total += counts[i];
+    }
+    if (total >= (int64_t)1 << 30) return fail(nullptr, MD_ERR_ARG, "too many particles for 32-bit slot ids");
+    int dev = 0;
+    int rc = check_device(&dev);
+    if (rc) return rc;
+    md_ctx* c = new md_ctx();
+    c->device = dev;
+    c->R = n_replicas;
+    c->T = n_types;
+    c->counts.assign(counts, counts + (size_t)n_replicas * n_types);
+    c->masses.assign(masses, masses + n_types);
+    c->box0.assign(boxes, boxes + (size_t)n_replicas * 3);
+    c->opt.force_mode = MD_FORCE_FP32_LIST;
+    c->opt.skin = 0.0;
+    c->opt.block_size = 256;
+    c->opt.list_capacity = 0;
+    c->opt.cell_capacity = 0;
+    c->opt.cells_per_atom = 2.0;
+    c->opt.graph_chunk = 32;
+    c->bs = 256;
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(nullptr, MD_ERR_HIP, "hipStreamCreate failed");
+    }
+    rc = alloc_base(c);
+    if (rc) {
+        g_error = c->err;
+        md_destroy(c);
+        return rc;
+    }
+    std::vector<double> zeros((size_t)total * 3, 0.0);
+    rc = md_upload_state(c, zeros.data(), zeros.data(), c->box0.data());
+    if (rc) {
+        g_error = c->err;
+        md_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return MD_OK;
+}
+
+int md_create(md_ctx** out, int n_types, const int64_t* counts, const double* masses,
+              const double box[3]) {
+    return md_create_replicas(out, 1, n_types, counts, masses, box);
+}
+
+void md_destroy(md_ctx* c) {
+    if (!c) return;
+    (void)hipStreamSynchronize(c->st);
+    free_all(c);
+    for (auto& e : c->ev_pool) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+int md_get_options(const md_ctx* c, md_options* out) {
+    if (!c || !out) return MD_ERR_ARG;
+    *out = c->opt;
+    return MD_OK;
+}
+
+int md_upload_state(md_ctx* c, const double* x, const double* v, const double* box) {
+    if (!c || !x || !v || !box) return fail(c, MD_ERR_ARG, "NULL argument");
+    const int64_t N = c->n_slots;
+    std::vector<double> h[6];
+    for (int k = 0; k < 6; ++k) h[k].assign((size_t)std::max<int64_t>(N, 1), k < 3 ? NAN : 0.0);
+    int64_t a = 0;
+    for (const GroupInfo& G : c->groups)
+        for (int64_t i = 0; i < G.count; ++i, ++a) {
+            const size_t s = (size_t)(G.start + i);
+            for (int k = 0; k < 3; ++k) {
+                h[k][s] = x[3 * a + k];
+                h[3 + k][s] = v[3 * a + k];
+            }
+        }
+    const size_t nb = sizeof(double) * (size_t)std::max<int64_t>(N, 1);
+    double* dp[6][2] = {{c->d.x[0], c->d.x[1]}, {c->d.y[0], c->d.y[1]}, {c->d.z[0], c->d.z[1]},
+                        {c->d.vx[0], c->d.vx[1]}, {c->d.vy[0], c->d.vy[1]}, {c->d.vz[0], c->d.vz[1]}};
+    for (int k = 0; k < 6; ++k)
+        for (int b = 0; b < 2; ++b) HIPCHK(c, hipMemcpyAsync(dp[k][b], h[k].data(), nb, hipMemcpyHostToDevice, c->st));
+    for (int b = 0; b < 2; ++b) {
+        HIPCHK(c, hipMemsetAsync(c->d.fx[b], 0, nb, c->st));
+        HIPCHK(c, hipMemsetAsync(c->d.fy[b], 0, nb, c->st));
+        HIPCHK(c, hipMemsetAsync(c->d.fz[b], 0, nb, c->st));
+    }
+    HIPCHK(c, hipMemsetAsync(c->d.pot, 0, nb, c->st));
+    HIPCHK(c, hipMemsetAsync(c->d.vir, 0, nb, c->st));
+    std::vector<RepCtrl> C((size_t)c->R);
+    for (int r = 0; r < c->R; ++r) {
+        RepCtrl X{};
+        for (int k = 0; k < 3; ++k) X.box[k] = box[3 * r + k];
+        X.lambda = X.myu = 1.0;
+        X.lambda_used = X.myu_used = 1.0;
+        X.s = 1.0;
+        X.D = 0.0;
+        X.flags = 0;
+        X.cur = 0;
+        C[r] = X;
+        for (int k = 0; k < 3; ++k) c->box0[(size_t)r * 3 + k] = box[3 * r + k];
+    }
+    int rc = write_ctrl(c, C);
+    if (rc) return rc;
+    c->sums_valid = false;
+    c->stats.rebuilds = 0;
+    c->stats.refresh_steps = 0;
+    return MD_OK;
+}
+
+int md_upload_forces(md_ctx* c, const double* force, const double* pot, const double* vir) {
+    if (!c) return MD_ERR_ARG;
+    std::vector<RepCtrl> C;
+    int rc = read_ctrl(c, C);
+    if (rc) return rc;
+    const int buf = C[0].cur;
+    const int64_t N = c->n_slots;
+    const size_t n = (size_t)std::max<int64_t>(N, 1);
+    std::vector<double> h[5];
+    for (int k = 0; k < 5; ++k) h[k].assign(n, 0.0);
+    int64_t a = 0;
+    for (const GroupInfo& G : c->groups)
+        for (int64_t i = 0; i < G.count; ++i, ++a) {
+            const size_t s = (size_t)(G.start + i);
+            if (force)
+                for (int k = 0; k < 3; ++k) h[k][s] = force[3 * a + k];
+            if (pot) h[3][s] = pot[a];
+            if (vir) h[4][s] = vir[a];
+        }
+    void* fd[3] = {c->d.fx[buf], c->d.fy[buf], c->d.fz[buf]};
+    for (int k = 0; k < 3; ++k) {
+        if (c->fp32) {
+            std::vector<float> t(h[k].begin(), h[k].end());
+            HIPCHK(c, hipMemcpy(fd[k], t.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+        } else {
+            HIPCHK(c, hipMemcpy(fd[k], h[k].data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        }
+    }
+    HIPCHK(c, hipMemcpy(c->d.pot, h[3].data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d.vir, h[4].data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    c->sums_valid = false;
+    return MD_OK;
+}
+
+int md_download_state(md_ctx* c, double* x, double* v, double* force, double* pot, double* vir,
+                      double* box) {
+    if (!c) return MD_ERR_ARG;
+    std::vector<RepCtrl> C;
+    int rc = read_ctrl(c, C);
+    if (rc) return rc;
+    const int64_t N = c->n_slots;
+    const size_t nb = sizeof(double) * (size_t)std::max<int64_t>(N, 1);
+    const int buf = C[0].cur;
+    auto fetch = [&](const void* src, std::vector<double>& dst, bool f32) -> int {
+        dst.assign((size_t)std::max<int64_t>(N, 1), 0.0);
+        if (f32) {
+            std::vector<float> tmp(dst.size());
+            HIPCHK(c, hipMemcpy(tmp.data(), src, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < tmp.size(); ++i) dst[i] = (double)tmp[i];
+        } else {
+            HIPCHK(c, hipMemcpy(dst.data(), src, nb, hipMemcpyDeviceToHost));
+        }
+        return MD_OK;
+    };
+    std::vector<double> a[3];
+    auto scatter = [&](double* out, int comp_stride, int comp) {
+        int64_t k = 0;
+        for (const GroupInfo& G : c->groups)
+            for (int64_t i = 0; i < G.count; ++i, ++k) out[comp_stride * k + comp] = a[comp][(size_t)(G.start + i)];
+    };
+    if (x) {
+        if ((rc = fetch(c->d.x[buf], a[0], false)) || (rc = fetch(c->d.y[buf], a[1], false)) ||
+            (rc = fetch(c->d.z[buf], a[2], false)))
+            return rc;
+        for (int k = 0; k < 3; ++k) scatter(x, 3, k);
+    }
+    if (v) {
+        if ((rc = fetch(c->d.vx[buf], a[0], false)) || (rc = fetch(c->d.vy[buf], a[1], false)) ||
+            (rc = fetch(c->d.vz[buf], a[2], false)))
+            return rc;
+        for (int k = 0; k < 3; ++k) scatter(v, 3, k);
+    }
+    if (force) {
+        if ((rc = fetch(c->d.fx[buf], a[0], c->fp32)) || (rc = fetch(c->d.fy[buf], a[1], c->fp32)) ||
+            (rc = fetch(c->d.fz[buf], a[2], c->fp32)))
+            return rc;
+        for (int k = 0; k < 3; ++k) scatter(force, 3, k);
+    }
+    if (pot) {
+        if ((rc = fetch(c->d.pot, a[0], false))) return rc;
+        scatter(pot, 1, 0);
+    }
+    if (vir) {
+        if ((rc = fetch(c->d.vir, a[0], false))) return rc;
+        scatter(vir, 1, 0);
+    }
+    if (box)
+        for (int r = 0; r < c->R; ++r)
+            for (int k = 0; k < 3; ++k) box[3 * r + k] = C[r].box[k];
+    return MD_OK;
+}
+
+// Relayout keeping the state (used when block size / force path change).
+static int relayout(md_ctx* c, const md_options& nopt) {
+    const int64_t n = c->n_atoms;
+    std::vector<double> x((size_t)n * 3), v((size_t)n * 3), f((size_t)n * 3), pot((size_t)n),
+        vir((size_t)n), box((size_t)c->R * 3);
+    int rc = md_download_state(c, x.data(), v.data(), f.data(), pot.data(), vir.data(), box.data());
+    if (rc) return rc;
+    std::vector<RepCtrl> C;
+    if ((rc = read_ctrl(c, C))) return rc;
+    c->opt = nopt;
+    c->bs = nopt.block_size;
+    for (int r = 0; r < c->R; ++r)
+        for (int k = 0; k < 3; ++k) c->box0[(size_t)r * 3 + k] = box[(size_t)r * 3 + k];
+    if ((rc = alloc_base(c))) return rc;
+    if ((rc = md_upload_state(c, x.data(), v.data(), box.data()))) return rc;
+    // restore forces, potentials and virials in the new precision
+    const int64_t N = c->n_slots;
+    std::vector<double> h[5];
+    for (int k = 0; k < 5; ++k) h[k].assign((size_t)std::max<int64_t>(N, 1), 0.0);
+    int64_t a = 0;
+    for (const GroupInfo& G : c->groups)
+        for (int64_t i = 0; i < G.count; ++i, ++a) {
+            const size_t s = (size_t)(G.start + i);
+            for (int k = 0; k < 3; ++k) h[k][s] = f[(size_t)(3 * a + k)];
+            h[3][s] = pot[(size_t)a];
+            h[4][s] = vir[(size_t)a];
+        }
+    void* fd[3] = {c->d.fx[0], c->d.fy[0], c->d.fz[0]};
+    for (int k = 0; k < 3; ++k) {
+        if (c->fp32) {
+            std::vector<float> t(h[k].begin(), h[k].end());
+            HIPCHK(c, hipMemcpy(fd[k], t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice));
+        } else {
+            HIPCHK(c, hipMemcpy(fd[k], h[k].data(), sizeof(double) * h[k].size(), hipMemcpyHostToDevice));
+        }
+    }
+    HIPCHK(c, hipMemcpy(c->d.pot, h[3].data(), sizeof(double) * h[3].size(), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d.vir, h[4].data(), sizeof(double) * h[4].size(), hipMemcpyHostToDevice));
+    std::vector<RepCtrl> C2;
+    if ((rc = read_ctrl(c, C2))) return rc;
+    for (int r = 0; r < c->R; ++r) {
+        C2[r].lambda = C[r].lambda;
+        C2[r].myu = C[r].myu;
+        C2[r].psi = C[r].psi;
+        C2[r].lambda_used = C[r].lambda_used;
+        C2[r].myu_used = C[r].myu_used;
+        C2[r].psi_used = C[r].psi_used;
+        C2[r].step = C[r].step;
+        C2[r].rebuilds = C[r].rebuilds;
+        C2[r].nulls = C[r].nulls;
+    }
+    return write_ctrl(c, C2);
+}
+
+int md_set_options(md_ctx* c, const md_options* o) {
+    if (!c || !o) return fail(c, MD_ERR_ARG, "NULL argument");
+    md_options n = *o;
+    if (n.force_mode < 0 || n.force_mode > 2) return fail(c, MD_ERR_ARG, "bad force_mode");
+    if (n.block_size == 0) n.block_size = 256;
+    if (n.block_size != 64 && n.block_size != 128 && n.block_size != 256 && n.block_size != 512)
+        return fail(c, MD_ERR_ARG, "block_size must be 64, 128, 256 or 512");
+    if (!(n.cells_per_atom > 0)) n.cells_per_atom = 2.0;
+    if (n.graph_chunk < 0) return fail(c, MD_ERR_ARG, "graph_chunk < 0");
+    if (n.skin < 0) return fail(c, MD_ERR_ARG, "skin < 0");
+    c->skin_set = n.skin > 0;
+    const bool relay = n.block_size != c->opt.block_size || n.force_mode != c->opt.force_mode ||
+                       n.cells_per_atom != c->opt.cells_per_atom;
+    if (relay) {
+        int rc = relayout(c, n);
+        if (rc) return rc;
+    } else {
+        c->opt = n;
+    }
+    rebuild_pairs(c);
+    if (n.list_capacity > 0) c->d.list_cap = n.list_capacity;
+    if (n.cell_capacity > 0) c->d.cell_cap = n.cell_capacity;
+    if (n.list_capacity <= 0 || n.cell_capacity <= 0) auto_capacities(c);
+    int rc = alloc_lists(c);
+    if (rc) return rc;
+    rc = upload_pairs(c);
+    if (rc) return rc;
+    std::vector<RepCtrl> C;
+    if ((rc = read_ctrl(c, C))) return rc;
+    for (auto& X : C) X.flags &= ~(F_HAVE_LIST | F_GRID_FRESH);
+    c->opt.skin = c->opt.skin;  // resolved by rebuild_pairs when unset
+    return write_ctrl(c, C);
+}
+
+static int set_pot(md_ctx* c, uint16_t t0, uint16_t t1) {
+    (void)t0;
+    (void)t1;
+    rebuild_pairs(c);
+    int rc = upload_pairs(c);
+    if (rc) return rc;
+    std::vector<RepCtrl> C;
+    if ((rc = read_ctrl(c, C))) return rc;
+    for (auto& X : C) X.flags &= ~(F_HAVE_LIST | F_GRID_FRESH);
+    c->sums_valid = false;
+    return write_ctrl(c, C);
+}
+
+int md_set_pair_potential(md_ctx* c, uint16_t t0, uint16_t t1, double sigma, double eps) {
+    if (!c || t0 >= c->T || t1 >= c->T) return fail(c, MD_ERR_ARG, "bad type id");
+    const std::pair<int, int> key(std::min(t0, t1), std::max(t0, t1));
+    c->pot_lj[key] = {sigma, eps};
+    c->pot_cut.erase(key);
+    return set_pot(c, t0, t1);
+}
+
+int md_set_pair_potential_lj(md_ctx* c, uint16_t t0, uint16_t t1, double sigma, double eps,
+                             double r_cut, double u_cut) {
+    if (!c || t0 >= c->T || t1 >= c->T) return fail(c, MD_ERR_ARG, "bad type id");
+    if (!(r_cut > 0)) return fail(c, MD_ERR_ARG, "r_cut must be positive");
+    const std::pair<int, int> key(std::min(t0, t1), std::max(t0, t1));
+    c->pot_lj[key] = {sigma, eps};
+    c->pot_cut[key] = {r_cut, u_cut};
+    return set_pot(c, t0, t1);
+}
+
+int md_update_force(md_ctx* c) {
+    if (!c) return MD_ERR_ARG;
+    if (c->list_mode) return list_force_eval(c);
+    std::vector<RepParams> P((size_t)c->R, base_params(c));
+    int rc = write_params(c, P);
+    if (rc) return rc;
+    if (c->n_blocks > 0) HIPCHK(c, launch_allpairs(c->d, c->bs, true, c->st));
+    HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->sums_valid = true;
+    return MD_OK;
+}
+
+int md_apply_boundary_conditions(md_ctx* c) {
+    if (!c) return MD_ERR_ARG;
+    if (c->n_blocks > 0) HIPCHK(c, launch_wrap(c->d, c->bs, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return MD_OK;
+}
+
+int md_step(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, int64_t n_steps) {
+    if (!c) return MD_ERR_ARG;
+    return run_steps(c, dt, th, ba, n_steps, false);
+}
+
+int md_step_replicas(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, int64_t n_steps) {
+    if (!c) return MD_ERR_ARG;
+    return run_steps(c, dt, th, ba, n_steps, true);
+}
+
+// Temperature and pressure of (replica 0, type) from the device sums.
+static int type_thermo(md_ctx* c, int rep, int type, double* temperature, double* pressure,
+                       double out[11]) {
+    int rc = ensure_sums(c);
+    if (rc) return rc;
+    std::vector<double> S;
+    if ((rc = read_sums(c, S))) return rc;
+    std::vector<RepCtrl> C;
+    if ((rc = read_ctrl(c, C))) return rc;
+    const int g = c->reps[rep].first_group + type;
+    const GroupInfo& G = c->groups[g];
+    const double* s = &S[(size_t)g * NS];
+    const double M = G.sum_m;
+    const double cx = s[S_MVX] / M, cy = s[S_MVY] / M, cz = s[S_MVZ] / M;
+    const double th2 = s[S_MV2] - M * ((cx * cx + cy * cy) + cz * cz);
+    const double thermal = th2 / 2.0;
+    const double t = (2.0 * thermal) / (3.0 * (double)G.count * MD_K_B) * 100.0;
+    const double* b = C[rep].box;
+    const double vol = b[0] * b[1] * b[2];
+    const double p = (th2 + (-s[S_VIR]) * 0.5) / vol / 3.0;
+    if (temperature) *temperature = t;
+    if (pressure) *pressure = p;
+    if (out) {
+        out[0] = s[S_MV2] / 2.0;
+        out[1] = s[S_POT] / 2.0;
+        out[2] = thermal;
+        out[3] = t;
+        out[4] = p;
+        out[5] = s[S_MVX];
+        out[6] = s[S_MVY];
+        out[7] = s[S_MVZ];
+        out[8] = cx;
+        out[9] = cy;
+        out[10] = cz;
+    }
+    return MD_OK;
+}
+
+int md_macro(md_ctx* c, int replica, uint16_t type_id, double out[11]) {
+    if (!c || !out || replica < 0 || replica >= c->R || type_id >= c->T)
+        return fail(c, MD_ERR_ARG, "bad argument");
+    return type_thermo(c, replica, type_id, nullptr, nullptr, out);
+}
+
+int md_thermostat_calculate_lambda(md_ctx* c, md_thermostat* th, double dt, uint16_t type_id) {
+    if (!c || !th || type_id >= c->T) return fail(c, MD_ERR_ARG, "bad argument");
+    if (th->kind == 3) return fail(c, MD_ERR_UNIMPLEMENTED, "Thermostat::Custom is todo!() in the reference");
+    double t;
+    int rc = type_thermo(c, 0, type_id, &t, nullptr, nullptr);
+    if (rc) return rc;
+    if (th->kind == 1) {
+        th->lambda = sqrt(1.0 + dt / th->tau * (th->target_t / t - 1.0));
+    } else if (th->kind == 2) {
+        const double psi_dot = -((th->target_t / t) - 1.0) / th->tau;
+        th->psi += psi_dot * (dt / 2.0);
+        th->lambda = exp(-th->psi * dt / 2.0);
+    }
+    return MD_OK;
+}
+
+int md_thermostat_update(md_ctx* c, md_thermostat* th, double dt, uint16_t type_id) {
+    if (!c || !th || type_id >= c->T) return fail(c, MD_ERR_ARG, "bad argument");
+    if (th->kind == 3) return fail(c, MD_ERR_UNIMPLEMENTED, "Thermostat::Custom is todo!() in the reference");
+    double t;
+    int rc = type_thermo(c, 0, type_id, &t, nullptr, nullptr);
+    if (rc) return rc;
+    if (th->kind == 1 || th->kind == 2) {
+        const GroupInfo& G = c->groups[c->reps[0].first_group + type_id];
+        HIPCHK(c, launch_scale(c->d, c->bs, G.start, G.count, th->lambda, 0, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        c->sums_valid = false;
+    }
+    if (th->kind == 2) {
+        const double psi_dot = -((th->target_t / t) - 1.0) / th->tau;
+        th->psi += psi_dot * (dt / 2.0);
+    }
+    return MD_OK;
+}
+
+int md_barostat_calculate_myu(md_ctx* c, md_barostat* ba, double dt, uint16_t type_id) {
+    if (!c || !ba || type_id >= c->T) return fail(c, MD_ERR_ARG, "bad argument");
+    if (ba->kind == 2) return fail(c, MD_ERR_UNIMPLEMENTED, "Barostat::Custom is todo!() in the reference");
+    double p;
+    int rc = type_thermo(c, 0, type_id, nullptr, &p, nullptr);
+    if (rc) return rc;
+    if (ba->kind == 1) ba->myu = cbrt(1.0 + dt * ba->beta / ba->tau * (p - ba->target_p));
+    return MD_OK;
+}
+
+int md_barostat_update(md_ctx* c, md_barostat* ba, uint16_t type_id) {
+    if (!c || !ba || type_id >= c->T) return fail(c, MD_ERR_ARG, "bad argument");
+    if (ba->kind == 2) return fail(c, MD_ERR_UNIMPLEMENTED, "Barostat::Custom is todo!() in the reference");
+    if (ba->kind != 1) return MD_OK;
+    const GroupInfo& G = c->groups[c->reps[0].first_group + type_id];
+    HIPCHK(c, launch_scale(c->d, c->bs, G.start, G.count, ba->myu, 1, c->st));
+    std::vector<RepCtrl> C;
+    int rc = read_ctrl(c, C);
+    if (rc) return rc;
+    for (int k = 0; k < 3; ++k) C[0].box[k] *= ba->myu;
+    C[0].flags &= ~(F_HAVE_LIST | F_GRID_FRESH);
+    c->sums_valid = false;
+    return write_ctrl(c, C);
+}
+
+int md_init_lattice(int kind, const double start[3], const int64_t grid[3], double a, double* pos) {
+    if (!start || !grid || !pos || (kind != 0 && kind != 1)) return fail(nullptr, MD_ERR_ARG, "bad argument");
+    // initializer/position.rs:34-99: index x * ny * nz + y * nz + z
+    for (int64_t x = 0; x < grid[0]; ++x)
+        for (int64_t y = 0; y < grid[1]; ++y)
+            for (int64_t z = 0; z < grid[2]; ++z) {
+                const int64_t cell = x * grid[1] * grid[2] + y * grid[2] + z;
+                const double X = (double)x, Y = (double)y, Z = (double)z;
+                if (kind == 0) {
+                    double* p = pos + 3 * cell;
+                    p[0] = start[0] + X * a;
+                    p[1] = start[1] + Y * a;
+                    p[2] = start[2] + Z * a;
+                } else {
+                    double* p = pos + 12 * cell;
+                    const double sites[4][3] = {{X, Y, Z}, {X, Y + 0.5, Z + 0.5}, {X + 0.5, Y, Z + 0.5},
+                                                {X + 0.5, Y + 0.5, Z}};
+                    for (int k = 0; k < 4; ++k)
+                        for (int dd = 0; dd < 3; ++dd) p[3 * k + dd] = start[dd] + sites[k][dd] * a;
+                }
+            }
+    return MD_OK;
+}
+
+// SplitMix64 stream and Box-Muller pairs (cos draw first, then sin).
+static uint64_t sm64(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static double open01(uint64_t* s) {
+    return ((double)(sm64(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+int md_init_mb_velocities(double* vel, int64_t n, double temperature, double mass, uint64_t seed) {
+    if (!vel || n < 0 || !(mass > 0)) return fail(nullptr, MD_ERR_ARG, "bad argument");
+    // initializer/velocity.rs:6-29: sigma = sqrt(K_B * T * 0.01 / m), i and i + n/2 opposite
+    const double tp = temperature * 0.01;
+    const double sigma = sqrt(MD_K_B * tp / mass);
+    uint64_t s = seed;
+    bool have = false;
+    double spare = 0.0;
+    auto normal = [&]() {
+        if (have) {
+            have = false;
+            return spare;
+        }
+        const double u1 = open01(&s), u2 = open01(&s);
+        const double r = sqrt(-2.0 * log(u1));
+        const double th = 6.283185307179586 * u2;
+        spare = r * sin(th);
+        have = true;
+        return r * cos(th);
+    };
+    memset(vel, 0, sizeof(double) * 3 * (size_t)n);
+    const int64_t h = n / 2;
+    for (int64_t i = 0; i < h; ++i) {
+        const double vx = sigma * normal();
+        const double vy = sigma * normal();
+        const double vz = sigma * normal();
+        vel[3 * i] = vx;
+        vel[3 * i + 1] = vy;
+        vel[3 * i + 2] = vz;
+        vel[3 * (i + h)] = -vx;
+        vel[3 * (i + h) + 1] = -vy;
+        vel[3 * (i + h) + 2] = -vz;
+    }
+    return MD_OK;
+}
+
+int md_get_stats(md_ctx* c, md_stats* out) {
+    if (!c || !out) return MD_ERR_ARG;
+    md_stats s = c->stats;
+    s.steps = c->stats.steps;
+    s.rebuilds = c->last_rebuilds;
+    s.refresh_steps = c->last_refresh;
+    s.list_capacity = c->d.list_cap;
+    s.cell_capacity = c->d.cell_cap;
+    s.n_slots = (int32_t)c->n_slots;
+    s.mean_list_len = 0.0;
+    s.max_list_len = 0;
+    if (c->list_mode && c->n_slots > 0) {
+        std::vector<int32_t> nc((size_t)c->n_slots);
+        if (hipMemcpy(nc.data(), c->d.ncount, sizeof(int32_t) * nc.size(), hipMemcpyDeviceToHost) == hipSuccess) {
+            double sum = 0;
+            int mx = 0;
+            for (const GroupInfo& G : c->groups)
+                for (int64_t i = 0; i < G.count; ++i) {
+                    const int v = nc[(size_t)(G.start + i)];
+                    sum += v;
+                    mx = std::max(mx, v);
+                }
+            s.mean_list_len = c->n_atoms ? sum / (double)c->n_atoms : 0.0;
+            s.max_list_len = mx;
+        }
+    }
+    *out = s;
+    return MD_OK;
+}
+
+int md_set_profiling(md_ctx* c, int on) {
+    if (!c) return MD_ERR_ARG;
+    c->prof = on != 0;
+    for (int k = 0; k < K_NUM; ++k) {
+        c->ktime[k] = 0.0;
+        c->kcount[k] = 0;
+    }
+    c->ev_used = 0;
+    return MD_OK;
+}
+
+int md_kernel_time(md_ctx* c, int kernel, double* total_ms, int64_t* launches) {
+    if (!c || kernel < 0 || kernel >= K_NUM) return MD_ERR_ARG;
+    if (total_ms) *total_ms = c->ktime[kernel];
+    if (launches) *launches = c->kcount[kernel];
+    return MD_OK;
+}
+
+}  // extern "C"

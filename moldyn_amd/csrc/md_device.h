@@ -1,0 +1,178 @@
+// md_device.h -- device data layout and kernel launch interface of the
+// MI355X step engine.  Shared by md_kernels.hip (device code) and md_api.hip
+// (host runtime behind the C ABI).
+//
+// Layout in HBM (one context, R replicas x T types):
+//  * particles live in "slots"; each (replica, type) group starts at a
+//    multiple of the per-atom block size, so every block belongs to exactly
+//    one group and the per-group FP64 reductions need no segmenting.
+//    Padding slots hold NaN positions, which fail every distance test.
+//  * state is SoA and double-buffered: x/y/z and vx/vy/vz in FP64, forces
+//    FP64 (exact paths) or FP32 (fast path).  The fused step kernel reads
+//    buffer `cur` and writes `cur ^ 1`.
+//  * Verlet lists are column-major (nbr[k * n_slots + slot]) so that the k-th
+//    neighbour of 64 consecutive slots is one coalesced load.
+//  * the cell grid is two epoch-tagged header arrays (count | epoch << 32)
+//    with fixed-capacity slot arrays, so filling it never needs a clear and a
+//    fill can run while the other buffer is being read.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace md {
+
+constexpr int kMaxTypes = 8;
+constexpr double kKB = 1.380648528;  // core/src/lib.rs:15
+
+// per-launch control flags of a replica
+enum : uint32_t {
+    F_REBUILD = 1u << 0,    // rebuild Verlet lists from grid `fill_epoch` at launch start
+    F_INSERT = 1u << 1,     // insert this launch's output positions into grid `ins_epoch`
+    F_NULL = 1u << 2,       // no time step: insert the input positions, rebuild next launch
+    F_EXIT = 1u << 3,       // target step reached: launch does nothing
+    F_WRITE_PV = 1u << 4,   // store per-atom potential and virial in this launch
+    F_HAVE_LIST = 1u << 5,  // lists exist (built at some fill)
+    F_GRID_FRESH = 1u << 6, // grid `fill_epoch` holds the current input positions
+    F_BAD = 1u << 7,        // non-finite T/P or mu^3 <= 0 seen
+};
+
+// per-block partial / per-group sum columns
+enum {
+    S_MVX, S_MVY, S_MVZ,  // sum m v (state after the step)
+    S_MV2,                // sum m |v|^2
+    S_POT,                // sum potential
+    S_VIR,                // sum virial (Particle::temp)
+    S_KVX, S_KVY, S_KVZ,  // sum m v after the first half kick (Nose-Hoover)
+    S_KV2,                // sum m |v|^2 after the first half kick
+    S_WV,                 // max |v| of the state after the step
+    S_WA,                 // max |F| / m of the state after the step
+    S_VMAX,               // max |v| of this step's drift
+    NS
+};
+constexpr int kFirstMax = S_WV;
+
+struct PairParam {
+    double sigma, eps, r_cut, u_cut;
+    double r2max;  // largest double d with sqrt(d) <= r_cut
+    float sig2f, eps24f, eps4f, ucutf;
+};
+
+struct GroupInfo {
+    int64_t start, count;
+    int32_t first_block, nblocks;
+    int32_t rep, type;
+    double mass;   // Particle::mass of the type
+    double sum_m;  // sequential fold of the masses
+};
+
+struct RepInfo {
+    int64_t start, end;  // slot range
+    int32_t first_group, n_types;
+    int64_t cell_base;   // first cell of this replica in each grid buffer
+    int64_t cell_budget; // cells available
+    int64_t n_atoms;
+};
+
+struct RepParams {
+    double dt;
+    int32_t th_kind, ba_kind;  // thermostat 0/1 Berendsen/2 Nose-Hoover, barostat 0/1
+    double th_tau, th_target, ba_beta, ba_tau, ba_target;
+    int64_t target_step;
+    int32_t list_mode;         // 1 when a Verlet list is used
+    int32_t rebuild_every_step;
+    double rlist, rc_max, margin, insert_growth;
+    double psi_in;        // Nose-Hoover psi handed in by the caller
+    int32_t use_psi_in;   // M_PRIME starts from psi_in instead of psi_used
+};
+
+struct RepCtrl {
+    double box[3];             // box at the start of the launch
+    double lambda, myu, psi;   // controls of the next real step (psi after calculate_lambda)
+    double lambda_used, myu_used, psi_used;  // of the last completed step (caller-visible)
+    double s, D;               // scale since list build, drift bound in build units
+    double gbox[2][3];         // box of grid buffer 0/1
+    int32_t gdim[2][3];
+    uint32_t flags;
+    uint32_t ins_epoch;        // epoch this launch inserts with
+    uint32_t fill_epoch;       // epoch of the last completed fill
+    int32_t cur;               // state buffer holding the input
+    int64_t step;              // completed real steps
+    int64_t rebuilds, nulls, launches;
+};
+
+struct RepStats {
+    unsigned int list_overflow, cell_overflow;
+    int max_list, max_cell;
+    unsigned long long list_sum;
+    unsigned long long list_atoms;
+};
+
+struct DevState {
+    double* x[2];
+    double* y[2];
+    double* z[2];
+    double* vx[2];
+    double* vy[2];
+    double* vz[2];
+    void* fx[2];
+    void* fy[2];
+    void* fz[2];
+    double* pot;
+    double* vir;
+    const uint8_t* slot_type;
+    int32_t* nbr;
+    int32_t* ncount;
+    int32_t list_cap;
+    unsigned long long* cell_hdr[2];
+    int32_t* cell_slots[2];
+    int32_t cell_cap;
+    const int32_t* blk_group;
+    const GroupInfo* groups;
+    const RepInfo* reps;
+    RepCtrl* ctrl;
+    const RepParams* params;
+    RepStats* stats;
+    const PairParam* pairs;  // n_types x n_types
+    double* partial;         // n_blocks x NS
+    double* sums;            // n_groups x NS
+    double mass_t[kMaxTypes];
+    int32_t n_types;
+    int32_t n_blocks;
+    int64_t n_slots;
+};
+
+// Cell grid of a fill: per axis floor(L / (r_list (1 + 1e-9))) cells in
+// [1, 1024], coarsened until the replica's cell budget holds.  Cell edges stay
+// >= r_list, so the 27-cell stencil around a particle covers its list sphere.
+__host__ __device__ inline void grid_dims(const double* gb, double rlist, int64_t budget, int* gd) {
+    for (int k = 0; k < 3; ++k) {
+        const double q = floor(gb[k] / (rlist * (1.0 + 1e-9)));
+        gd[k] = q < 1.0 ? 1 : (q > 1024.0 ? 1024 : (int)q);
+    }
+    while ((int64_t)gd[0] * gd[1] * gd[2] > budget) {
+        int k = 0;
+        if (gd[1] > gd[k]) k = 1;
+        if (gd[2] > gd[k]) k = 2;
+        const int dec = gd[k] / 8;
+        gd[k] = gd[k] - (dec < 1 ? 1 : dec);
+        if (gd[k] < 1) gd[k] = 1;
+    }
+}
+
+enum FinalizeMode { M_STEP = 0, M_REDUCE = 1, M_PRIME = 2 };
+
+// host-side launchers (md_kernels.hip)
+hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st);
+hipError_t launch_finalize(const DevState& d, int mode, int n_reps, hipStream_t st);
+hipError_t launch_drift(const DevState& d, int bs, hipStream_t st);
+hipError_t launch_allpairs(const DevState& d, int bs, bool force_only, hipStream_t st);
+hipError_t launch_insert_all(const DevState& d, int bs, hipStream_t st);
+hipError_t launch_force_eval(const DevState& d, bool fp32, int bs, hipStream_t st);
+hipError_t launch_reduce_state(const DevState& d, bool fp32, int bs, hipStream_t st);
+hipError_t launch_scale(const DevState& d, int bs, int64_t start, int64_t count, double fac,
+                        int which /*0 velocity, 1 position*/, hipStream_t st);
+hipError_t launch_wrap(const DevState& d, int bs, hipStream_t st);
+
+}  // namespace md

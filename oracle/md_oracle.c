@@ -126,6 +126,30 @@ void orc_update_force(orc_state* s, const orc_lj* table, int nthreads) {
     (void)nthreads;
 }
 
+void orc_update_force_rows(orc_state* s, const orc_lj* table, int64_t i0, int64_t i1,
+                           int nthreads) {
+    const int64_t n = s->counts[0];
+    const double* bb = s->box;
+    if (i1 > n) i1 = n;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+    for (int64_t i = i0; i < i1; ++i) {
+        const double* pi = s->pos + 3 * i;
+        double f[3] = {0.0, 0.0, 0.0}, pot = 0.0, temp = 0.0;
+        for (int64_t j = 0; j < n; ++j) {
+            if (i == j) continue;
+            pair_term(pi, s->pos + 3 * j, bb, &table[0], f, &pot, &temp);
+        }
+        s->force[3 * i] = f[0];
+        s->force[3 * i + 1] = f[1];
+        s->force[3 * i + 2] = f[2];
+        s->pot[i] = pot;
+        s->temp[i] = temp;
+    }
+    (void)nthreads;
+}
+
 static int cmp_i64(const void* a, const void* b) {
     int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
     return (x > y) - (x < y);

@@ -79,6 +79,11 @@ void orc_new_lennard_jones(double sigma, double eps, orc_lj* out);
 void orc_lj_potential_and_force(const orc_lj* p, double r, double* u, double* dudr);
 /* potential.rs:158-216; table is n_types x n_types, symmetric */
 void orc_update_force(orc_state* s, const orc_lj* table, int nthreads);
+/* Rows [i0, i1) of orc_update_force for a single-type state (the reference's
+ * per-i loop body, potential.rs:176-212): a bounded sample of the O(N^2) work
+ * for CPU timing. */
+void orc_update_force_rows(orc_state* s, const orc_lj* table, int64_t i0, int64_t i1,
+                           int nthreads);
 /* Same pair set and per-atom summation order as orc_update_force, candidates
  * found through a periodic cell grid instead of all pairs. */
 int orc_update_force_cells(orc_state* s, const orc_lj* table, int nthreads);

@@ -78,6 +78,7 @@ def lib():
         L.orc_new_lennard_jones.argtypes = [C.c_double, C.c_double, lp]
         L.orc_lj_potential_and_force.argtypes = [lp, C.c_double, dp, dp]
         L.orc_update_force.argtypes = [sp, lp, C.c_int]
+        L.orc_update_force_rows.argtypes = [sp, lp, C.c_int64, C.c_int64, C.c_int]
         L.orc_update_force_cells.argtypes = [sp, lp, C.c_int]
         L.orc_update_force_cells.restype = C.c_int
         L.orc_pair_counts.argtypes = [sp, lp, C.c_double, C.POINTER(C.c_int32), C.c_int]
@@ -193,6 +194,11 @@ def potential_table(n_types, overrides=None):
 def update_force(state, table=None, nthreads=None):
     table = table if table is not None else potential_table(state.n_types)
     lib().orc_update_force(state._sync_in(), table, nthreads or default_threads())
+
+
+def update_force_rows(state, i0, i1, table=None, nthreads=None):
+    table = table if table is not None else potential_table(state.n_types)
+    lib().orc_update_force_rows(state._sync_in(), table, i0, i1, nthreads or default_threads())
 
 
 def update_force_cells(state, table=None, nthreads=None):

@@ -1,0 +1,384 @@
+"""HIP path vs the CPU oracle, through the C ABI (needs an MI355X).
+
+Bit-exact where the arithmetic is the reference's FP64 in the reference's
+order (all-pairs and FP64-list update_force, NVE trajectories); FP64
+reduction-order tolerance where Berendsen / Nose-Hoover feedback reads global
+sums (rtol 1e-10); FP32 forces against the north-star bound (force RMSE
+<= 1e-5 of the RMS force).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.common import AR_MASS, engine_for, gas_state, liquid_state, rms, sc_state
+
+pytestmark = pytest.mark.gpu
+
+FP64_MODES = ["fp64_allpairs", "fp64_list"]
+ALL_MODES = FP64_MODES + ["fp32_list"]
+
+
+def f8(x):
+    return "%.8f" % x
+
+
+def two_body(golden, key):
+    c = golden[key]
+    return O.State([2], [c["mass"]], c["box"], pos=c["pos"], vel=c["vel"])
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_update_force_golden(golden, mode):
+    c = golden["update_force_lennard_jones"]
+    s = O.State([2], [c["mass"]], c["box"], pos=c["pos"])
+    e = engine_for(s, mode)
+    e.update_force()
+    f = e.download()["force"]
+    O.update_force(s)
+    if mode != "fp32_list":
+        assert [f8(x) for x in f[0]] == c["force_p1"]
+        assert np.array_equal(f, s.force)
+    else:
+        assert np.allclose(f, s.force, rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_verlet_with_lennard_jones(golden, mode):
+    """solver/src/lib.rs:109-262 through md_step one step at a time."""
+    c = golden["verlet_with_lennard_jones"]
+    s = two_body(golden, "verlet_with_lennard_jones")
+    e = engine_for(s, mode)
+    e.update_force()
+    O.update_force(s)
+    for k, ref in enumerate(c["steps"]):
+        if k > 0:
+            e.step(c["dt"], 1)
+            O.verlet_step(s, c["dt"])
+        d = e.download()
+        if mode != "fp32_list":
+            assert [[f8(x) for x in row] for row in d["x"]] == ref["pos"], k
+            assert [[f8(x) for x in row] for row in d["v"]] == ref["vel"], k
+            assert [[f8(x) for x in row] for row in d["force"]] == ref["force"], k
+            assert np.array_equal(d["x"], s.pos) and np.array_equal(d["v"], s.vel)
+            assert np.array_equal(d["force"], s.force)
+        else:
+            # FP32 pair forces: 1e-6 relative on F, far below the 8th decimal on x, v
+            assert np.allclose(d["force"], s.force, rtol=1e-6, atol=1e-12), k
+            assert np.allclose(d["x"], s.pos, rtol=0, atol=1e-11), k
+            assert np.allclose(d["v"], s.vel, rtol=0, atol=1e-9), k
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_verlet_lj_1000_iterations(golden, mode):
+    c = golden["verlet_lj_1000_iterations"]
+    s = two_body(golden, "verlet_lj_1000_iterations")
+    e = engine_for(s, mode)
+    e.update_force()
+    e.step(c["dt"], c["n_steps"])
+    d = e.download()
+    m = e.macro()
+    if mode == "fp32_list":
+        gx = np.array([[float(x) for x in row] for row in c["final_pos"]])
+        gv = np.array([[float(x) for x in row] for row in c["final_vel"]])
+        assert np.allclose(d["x"], gx, atol=1e-7) and np.allclose(d["v"], gv, atol=1e-7)
+        assert abs(m["pressure"] - float(c["pressure"])) < 1e-6
+        return
+    assert [[f8(x) for x in row] for row in d["x"]] == c["final_pos"]
+    assert [[f8(x) for x in row] for row in d["v"]] == c["final_vel"]
+    assert [[f8(x) for x in row] for row in d["force"]] == c["final_force"]
+    assert f8(m["kinetic"]) == c["kinetic"]
+    assert f8(m["thermal"]) == c["thermal"]
+    assert f8(m["potential"]) == c["potential"]
+    assert f8(m["temperature"] / 100.0) == c["temperature_over_100"]
+    assert f8(m["pressure"]) == c["pressure"]
+    if mode != "fp32_list":
+        tab = O.potential_table(1)
+        O.update_force(s, tab)
+        for _ in range(c["n_steps"]):
+            O.verlet_step(s, c["dt"], tab)
+        assert np.array_equal(d["x"], s.pos) and np.array_equal(d["v"], s.vel)
+
+
+def test_energies_temperature_pressure(golden):
+    c = golden["energies"]
+    s = two_body(golden, "energies")
+    e = engine_for(s, "fp64_allpairs")
+    e.update_force()
+    m = e.macro()
+    assert list(m["com_velocity"]) == c["com_velocity"]
+    assert f8(m["kinetic"]) == c["kinetic"]
+    assert f8(m["thermal"]) == c["thermal"]
+    assert f8(m["potential"]) == c["potential"]
+    assert f8(m["temperature"]) == golden["temperature"]["temperature"]
+    assert f8(m["pressure"]) == golden["pressure"]["pressure"]
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_dense_liquid_forces(mode):
+    s = liquid_state(5)
+    e = engine_for(s, mode)
+    e.update_force()
+    d = e.download()
+    O.update_force(s)
+    if mode == "fp32_list":
+        err = d["force"] - s.force
+        rel_rmse = rms(err) / rms(s.force)
+        print("fp32 force rmse/rms", rel_rmse, "max", np.abs(err).max() / rms(s.force))
+        assert rel_rmse <= 1e-5
+        assert np.allclose(d["pot"], s.pot, rtol=1e-4, atol=1e-4 * np.abs(s.pot).max())
+        assert abs(d["virial"].sum() - s.temp.sum()) <= 1e-5 * np.abs(s.temp).sum()
+    else:
+        assert np.array_equal(d["force"], s.force)
+        assert np.array_equal(d["pot"], s.pot)
+        assert np.array_equal(d["virial"], s.temp)
+
+
+def adversarial_gas(seed=7):
+    """Gas with planted pairs at r_cut exactly, one ulp either side, across
+    the periodic face and at |dx| = L / 2."""
+    sigma, eps, rc, uc = O.new_lennard_jones(0.3418, 1.712)
+    box = 6.0
+    s = gas_state(300, box, seed)
+    p = s.pos
+    p[0] = [1.0, 1.0, 1.0]
+    p[1] = [1.0 + rc, 1.0, 1.0]
+    p[2] = [3.0, 3.0, 3.0]
+    p[3] = [3.0 + np.nextafter(rc, 0), 3.0, 3.0]
+    p[4] = [2.0, 4.5, 2.0]
+    p[5] = [2.0 + np.nextafter(rc, 1), 4.5, 2.0]
+    p[6] = [0.1, 5.0, 5.0]
+    p[7] = [box - (rc - 0.1), 5.0, 5.0]
+    p[8] = [0.0, 0.5, 0.5]
+    p[9] = [box / 2, 0.5, 0.5]
+    return s
+
+
+@pytest.mark.parametrize("mode", FP64_MODES)
+def test_pair_set_exact_adversarial(mode):
+    s = adversarial_gas()
+    e = engine_for(s, mode)
+    e.update_force()
+    d = e.download()
+    O.update_force(s)
+    assert np.array_equal(d["force"], s.force)
+    assert np.array_equal(d["virial"], s.temp)
+
+
+def test_fp32_pair_set_exact_adversarial():
+    """FP32 forces, but the pair set is decided in FP64 by the reference
+    criterion: atoms with/without an interaction agree exactly."""
+    s = adversarial_gas()
+    e = engine_for(s, "fp32_list")
+    e.update_force()
+    d = e.download()
+    O.update_force(s)
+    assert np.array_equal(d["force"] != 0, s.force != 0)
+    assert np.array_equal(d["virial"] != 0, s.temp != 0)
+
+
+def run_oracle(s, n, dt, th=None, t0=0.0, ba=None, p0=0.0):
+    tab = O.potential_table(s.n_types)
+    O.update_force(s, tab)
+    for _ in range(n):
+        O.verlet_step(s, dt, tab, ba, p0, th, t0)
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_config1_npt(mode):
+    """1,000-atom argon gas, Berendsen T (tau 10, 300 K) and P (1, 5; 1.01325),
+    dt 0.002, 1,000 steps: x, v, box, lambda, mu vs the oracle."""
+    import moldyn_amd as M
+    s = sc_state(10, seed=1)
+    e = engine_for(s, mode)
+    e.update_force()
+    th = M.thermostat("berendsen", tau=10.0, target_t=300.0)
+    ba = M.barostat("berendsen", beta=1.0, tau=5.0, target_p=1.01325)
+    e.step(0.002, 1000, th, ba)
+    d = e.download()
+    oth = O.Thermostat.berendsen(10.0)
+    oba = O.Barostat.berendsen(1.0, 5.0)
+    run_oracle(s, 1000, 0.002, oth, 300.0, oba, 1.01325)
+    assert np.allclose(d["box"], s.box, rtol=1e-10, atol=0)
+    assert np.allclose(d["x"], s.pos, rtol=1e-10, atol=1e-10)
+    assert np.allclose(d["v"], s.vel, rtol=1e-10, atol=1e-12)
+    assert abs(th.lambda_ - oth.lambda_) < 1e-12
+    assert abs(ba.myu - oba.myu) < 1e-12
+    m = e.macro()
+    mv = O.center_of_mass_velocity(s, 0)
+    te = O.temperature(O.thermal_energy(s, 0, mv), 1000)
+    assert abs(m["temperature"] - te) < 1e-9 * te
+    assert abs(m["pressure"] - O.pressure(s, 0, mv)) < 1e-9 * abs(O.pressure(s, 0, mv))
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_dense_nvt_nose_hoover(mode):
+    """Liquid argon under Nose-Hoover, 100 steps: FP64 paths track the oracle
+    to reduction-order tolerance; FP32 within a trajectory bound."""
+    import moldyn_amd as M
+    s = liquid_state(4, temperature=100.0)
+    e = engine_for(s, mode)
+    e.update_force()
+    th = M.thermostat("nose_hoover", tau=0.5, target_t=90.0, psi=0.01)
+    e.step(0.002, 100, th)
+    d = e.download()
+    oth = O.Thermostat.nose_hoover(0.5, psi=0.01)
+    run_oracle(s, 100, 0.002, oth, 90.0)
+    tol = 1e-8 if mode != "fp32_list" else 1e-3
+    dx = d["x"] - s.pos
+    dx -= s.box * np.round(dx / s.box)
+    assert np.abs(dx).max() < tol
+    assert abs(th.psi - oth.psi) < tol
+    assert abs(th.lambda_ - oth.lambda_) < tol
+
+
+@pytest.mark.parametrize("mode", ["fp64_list", "fp32_list"])
+def test_list_schedule_invariance(mode):
+    """Trajectories do not depend on when lists are rebuilt, on the cell grid,
+    on capacity-overflow replays or on graph capture."""
+    s = liquid_state(4, temperature=150.0)
+    outs = []
+    for kw in [dict(), dict(skin=0.05), dict(skin=0.4, cells_per_atom=0.2),
+               dict(list_capacity=2, cell_capacity=1), dict(graph_chunk=1)]:
+        e = engine_for(s, mode, **kw)
+        e.update_force()
+        e.step(0.002, 200)
+        outs.append(e.download())
+        st = e.stats()
+        print(kw, st)
+        e.close()
+    for o in outs[1:]:
+        assert np.array_equal(o["x"], outs[0]["x"])
+        assert np.array_equal(o["v"], outs[0]["v"])
+        assert np.array_equal(o["force"], outs[0]["force"])
+
+
+def test_fp64_list_equals_allpairs_trajectory():
+    s = liquid_state(4, temperature=150.0)
+    a = engine_for(s, "fp64_allpairs")
+    b = engine_for(s, "fp64_list")
+    for e in (a, b):
+        e.update_force()
+        e.step(0.002, 300)
+    da, db = a.download(), b.download()
+    assert np.array_equal(da["x"], db["x"]) and np.array_equal(da["v"], db["v"])
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_replicas_match_single(mode):
+    """A batch of uneven replicas equals each replica run alone."""
+    import moldyn_amd as M
+    sizes = [(3, 3.0), (5, 3.6), (2, 4.1), (4, 0.6)]
+    states = []
+    for k, (n, a) in enumerate(sizes):
+        st = sc_state(n, a=a, temperature=150.0 + 40 * k, seed=10 + k, jitter=0.05, jitter_seed=20 + k)
+        states.append(st)
+    counts = [[st.n] for st in states]
+    boxes = [st.box for st in states]
+    e = M.Engine(counts, [AR_MASS], boxes, force_mode=mode)
+    e.upload(np.concatenate([st.pos for st in states]), np.concatenate([st.vel for st in states]), boxes)
+    e.update_force()
+    th = (M.engine.L.MdThermostat * 4)(*[M.thermostat("berendsen", 10.0, 100.0 + 50 * k) for k in range(4)])
+    ba = (M.engine.L.MdBarostat * 4)(*[M.barostat("berendsen", 1.0, 5.0, 0.5 + k) for k in range(4)])
+    e.step(0.002, 150, th, ba)
+    d = e.download()
+    off = 0
+    for k, st in enumerate(states):
+        one = engine_for(st, mode)
+        one.update_force()
+        th1 = M.thermostat("berendsen", 10.0, 100.0 + 50 * k)
+        ba1 = M.barostat("berendsen", 1.0, 5.0, 0.5 + k)
+        one.step(0.002, 150, th1, ba1)
+        d1 = one.download()
+        sl = slice(off, off + st.n)
+        assert np.array_equal(d["x"][sl], d1["x"]), k
+        assert np.array_equal(d["v"][sl], d1["v"]), k
+        assert np.array_equal(d["box"][k], d1["box"]), k
+        assert th[k].lambda_ == th1.lambda_ and ba[k].myu == ba1.myu
+        off += st.n
+        one.close()
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_multitype_update_force_and_nve(mode):
+    """Two types with distinct potentials: the reference's type-pair rule
+    (potential.rs:171-180: only t1 <= t2 accumulates) and per-type masses."""
+    import moldyn_amd as M
+    rng = np.random.default_rng(3)
+    n0, n1 = 150, 106
+    box = 2.4
+    pos = rng.uniform(0, box, size=(n0 + n1, 3))
+    vel = rng.normal(0, 0.2, size=(n0 + n1, 3))
+    s = O.State([n0, n1], [AR_MASS, 39.948], [box] * 3, pos=pos, vel=vel)
+    p01 = O.new_lennard_jones(0.30, 1.2)
+    p11 = O.new_lennard_jones(0.36, 0.9)
+    tab = O.potential_table(2, {(0, 1): p01, (1, 1): p11})
+    e = M.Engine([n0, n1], [AR_MASS, 39.948], [box] * 3, force_mode=mode)
+    e.set_pair_potential(0, 1, 0.30, 1.2)
+    e.set_pair_potential(1, 1, 0.36, 0.9)
+    e.upload(pos, vel, [box] * 3)
+    e.update_force()
+    d = e.download()
+    O.update_force(s, tab)
+    if mode == "fp32_list":
+        assert rms(d["force"] - s.force) <= 1e-5 * rms(s.force)
+    else:
+        assert np.array_equal(d["force"], s.force)
+        e.step(0.001, 50)
+        for _ in range(50):
+            O.verlet_step(s, 0.001, tab)
+        d = e.download()
+        assert np.array_equal(d["x"], s.pos) and np.array_equal(d["v"], s.vel)
+
+
+def test_fp32_force_rmse_gas_and_compressed():
+    """North-star bound on FP32 pair forces: RMSE <= 1e-5 of the RMS force,
+    on a 10x-density gas and on the liquid fixture."""
+    for s in (gas_state(4000, 5.0, 11), liquid_state(6)):
+        e = engine_for(s, "fp32_list")
+        e.update_force()
+        f = e.download()["force"]
+        O.update_force_cells(s)
+        assert rms(f - s.force) <= 1e-5 * rms(s.force)
+        e.close()
+
+
+def test_unimplemented_custom_variants():
+    import moldyn_amd as M
+    s = sc_state(3)
+    e = engine_for(s, "fp32_list")
+    with pytest.raises(M.MdError) as ei:
+        e.step(0.002, 1, M.thermostat("custom"))
+    assert ei.value.code == -4
+    with pytest.raises(M.MdError) as ei:
+        e.step(0.002, 1, None, M.barostat("custom"))
+    assert ei.value.code == -4
+
+
+def test_standalone_thermostat_barostat_api():
+    """md_thermostat_* / md_barostat_* mirror thermostat.rs / barostat.rs."""
+    import moldyn_amd as M
+    from moldyn_amd import _lib as L
+    import ctypes as C
+    s = liquid_state(3, temperature=130.0)
+    e = engine_for(s, "fp64_allpairs")
+    e.update_force()
+    O.update_force(s)
+    lib = L.load()
+    th = M.thermostat("berendsen", tau=0.5, target_t=100.0)
+    e._chk(lib.md_thermostat_calculate_lambda(e._h, C.byref(th), 0.002, 0))
+    oth = O.Thermostat.berendsen(0.5)
+    O.thermostat_calculate_lambda(oth, s, 0.002, 0, 100.0)
+    assert abs(th.lambda_ - oth.lambda_) < 1e-13
+    e._chk(lib.md_thermostat_update(e._h, C.byref(th), 0.002, 0))
+    O.thermostat_update(oth, s, 0.002, 0, 100.0)
+    assert np.array_equal(e.download()["v"], s.vel)
+    ba = M.barostat("berendsen", beta=1.0, tau=5.0, target_p=1.0)
+    e._chk(lib.md_barostat_calculate_myu(e._h, C.byref(ba), 0.002, 0))
+    oba = O.Barostat.berendsen(1.0, 5.0)
+    O.barostat_calculate_myu(oba, s, 0.002, 0, 1.0)
+    assert abs(ba.myu - oba.myu) < 1e-13
+    ba.myu = oba.myu
+    e._chk(lib.md_barostat_update(e._h, C.byref(ba), 0))
+    O.barostat_update(oba, s, 0)
+    d = e.download()
+    assert np.array_equal(d["x"], s.pos) and np.array_equal(d["box"], s.box)
