@@ -36,6 +36,8 @@ enum : uint32_t {
     F_HAVE_LIST = 1u << 5,  // lists exist (built at some fill)
     F_GRID_FRESH = 1u << 6, // grid `fill_epoch` holds the current input positions
     F_BAD = 1u << 7,        // non-finite T/P or mu^3 <= 0 seen
+    F_SPLIT_A = 1u << 8,    // drift exceeds the skin: kick + drift + grid fill only
+    F_SPLIT_B = 1u << 9,    // ... then rebuild at the drifted positions, force, kick
 };
 
 // per-block partial / per-group sum columns
@@ -99,7 +101,7 @@ struct RepCtrl {
     uint32_t fill_epoch;       // epoch of the last completed fill
     int32_t cur;               // state buffer holding the input
     int64_t step;              // completed real steps
-    int64_t rebuilds, nulls, launches;
+    int64_t rebuilds, nulls, launches, splits;
 };
 
 struct RepStats {
@@ -135,7 +137,7 @@ struct DevState {
     const RepParams* params;
     RepStats* stats;
     const PairParam* pairs;  // n_types x n_types
-    double* partial;         // n_blocks x NS
+    double* partial;         // NS x n_blocks (column-major)
     double* sums;            // n_groups x NS
     double mass_t[kMaxTypes];
     int32_t n_types;

@@ -59,9 +59,10 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // Block reduction of NS columns in a fixed order (wave butterfly, then waves
-// in index order); columns [k0, k1) are stored to out.
+// in index order); columns [k0, k1) are stored to out[k * stride].
 template <int BS>
-__device__ __forceinline__ void block_store(const double (&v)[NS], double* out, int k0, int k1) {
+__device__ __forceinline__ void block_store(const double (&v)[NS], double* out, int64_t stride,
+                                            int k0, int k1) {
     constexpr int NW = BS / 64;
     __shared__ double red[NW * NS];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -77,7 +78,7 @@ __device__ __forceinline__ void block_store(const double (&v)[NS], double* out, 
         double acc = red[k];
         for (int i = 1; i < NW; ++i)
             acc = k >= kFirstMax ? fmax(acc, red[i * NS + k]) : acc + red[i * NS + k];
-        out[k] = acc;
+        out[k * stride] = acc;
     }
     __syncthreads();
 }
@@ -264,19 +265,12 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
     const int64_t s = (int64_t)b * BS + threadIdx.x;
     const bool valid = (s - G.start) < G.count;
     const int cur = C.cur, nx = cur ^ 1;
-    const double* xs = d.x[cur];
-    const double* ys = d.y[cur];
-    const double* zs = d.z[cur];
+    double* const part_out = d.partial + b;
+    const int64_t pstride = d.n_blocks;
     if (flags & F_NULL) {
-        if (valid) grid_insert(d, C, R, s, xs[s], ys[s], zs[s]);
+        if (valid) grid_insert(d, C, R, s, d.x[cur][s], d.y[cur][s], d.z[cur][s]);
         return;
     }
-    const double* vxs = d.vx[cur];
-    const double* vys = d.vy[cur];
-    const double* vzs = d.vz[cur];
-    const FT* fxs = (const FT*)d.fx[cur];
-    const FT* fys = (const FT*)d.fy[cur];
-    const FT* fzs = (const FT*)d.fz[cur];
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double dt = P.dt, lam = C.lambda, mu = C.myu;
     const double m = G.mass;
@@ -285,24 +279,80 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
     double part[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
+    if (flags & F_SPLIT_A) {
+        // first half of a split step: kick, scale, drift, wrap into cur ^ 1 and
+        // fill the grid with the drifted positions (the drift exceeds the skin)
+        if (valid) {
+            const double vkx = d.vx[cur][s] + (double)((const FT*)d.fx[cur])[s] * c;
+            const double vky = d.vy[cur][s] + (double)((const FT*)d.fy[cur])[s] * c;
+            const double vkz = d.vz[cur][s] + (double)((const FT*)d.fz[cur])[s] * c;
+            part[S_KVX] = vkx * m;
+            part[S_KVY] = vky * m;
+            part[S_KVZ] = vkz * m;
+            part[S_KV2] = m * ((vkx * vkx + vky * vky) + vkz * vkz);
+            const double v1x = vkx * lam, v1y = vky * lam, v1z = vkz * lam;
+            const double x1 = wrap1(d.x[cur][s] + v1x * dt, bx);
+            const double y1 = wrap1(d.y[cur][s] + v1y * dt, by);
+            const double z1 = wrap1(d.z[cur][s] + v1z * dt, bz);
+            d.x[nx][s] = x1;
+            d.y[nx][s] = y1;
+            d.z[nx][s] = z1;
+            d.vx[nx][s] = v1x;
+            d.vy[nx][s] = v1y;
+            d.vz[nx][s] = v1z;
+            part[S_VMAX] = sqrt((v1x * v1x + v1y * v1y) + v1z * v1z);
+            grid_insert(d, C, R, s, x1, y1, z1);
+        }
+        block_store<BS>(part, part_out, pstride, S_KVX, S_KV2 + 1);
+        block_store<BS>(part, part_out, pstride, S_VMAX, NS);
+        return;
+    }
+    // second half of a split step: drifted state in cur ^ 1, results to cur
+    const bool split = (flags & F_SPLIT_B) != 0;
+    const int src = split ? nx : cur;
+    const int dst = split ? cur : nx;
+    const double* xs = d.x[src];
+    const double* ys = d.y[src];
+    const double* zs = d.z[src];
+    const double* vxs = d.vx[src];
+    const double* vys = d.vy[src];
+    const double* vzs = d.vz[src];
+    const FT* fxs = (const FT*)d.fx[cur];
+    const FT* fys = (const FT*)d.fy[cur];
+    const FT* fzs = (const FT*)d.fz[cur];
     if (valid) {
-        const double x = xs[s], y = ys[s], z = zs[s];
-        if (flags & F_REBUILD) build_list<MULTI>(d, C, P, R, s, ti, xs, ys, zs, x, y, z);
-        // integrator.rs:28-34 first half kick
-        const double vkx = vxs[s] + (double)fxs[s] * c;
-        const double vky = vys[s] + (double)fys[s] * c;
-        const double vkz = vzs[s] + (double)fzs[s] * c;
-        part[S_KVX] = vkx * m;
-        part[S_KVY] = vky * m;
-        part[S_KVZ] = vkz * m;
-        part[S_KV2] = m * ((vkx * vkx + vky * vky) + vkz * vkz);
-        // thermostat.rs:55-57 scale, integrator.rs:40-45 drift + wrap
-        const double v1x = vkx * lam, v1y = vky * lam, v1z = vkz * lam;
-        const double x1 = wrap1(x + v1x * dt, bx);
-        const double y1 = wrap1(y + v1y * dt, by);
-        const double z1 = wrap1(z + v1z * dt, bz);
-        part[S_VMAX] = sqrt((v1x * v1x + v1y * v1y) + v1z * v1z);
-        // potential.rs:158-216 over the Verlet list at the drifted positions
+        double x1, y1, z1, v1x, v1y, v1z;
+        if (split) {
+            x1 = xs[s];
+            y1 = ys[s];
+            z1 = zs[s];
+            v1x = vxs[s];
+            v1y = vys[s];
+            v1z = vzs[s];
+            build_list<MULTI>(d, C, P, R, s, ti, xs, ys, zs, x1, y1, z1);
+        } else {
+            const double x = xs[s], y = ys[s], z = zs[s];
+            if (flags & F_REBUILD) build_list<MULTI>(d, C, P, R, s, ti, xs, ys, zs, x, y, z);
+            // integrator.rs:28-34 first half kick
+            const double vkx = vxs[s] + (double)fxs[s] * c;
+            const double vky = vys[s] + (double)fys[s] * c;
+            const double vkz = vzs[s] + (double)fzs[s] * c;
+            part[S_KVX] = vkx * m;
+            part[S_KVY] = vky * m;
+            part[S_KVZ] = vkz * m;
+            part[S_KV2] = m * ((vkx * vkx + vky * vky) + vkz * vkz);
+            // thermostat.rs:55-57 scale, integrator.rs:40-45 drift + wrap
+            v1x = vkx * lam;
+            v1y = vky * lam;
+            v1z = vkz * lam;
+            x1 = wrap1(x + v1x * dt, bx);
+            y1 = wrap1(y + v1y * dt, by);
+            z1 = wrap1(z + v1z * dt, bz);
+            part[S_VMAX] = sqrt((v1x * v1x + v1y * v1y) + v1z * v1z);
+        }
+        // potential.rs:158-216 over the Verlet list at the drifted positions;
+        // a neighbour's drift is recomputed from the input buffer with the
+        // same operations its own thread performs, so it is bitwise the same
         Acc<FT> acc;
         const int nn = d.ncount[s];
         const PairParam p0 = d.pairs[0];
@@ -314,9 +364,16 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
                 tj = d.slot_type[j];
                 cj = dt / (2.0 * d.mass_t[tj]);
             }
-            const double xj = wrap1(xs[j] + ((vxs[j] + (double)fxs[j] * cj) * lam) * dt, bx);
-            const double yj = wrap1(ys[j] + ((vys[j] + (double)fys[j] * cj) * lam) * dt, by);
-            const double zj = wrap1(zs[j] + ((vzs[j] + (double)fzs[j] * cj) * lam) * dt, bz);
+            double xj, yj, zj;
+            if (split) {
+                xj = xs[j];
+                yj = ys[j];
+                zj = zs[j];
+            } else {
+                xj = wrap1(xs[j] + ((vxs[j] + (double)fxs[j] * cj) * lam) * dt, bx);
+                yj = wrap1(ys[j] + ((vys[j] + (double)fys[j] * cj) * lam) * dt, by);
+                zj = wrap1(zs[j] + ((vzs[j] + (double)fzs[j] * cj) * lam) * dt, bz);
+            }
             const double dx = min_image(xj - x1, bx);
             const double dy = min_image(yj - y1, by);
             const double dz = min_image(zj - z1, bz);
@@ -329,15 +386,15 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
         const double v2y = v1y + (double)acc.fy * c;
         const double v2z = v1z + (double)acc.fz * c;
         const double x2 = x1 * mu, y2 = y1 * mu, z2 = z1 * mu;
-        d.x[nx][s] = x2;
-        d.y[nx][s] = y2;
-        d.z[nx][s] = z2;
-        d.vx[nx][s] = v2x;
-        d.vy[nx][s] = v2y;
-        d.vz[nx][s] = v2z;
-        ((FT*)d.fx[nx])[s] = acc.fx;
-        ((FT*)d.fy[nx])[s] = acc.fy;
-        ((FT*)d.fz[nx])[s] = acc.fz;
+        d.x[dst][s] = x2;
+        d.y[dst][s] = y2;
+        d.z[dst][s] = z2;
+        d.vx[dst][s] = v2x;
+        d.vy[dst][s] = v2y;
+        d.vz[dst][s] = v2z;
+        ((FT*)d.fx[dst])[s] = acc.fx;
+        ((FT*)d.fy[dst])[s] = acc.fy;
+        ((FT*)d.fz[dst])[s] = acc.fz;
         if (flags & F_WRITE_PV) {
             d.pot[s] = (double)acc.pot;
             d.vir[s] = (double)acc.vir;
@@ -353,7 +410,13 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
                           (double)acc.fz * (double)acc.fz) / m;
         if (flags & F_INSERT) grid_insert(d, C, R, s, x2, y2, z2);
     }
-    block_store<BS>(part, d.partial + (int64_t)b * NS, 0, NS);
+    if (split) {
+        // the kick / drift columns were stored by the first half
+        block_store<BS>(part, part_out, pstride, S_MVX, S_VIR + 1);
+        block_store<BS>(part, part_out, pstride, S_WV, S_WA + 1);
+    } else {
+        block_store<BS>(part, part_out, pstride, 0, NS);
+    }
 }
 
 // Force evaluation at the current positions through a freshly built list
@@ -409,7 +472,7 @@ __global__ __launch_bounds__(BS) void k_force_eval(DevState d) {
         part[S_WA] = sqrt(((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
                           (double)acc.fz * (double)acc.fz) / m;
     }
-    block_store<BS>(part, d.partial + (int64_t)b * NS, 0, NS);
+    block_store<BS>(part, d.partial + b, d.n_blocks, 0, NS);
 }
 
 template <int BS>
@@ -450,7 +513,7 @@ __global__ __launch_bounds__(BS) void k_reduce_state(DevState d) {
         part[S_WV] = sqrt((vx * vx + vy * vy) + vz * vz);
         part[S_WA] = sqrt((fx * fx + fy * fy) + fz * fz) / m;
     }
-    block_store<BS>(part, d.partial + (int64_t)b * NS, 0, NS);
+    block_store<BS>(part, d.partial + b, d.n_blocks, 0, NS);
 }
 
 // ---------------------------------------------------------------- O(N^2) FP64 path
@@ -491,8 +554,8 @@ __global__ __launch_bounds__(BS) void k_drift(DevState d) {
         d.z[nx][s] = wrap1(d.z[cur][s] + v1z * dt, C.box[2]);
         part[S_VMAX] = sqrt((v1x * v1x + v1y * v1y) + v1z * v1z);
     }
-    block_store<BS>(part, d.partial + (int64_t)b * NS, S_KVX, S_KV2 + 1);
-    block_store<BS>(part, d.partial + (int64_t)b * NS, S_VMAX, NS);
+    block_store<BS>(part, d.partial + b, d.n_blocks, S_KVX, S_KV2 + 1);
+    block_store<BS>(part, d.partial + b, d.n_blocks, S_VMAX, NS);
 }
 
 // update_force by all pairs: thread per i, j tiles staged through LDS, j
@@ -587,10 +650,10 @@ __global__ __launch_bounds__(BS) void k_allpairs(DevState d) {
         part[S_WA] = sqrt((fx * fx + fy * fy) + fz * fz) / m;
     }
     if (FORCE_ONLY) {
-        block_store<BS>(part, d.partial + (int64_t)b * NS, 0, NS);
+        block_store<BS>(part, d.partial + b, d.n_blocks, 0, NS);
     } else {
-        block_store<BS>(part, d.partial + (int64_t)b * NS, 0, S_KVX);
-        block_store<BS>(part, d.partial + (int64_t)b * NS, S_WV, S_WA + 1);
+        block_store<BS>(part, d.partial + b, d.n_blocks, 0, S_KVX);
+        block_store<BS>(part, d.partial + b, d.n_blocks, S_WV, S_WA + 1);
     }
 }
 
@@ -620,7 +683,7 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
     const int T = R.n_types;
     const double dt = P.dt;
     const bool baro = P.ba_kind == 1;
-    bool controls = true;
+    bool controls = true, decide = true;
     if (mode == M_STEP) {
         C.launches++;
         const uint32_t f = C.flags;
@@ -630,6 +693,13 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
             C.flags = (f & F_HAVE_LIST) | F_GRID_FRESH;
             C.nulls++;
             controls = false;
+        } else if (f & F_SPLIT_A) {
+            // drifted state and its grid are ready; the second half follows
+            C.fill_epoch = C.ins_epoch;
+            C.flags = F_HAVE_LIST | F_SPLIT_B;
+            C.splits++;
+            controls = false;
+            decide = false;
         } else {
             C.step++;
             C.lambda_used = C.lambda;
@@ -654,22 +724,26 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
             }
             uint32_t nf = f & F_HAVE_LIST;
             if (P.list_mode) {
-                if (f & F_REBUILD) {
+                if (f & (F_REBUILD | F_SPLIT_B)) {
                     C.D = 0.0;
                     C.s = 1.0;
                     C.rebuilds++;
                     nf |= F_HAVE_LIST;
                 }
-                double vmax = 0.0;
-                for (int t = 0; t < T; ++t) vmax = fmax(vmax, S[t * NS + S_VMAX]);
-                C.D += vmax * dt / C.s;
+                if (!(f & F_SPLIT_B)) {
+                    // the list of a split step was built after this step's drift
+                    double vmax = 0.0;
+                    for (int t = 0; t < T; ++t) vmax = fmax(vmax, S[t * NS + S_VMAX]);
+                    C.D += vmax * dt / C.s;
+                }
                 if (baro) C.s *= C.myu;
             }
             if (f & F_INSERT) {
                 C.fill_epoch = C.ins_epoch;
                 nf |= F_GRID_FRESH;
             }
-            if (P.list_mode) C.cur ^= 1;  // k_step wrote buffer cur ^ 1
+            // k_step wrote buffer cur ^ 1 (a split step ends in cur)
+            if (P.list_mode && !(f & F_SPLIT_B)) C.cur ^= 1;
             C.flags = nf;
         }
     } else {
@@ -704,7 +778,7 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
         C.myu = mu;
         if (bad) C.flags |= F_BAD;
     }
-    if (P.list_mode) {
+    if (P.list_mode && decide) {
         // |v + F dt / 2m| <= |v| + (dt / 2) |F| / m for every particle
         double wmax = 0.0;
         for (int t = 0; t < T; ++t)
@@ -713,7 +787,15 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
         const double thr = P.rlist - P.rc_max / C.s - P.margin;
         bool need = !(C.flags & F_HAVE_LIST) || P.rebuild_every_step || !(2.0 * (C.D + w) < thr);
         if (need) {
-            if (C.flags & F_GRID_FRESH) {
+            const double w0 = C.lambda * wmax * dt;
+            if (!(2.0 * w0 < P.rlist - P.rc_max - P.margin)) {
+                // one drift exceeds what any list built before it can cover
+                C.flags |= F_SPLIT_A;
+                C.ins_epoch = C.fill_epoch + 1u;
+                const int gbf = C.ins_epoch & 1;
+                for (int k = 0; k < 3; ++k) C.gbox[gbf][k] = C.box[k];
+                grid_dims(C.gbox[gbf], P.rlist, R.cell_budget, C.gdim[gbf]);
+            } else if (C.flags & F_GRID_FRESH) {
                 C.flags |= F_REBUILD;
             } else {
                 C.flags |= F_NULL;
@@ -723,7 +805,7 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
                 grid_dims(C.gbox[gbf], P.rlist, R.cell_budget, C.gdim[gbf]);
             }
         }
-        if (!(C.flags & F_NULL)) {
+        if (!(C.flags & (F_NULL | F_SPLIT_A))) {
             const bool rb = (C.flags & F_REBUILD) != 0;
             const double d1 = rb ? 0.0 : C.D;
             const double s1 = rb ? 1.0 : C.s;
@@ -758,21 +840,33 @@ __global__ __launch_bounds__(BS) void k_finalize(DevState d, int mode) {
     RepCtrl* Cp = d.ctrl + r;
     const uint32_t f0 = Cp->flags;
     if (mode == M_STEP && (f0 & F_EXIT)) return;
-    const bool reduce = mode == M_REDUCE || (mode == M_STEP && !(f0 & F_NULL));
+    const bool reduce = mode == M_REDUCE || (mode == M_STEP && !(f0 & (F_NULL | F_SPLIT_A)));
     if (reduce) {
+        const int64_t nb = d.n_blocks;
         for (int t = 0; t < R.n_types; ++t) {
             const int g = R.first_group + t;
             const GroupInfo G = d.groups[g];
             double acc[NS];
 #pragma unroll
             for (int k = 0; k < NS; ++k) acc[k] = 0.0;
-            for (int bb = threadIdx.x; bb < G.nblocks; bb += BS) {
-                const double* row = d.partial + (int64_t)(G.first_block + bb) * NS;
+            // column-major partials: consecutive threads read consecutive blocks;
+            // four rows per thread are loaded before any is summed
+            for (int b0 = threadIdx.x; b0 < G.nblocks; b0 += 4 * BS) {
+                double v[4][NS];
 #pragma unroll
-                for (int k = 0; k < NS; ++k)
-                    acc[k] = k >= kFirstMax ? fmax(acc[k], row[k]) : acc[k] + row[k];
+                for (int u = 0; u < 4; ++u) {
+                    const int bb = b0 + u * BS;
+#pragma unroll
+                    for (int k = 0; k < NS; ++k)
+                        v[u][k] = bb < G.nblocks ? d.partial[k * nb + G.first_block + bb] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int k = 0; k < NS; ++k)
+                        acc[k] = k >= kFirstMax ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
             }
-            block_store<BS>(acc, d.sums + (int64_t)g * NS, 0, NS);
+            block_store<BS>(acc, d.sums + (int64_t)g * NS, 1, 0, NS);
         }
     }
     if (mode == M_REDUCE) return;
@@ -859,7 +953,7 @@ hipError_t launch_force_eval(const DevState& d, bool fp32, int bs, hipStream_t s
 }
 
 hipError_t launch_finalize(const DevState& d, int mode, int n_reps, hipStream_t st) {
-    hipLaunchKernelGGL((k_finalize<256>), dim3(n_reps), dim3(256), 0, st, d, mode);
+    hipLaunchKernelGGL((k_finalize<512>), dim3(n_reps), dim3(512), 0, st, d, mode);
     return hipGetLastError();
 }
 
