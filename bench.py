@@ -59,6 +59,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--skin", type=float, default=None)
     ap.add_argument("--block-size", type=int, default=256)
+    ap.add_argument("--cells-per-atom", type=float, default=2.0)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--tune-steps", type=int, default=200)
     ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "step_kernel_traffic.json"))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -70,7 +73,14 @@ def main():
 
     n, pos, vel, box = initial_state(M, args.n_side)
     log(f"[bench] state ready: {n} atoms, box {box[0]:.4f}")
-    e = M.Engine([n], [MASS], box, force_mode="fp32_list", skin=args.skin, block_size=args.block_size)
+    opts = {"skin": args.skin if args.skin is not None else 0.3 * 0.3418,
+            "block_size": args.block_size, "cells_per_atom": args.cells_per_atom}
+    if not args.no_tune:
+        from moldyn_amd.autotune import autotune
+        opts, _ = autotune([n], [MASS], box, pos, vel, DT, lambda: controls(M),
+                           steps=args.tune_steps, warmup=20, log=log)
+        log(f"[bench] autotuned options: {opts}")
+    e = M.Engine([n], [MASS], box, force_mode="fp32_list", **opts)
     e.upload(pos, vel, box)
     e.update_force()
     th, ba = controls(M)
@@ -105,7 +115,7 @@ def main():
             traffic = json.load(f).get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_step<float,256>", "avg_us": round(avg_step_s * 1e6, 3),
+                "kernel": f"k_step<float,{opts['block_size']}>", "avg_us": round(avg_step_s * 1e6, 3),
                 "algorithmic_bytes_per_launch": round(bytes_per_launch)}
 
     # FP32 force error on the final state, against the FP64 path (bitwise the
@@ -176,7 +186,8 @@ def main():
         "config": {"workload": f"argon SC {args.n_side}^3 ({n} atoms) LJ NPT, Berendsen T 300 K "
                                f"tau 10, Berendsen P 1.01325 (1, 5), dt {DT}, Verlet list",
                    "n_atoms": n, "force_mode": "fp32_list",
-                   "skin_nm": e_skin(args), "block_size": args.block_size,
+                   "skin_nm": opts["skin"], "block_size": opts["block_size"],
+                   "cells_per_atom": opts["cells_per_atom"], "autotuned": not args.no_tune,
                    "rebuilds": st["rebuilds"], "mean_list_len": n_l},
         "roofline": roofline,
         "cpu_baseline": cpu,
@@ -184,10 +195,6 @@ def main():
         "force_error_vs_fp64": force_err,
     }
     print(json.dumps(out), flush=True)
-
-
-def e_skin(args):
-    return args.skin if args.skin is not None else 0.3 * 0.3418
 
 
 if __name__ == "__main__":

@@ -1,0 +1,58 @@
+"""Small autotuner for the list path's launch and list parameters.
+
+Results of the list paths do not depend on these parameters (lists are sorted
+and pairs outside r_cut contribute nothing; tests/test_gpu_parity.py checks
+it), so the tuner only measures speed: it runs short trajectories from the
+caller's state and keeps the fastest setting, one parameter at a time
+(skin, then block size, then cell-grid budget).
+"""
+from __future__ import annotations
+
+import time
+
+from .engine import Engine
+
+SKINS = (0.1, 0.2, 0.35, 0.5, 0.7, 1.0)
+BLOCKS = (128, 256, 512)
+CELLS = (0.5, 1.0, 2.0, 4.0)
+
+
+def _trial(counts, masses, box, x, v, dt, controls, steps, warmup, force_mode, opts):
+    e = Engine(counts, masses, box, force_mode=force_mode, **opts)
+    try:
+        e.upload(x, v, box)
+        e.update_force()
+        th, ba = controls()
+        e.step(dt, warmup, th, ba)
+        t0 = time.perf_counter()
+        e.step(dt, steps, th, ba)
+        t = (time.perf_counter() - t0) / steps
+        return t, e.stats()
+    finally:
+        e.close()
+
+
+def autotune(counts, masses, box, x, v, dt, controls, steps=300, warmup=20,
+             force_mode="fp32_list", skins=SKINS, blocks=BLOCKS, cells=CELLS, log=None):
+    """controls() -> (thermostat, barostat) as passed to Engine.step (fresh objects).
+
+    Returns (best_options, table) where table lists (options, seconds_per_step, stats)."""
+    best = {"skin": 0.3 * 0.3418, "block_size": 256, "cells_per_atom": 2.0}
+    table = []
+
+    def run(opts):
+        t, st = _trial(counts, masses, box, x, v, dt, controls, steps, warmup, force_mode, opts)
+        table.append((dict(opts), t, st))
+        if log:
+            log(f"[autotune] {opts} -> {t * 1e6:.2f} us/step (rebuilds {st['rebuilds']}, "
+                f"<n_l> {st['mean_list_len']:.3f})")
+        return t
+
+    for key, values in (("skin", skins), ("block_size", blocks), ("cells_per_atom", cells)):
+        scores = {}
+        for val in values:
+            opts = dict(best)
+            opts[key] = val
+            scores[val] = run(opts)
+        best[key] = min(scores, key=scores.get)
+    return best, table

@@ -360,6 +360,17 @@ static int alloc_base(md_ctx* c) {
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * (size_t)std::max(c->n_blocks, 1))))
         return rc;
     d.partial = (double*)p;
+    {
+        int maxnb = 1;
+        for (const GroupInfo& G : c->groups) maxnb = std::max(maxnb, (int)G.nblocks);
+        d.fin_slices = std::min(64, std::max(1, (maxnb + 255) / 256));
+    }
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * c->groups.size() * d.fin_slices)))
+        return rc;
+    d.part2 = (double*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(unsigned int) * c->R))) return rc;
+    d.fin_count = (unsigned int*)p;
+    HIPCHK(c, hipMemset(p, 0, sizeof(unsigned int) * c->R));
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * c->groups.size()))) return rc;
     d.sums = (double*)p;
     HIPCHK(c, hipMemset(p, 0, sizeof(double) * NS * c->groups.size()));
@@ -390,9 +401,17 @@ static int write_ctrl(md_ctx* c, const std::vector<RepCtrl>& in) {
     return MD_OK;
 }
 
+static void set_kick(md_ctx* c, RepParams& P, double dt) {
+    P.dt = dt;
+    for (int t = 0; t < c->T; ++t) {
+        P.cm[t] = dt / (2.0 * c->masses[t]);  // integrator.rs:29 temp = delta_time / (2.0 * mass)
+        P.inv_m2[t] = 1.0 / (c->masses[t] * c->masses[t]);
+    }
+}
+
 static RepParams base_params(md_ctx* c) {
     RepParams P{};
-    P.dt = c->last_dt;
+    set_kick(c, P, c->last_dt);
     P.list_mode = c->list_mode ? 1 : 0;
     P.rlist = c->rlist;
     P.rc_max = c->rc_max;
@@ -414,7 +433,7 @@ static int ensure_sums(md_ctx* c) {
     int rc;
     if ((rc = write_params(c, P))) return rc;
     if (c->n_blocks > 0) HIPCHK(c, launch_reduce_state(c->d, c->fp32, c->bs, c->st));
-    HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->st));
+    HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->d.fin_slices, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->sums_valid = true;
     return MD_OK;
@@ -480,7 +499,7 @@ static hipError_t launch_once(md_ctx* c) {
         if ((e = ev_end(c, ev))) return e;
     }
     if ((e = ev_begin(c, K_FINALIZE, &ev))) return e;
-    if ((e = launch_finalize(c->d, M_STEP, c->R, c->st))) return e;
+    if ((e = launch_finalize(c->d, M_STEP, c->R, c->d.fin_slices, c->st))) return e;
     return ev_end(c, ev);
 }
 
@@ -581,7 +600,7 @@ static int list_force_eval(md_ctx* c) {
             HIPCHK(c, launch_insert_all(c->d, c->bs, c->st));
             HIPCHK(c, launch_force_eval(c->d, c->fp32, c->bs, c->st));
         }
-        HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->st));
+        HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->d.fin_slices, c->st));
         RepStats s;
         if ((rc = read_stats(c, s))) return rc;
         int grown = 0;
@@ -629,7 +648,7 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
     std::vector<RepParams> P((size_t)c->R, base_params(c));
     for (int r = 0; r < c->R; ++r) {
         RepParams& p = P[r];
-        p.dt = dt;
+        set_kick(c, p, dt);
         const md_thermostat* t = th ? &th[per_rep ? r : 0] : nullptr;
         const md_barostat* b = ba ? &ba[per_rep ? r : 0] : nullptr;
         if (t && t->kind > 0) {
@@ -667,7 +686,7 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
         }
         if ((rc = write_params(c, P))) return rc;
         if ((rc = reset_stats(c))) return rc;
-        HIPCHK(c, launch_finalize(c->d, M_PRIME, c->R, c->st));
+        HIPCHK(c, launch_finalize(c->d, M_PRIME, c->R, 1, c->st));
         int64_t k = 0;
         if (!c->prof && K > 1 && cn >= K) {
             if ((rc = ensure_graph(c, K))) return rc;
@@ -1114,7 +1133,7 @@ int md_update_force(md_ctx* c) {
     int rc = write_params(c, P);
     if (rc) return rc;
     if (c->n_blocks > 0) HIPCHK(c, launch_allpairs(c->d, c->bs, true, c->st));
-    HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->st));
+    HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->d.fin_slices, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->sums_valid = true;
     return MD_OK;

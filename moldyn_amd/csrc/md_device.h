@@ -48,9 +48,9 @@ enum {
     S_VIR,                // sum virial (Particle::temp)
     S_KVX, S_KVY, S_KVZ,  // sum m v after the first half kick (Nose-Hoover)
     S_KV2,                // sum m |v|^2 after the first half kick
-    S_WV,                 // max |v| of the state after the step
-    S_WA,                 // max |F| / m of the state after the step
-    S_VMAX,               // max |v| of this step's drift
+    S_WV,                 // max |v|^2 of the state after the step
+    S_WA,                 // max |F|^2 / m^2 of the state after the step
+    S_VMAX,               // max |v|^2 of this step's drift
     NS
 };
 constexpr int kFirstMax = S_WV;
@@ -87,6 +87,8 @@ struct RepParams {
     double rlist, rc_max, margin, insert_growth;
     double psi_in;        // Nose-Hoover psi handed in by the caller
     int32_t use_psi_in;   // M_PRIME starts from psi_in instead of psi_used
+    double cm[kMaxTypes];     // dt / (2 m_t), the half-kick factor (integrator.rs:29)
+    double inv_m2[kMaxTypes]; // 1 / m_t^2 (drift-bound bookkeeping only)
 };
 
 struct RepCtrl {
@@ -138,10 +140,13 @@ struct DevState {
     RepStats* stats;
     const PairParam* pairs;  // n_types x n_types
     double* partial;         // NS x n_blocks (column-major)
+    double* part2;           // n_groups x fin_slices x NS, finalize slice sums
+    unsigned int* fin_count; // per replica, finalize slices done (last one resets)
     double* sums;            // n_groups x NS
     double mass_t[kMaxTypes];
     int32_t n_types;
     int32_t n_blocks;
+    int32_t fin_slices;
     int64_t n_slots;
 };
 
@@ -167,7 +172,14 @@ enum FinalizeMode { M_STEP = 0, M_REDUCE = 1, M_PRIME = 2 };
 
 // host-side launchers (md_kernels.hip)
 hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st);
-hipError_t launch_finalize(const DevState& d, int mode, int n_reps, hipStream_t st);
+hipError_t launch_finalize(const DevState& d, int mode, int n_reps, int slices, hipStream_t st);
+
+// column masks for block partial stores
+constexpr uint32_t kColsKick = (1u << S_KVX) | (1u << S_KVY) | (1u << S_KVZ) | (1u << S_KV2);
+constexpr uint32_t kColsState = (1u << S_MVX) | (1u << S_MVY) | (1u << S_MVZ) | (1u << S_MV2) |
+                                (1u << S_POT) | (1u << S_VIR) | (1u << S_WV) | (1u << S_WA);
+constexpr uint32_t kColsDrift = 1u << S_VMAX;
+constexpr uint32_t kColsAll = (1u << NS) - 1u;
 hipError_t launch_drift(const DevState& d, int bs, hipStream_t st);
 hipError_t launch_allpairs(const DevState& d, int bs, bool force_only, hipStream_t st);
 hipError_t launch_insert_all(const DevState& d, int bs, hipStream_t st);

@@ -59,22 +59,22 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 // Block reduction of NS columns in a fixed order (wave butterfly, then waves
-// in index order); columns [k0, k1) are stored to out[k * stride].
+// in index order); the columns set in mask are stored to out[k * stride].
 template <int BS>
 __device__ __forceinline__ void block_store(const double (&v)[NS], double* out, int64_t stride,
-                                            int k0, int k1) {
+                                            uint32_t mask) {
     constexpr int NW = BS / 64;
     __shared__ double red[NW * NS];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        if (k < k0 || k >= k1) continue;
+        if (!((mask >> k) & 1u)) continue;
         const double t = k >= kFirstMax ? wave_max(v[k]) : wave_sum(v[k]);
         if (lane == 0) red[w * NS + k] = t;
     }
     __syncthreads();
     const int k = threadIdx.x;
-    if (k >= k0 && k < k1) {
+    if (k < NS && ((mask >> k) & 1u)) {
         double acc = red[k];
         for (int i = 1; i < NW; ++i)
             acc = k >= kFirstMax ? fmax(acc, red[i * NS + k]) : acc + red[i * NS + k];
@@ -203,9 +203,9 @@ __device__ void build_list(const DevState& d, const RepCtrl& C, const RepParams&
     const int cap = d.list_cap;
     const int cc = d.cell_cap;
     const int64_t N = d.n_slots;
-    int32_t* col = d.nbr + slot;
-    const unsigned long long* hdr = d.cell_hdr[buf];
-    const int32_t* cs = d.cell_slots[buf];
+    int32_t* __restrict__ col = d.nbr + slot;
+    const unsigned long long* __restrict__ hdr = d.cell_hdr[buf];
+    const int32_t* __restrict__ cs = d.cell_slots[buf];
     int n = 0;
     const int lx = gd[0] >= 3 ? -1 : 0, hx = gd[0] >= 2 ? 1 : 0;
     const int ly = gd[1] >= 3 ? -1 : 0, hy = gd[1] >= 2 ? 1 : 0;
@@ -274,8 +274,9 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double dt = P.dt, lam = C.lambda, mu = C.myu;
     const double m = G.mass;
-    const double c = dt / (2.0 * m);
     const int ti = G.type;
+    const double c = P.cm[ti];
+    const uint32_t kick_cols = P.th_kind == 2 ? kColsKick : 0u;
     double part[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
@@ -300,11 +301,10 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
             d.vx[nx][s] = v1x;
             d.vy[nx][s] = v1y;
             d.vz[nx][s] = v1z;
-            part[S_VMAX] = sqrt((v1x * v1x + v1y * v1y) + v1z * v1z);
+            part[S_VMAX] = (v1x * v1x + v1y * v1y) + v1z * v1z;
             grid_insert(d, C, R, s, x1, y1, z1);
         }
-        block_store<BS>(part, part_out, pstride, S_KVX, S_KV2 + 1);
-        block_store<BS>(part, part_out, pstride, S_VMAX, NS);
+        block_store<BS>(part, part_out, pstride, kick_cols | kColsDrift);
         return;
     }
     // second half of a split step: drifted state in cur ^ 1, results to cur
@@ -348,7 +348,7 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
             x1 = wrap1(x + v1x * dt, bx);
             y1 = wrap1(y + v1y * dt, by);
             z1 = wrap1(z + v1z * dt, bz);
-            part[S_VMAX] = sqrt((v1x * v1x + v1y * v1y) + v1z * v1z);
+            part[S_VMAX] = (v1x * v1x + v1y * v1y) + v1z * v1z;
         }
         // potential.rs:158-216 over the Verlet list at the drifted positions;
         // a neighbour's drift is recomputed from the input buffer with the
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
             double cj = c;
             if (MULTI) {
                 tj = d.slot_type[j];
-                cj = dt / (2.0 * d.mass_t[tj]);
+                cj = P.cm[tj];
             }
             double xj, yj, zj;
             if (split) {
@@ -405,18 +405,13 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
         part[S_MV2] = m * ((v2x * v2x + v2y * v2y) + v2z * v2z);
         part[S_POT] = (double)acc.pot;
         part[S_VIR] = (double)acc.vir;
-        part[S_WV] = sqrt((v2x * v2x + v2y * v2y) + v2z * v2z);
-        part[S_WA] = sqrt(((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
-                          (double)acc.fz * (double)acc.fz) / m;
+        part[S_WV] = (v2x * v2x + v2y * v2y) + v2z * v2z;
+        part[S_WA] = (((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
+                      (double)acc.fz * (double)acc.fz) * P.inv_m2[ti];
         if (flags & F_INSERT) grid_insert(d, C, R, s, x2, y2, z2);
     }
-    if (split) {
-        // the kick / drift columns were stored by the first half
-        block_store<BS>(part, part_out, pstride, S_MVX, S_VIR + 1);
-        block_store<BS>(part, part_out, pstride, S_WV, S_WA + 1);
-    } else {
-        block_store<BS>(part, part_out, pstride, 0, NS);
-    }
+    // a split step stored its kick / drift columns in the first half
+    block_store<BS>(part, part_out, pstride, split ? kColsState : (kColsState | kick_cols | kColsDrift));
 }
 
 // Force evaluation at the current positions through a freshly built list
@@ -468,11 +463,11 @@ __global__ __launch_bounds__(BS) void k_force_eval(DevState d) {
         part[S_MV2] = m * ((vx * vx + vy * vy) + vz * vz);
         part[S_POT] = (double)acc.pot;
         part[S_VIR] = (double)acc.vir;
-        part[S_WV] = sqrt((vx * vx + vy * vy) + vz * vz);
-        part[S_WA] = sqrt(((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
-                          (double)acc.fz * (double)acc.fz) / m;
+        part[S_WV] = (vx * vx + vy * vy) + vz * vz;
+        part[S_WA] = (((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
+                      (double)acc.fz * (double)acc.fz) / (m * m);
     }
-    block_store<BS>(part, d.partial + b, d.n_blocks, 0, NS);
+    block_store<BS>(part, d.partial + b, d.n_blocks, kColsAll);
 }
 
 template <int BS>
@@ -510,10 +505,10 @@ __global__ __launch_bounds__(BS) void k_reduce_state(DevState d) {
         const double fx = (double)((const FT*)d.fx[cur])[s];
         const double fy = (double)((const FT*)d.fy[cur])[s];
         const double fz = (double)((const FT*)d.fz[cur])[s];
-        part[S_WV] = sqrt((vx * vx + vy * vy) + vz * vz);
-        part[S_WA] = sqrt((fx * fx + fy * fy) + fz * fz) / m;
+        part[S_WV] = (vx * vx + vy * vy) + vz * vz;
+        part[S_WA] = ((fx * fx + fy * fy) + fz * fz) / (m * m);
     }
-    block_store<BS>(part, d.partial + b, d.n_blocks, 0, NS);
+    block_store<BS>(part, d.partial + b, d.n_blocks, kColsAll);
 }
 
 // ---------------------------------------------------------------- O(N^2) FP64 path
@@ -534,7 +529,7 @@ __global__ __launch_bounds__(BS) void k_drift(DevState d) {
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
     if (valid) {
         const double dt = P.dt, lam = C.lambda, m = G.mass;
-        const double c = dt / (2.0 * m);
+        const double c = P.cm[G.type];
         const double* fx = (const double*)d.fx[cur];
         const double* fy = (const double*)d.fy[cur];
         const double* fz = (const double*)d.fz[cur];
@@ -552,10 +547,10 @@ __global__ __launch_bounds__(BS) void k_drift(DevState d) {
         d.x[nx][s] = wrap1(d.x[cur][s] + v1x * dt, C.box[0]);
         d.y[nx][s] = wrap1(d.y[cur][s] + v1y * dt, C.box[1]);
         d.z[nx][s] = wrap1(d.z[cur][s] + v1z * dt, C.box[2]);
-        part[S_VMAX] = sqrt((v1x * v1x + v1y * v1y) + v1z * v1z);
+        part[S_VMAX] = (v1x * v1x + v1y * v1y) + v1z * v1z;
     }
-    block_store<BS>(part, d.partial + b, d.n_blocks, S_KVX, S_KV2 + 1);
-    block_store<BS>(part, d.partial + b, d.n_blocks, S_VMAX, NS);
+    block_store<BS>(part, d.partial + b, d.n_blocks,
+                    (P.th_kind == 2 ? kColsKick : 0u) | kColsDrift);
 }
 
 // update_force by all pairs: thread per i, j tiles staged through LDS, j
@@ -617,7 +612,7 @@ __global__ __launch_bounds__(BS) void k_allpairs(DevState d) {
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
     if (valid) {
         const double m = G.mass;
-        const double c = P.dt / (2.0 * m);
+        const double c = P.cm[ti];
         double vx, vy, vz;
         if (FORCE_ONLY) {
             vx = d.vx[cur][s];
@@ -646,14 +641,13 @@ __global__ __launch_bounds__(BS) void k_allpairs(DevState d) {
         part[S_MV2] = m * ((vx * vx + vy * vy) + vz * vz);
         part[S_POT] = pot;
         part[S_VIR] = vir;
-        part[S_WV] = sqrt((vx * vx + vy * vy) + vz * vz);
-        part[S_WA] = sqrt((fx * fx + fy * fy) + fz * fz) / m;
+        part[S_WV] = (vx * vx + vy * vy) + vz * vz;
+        part[S_WA] = ((fx * fx + fy * fy) + fz * fz) / (m * m);
     }
     if (FORCE_ONLY) {
-        block_store<BS>(part, d.partial + b, d.n_blocks, 0, NS);
+        block_store<BS>(part, d.partial + b, d.n_blocks, kColsAll);
     } else {
-        block_store<BS>(part, d.partial + b, d.n_blocks, 0, S_KVX);
-        block_store<BS>(part, d.partial + b, d.n_blocks, S_WV, S_WA + 1);
+        block_store<BS>(part, d.partial + b, d.n_blocks, kColsState);
     }
 }
 
@@ -734,7 +728,7 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
                     // the list of a split step was built after this step's drift
                     double vmax = 0.0;
                     for (int t = 0; t < T; ++t) vmax = fmax(vmax, S[t * NS + S_VMAX]);
-                    C.D += vmax * dt / C.s;
+                    C.D += sqrt(vmax) * (1.0 + 1e-12) * dt / C.s;
                 }
                 if (baro) C.s *= C.myu;
             }
@@ -782,7 +776,8 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
         // |v + F dt / 2m| <= |v| + (dt / 2) |F| / m for every particle
         double wmax = 0.0;
         for (int t = 0; t < T; ++t)
-            wmax = fmax(wmax, (S[t * NS + S_WV] + (dt / 2.0) * S[t * NS + S_WA]) * (1.0 + 1e-12));
+            wmax = fmax(wmax, (sqrt(S[t * NS + S_WV]) + (dt / 2.0) * sqrt(S[t * NS + S_WA])) *
+                                  (1.0 + 1e-12));
         const double w = C.lambda * wmax * dt / C.s;
         const double thr = P.rlist - P.rc_max / C.s - P.margin;
         bool need = !(C.flags & F_HAVE_LIST) || P.rebuild_every_step || !(2.0 * (C.D + w) < thr);
@@ -833,32 +828,42 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
     }
 }
 
+// grid (slices, replicas).  Each block reduces its slice of a group's block
+// partials in a fixed order; with several slices the last block to finish
+// (release / acquire on a per-replica counter) adds the slice sums in slice
+// order, so the result does not depend on which block finishes last.  Then one
+// thread advances the replica's controls.
 template <int BS>
-__global__ __launch_bounds__(BS) void k_finalize(DevState d, int mode) {
-    const int r = blockIdx.x;
+__global__ __launch_bounds__(BS) void k_finalize(DevState d, int mode, int S) {
+    const int sl = blockIdx.x;
+    const int r = blockIdx.y;
     const RepInfo R = d.reps[r];
     RepCtrl* Cp = d.ctrl + r;
     const uint32_t f0 = Cp->flags;
     if (mode == M_STEP && (f0 & F_EXIT)) return;
     const bool reduce = mode == M_REDUCE || (mode == M_STEP && !(f0 & (F_NULL | F_SPLIT_A)));
+    __shared__ int s_last;
+    if (!reduce && sl != 0) return;
     if (reduce) {
         const int64_t nb = d.n_blocks;
         for (int t = 0; t < R.n_types; ++t) {
             const int g = R.first_group + t;
             const GroupInfo G = d.groups[g];
+            const int lo = (int)((int64_t)G.nblocks * sl / S);
+            const int hi = (int)((int64_t)G.nblocks * (sl + 1) / S);
             double acc[NS];
 #pragma unroll
             for (int k = 0; k < NS; ++k) acc[k] = 0.0;
             // column-major partials: consecutive threads read consecutive blocks;
             // four rows per thread are loaded before any is summed
-            for (int b0 = threadIdx.x; b0 < G.nblocks; b0 += 4 * BS) {
+            for (int b0 = lo + threadIdx.x; b0 < hi; b0 += 4 * BS) {
                 double v[4][NS];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     const int bb = b0 + u * BS;
 #pragma unroll
                     for (int k = 0; k < NS; ++k)
-                        v[u][k] = bb < G.nblocks ? d.partial[k * nb + G.first_block + bb] : 0.0;
+                        v[u][k] = bb < hi ? d.partial[k * nb + G.first_block + bb] : 0.0;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
@@ -866,7 +871,32 @@ __global__ __launch_bounds__(BS) void k_finalize(DevState d, int mode) {
                     for (int k = 0; k < NS; ++k)
                         acc[k] = k >= kFirstMax ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
             }
-            block_store<BS>(acc, d.sums + (int64_t)g * NS, 1, 0, NS);
+            if (S == 1)
+                block_store<BS>(acc, d.sums + (int64_t)g * NS, 1, kColsAll);
+            else
+                block_store<BS>(acc, d.part2 + ((int64_t)g * S + sl) * NS, 1, kColsAll);
+        }
+        if (S > 1) {
+            if (threadIdx.x == 0) {
+                __threadfence();
+                const unsigned int done = atomicAdd(&d.fin_count[r], 1u);
+                s_last = done == (unsigned int)(S - 1);
+            }
+            __syncthreads();
+            if (!s_last) return;
+            __threadfence();
+            const int k = threadIdx.x;
+            if (k < NS) {
+                for (int t = 0; t < R.n_types; ++t) {
+                    const int64_t g = R.first_group + t;
+                    const double* row = d.part2 + g * S * NS;
+                    double a = row[k];
+                    for (int q = 1; q < S; ++q)
+                        a = k >= kFirstMax ? fmax(a, row[q * NS + k]) : a + row[q * NS + k];
+                    d.sums[g * NS + k] = a;
+                }
+            }
+            if (threadIdx.x == 0) d.fin_count[r] = 0u;
         }
     }
     if (mode == M_REDUCE) return;
@@ -952,8 +982,9 @@ hipError_t launch_force_eval(const DevState& d, bool fp32, int bs, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const DevState& d, int mode, int n_reps, hipStream_t st) {
-    hipLaunchKernelGGL((k_finalize<512>), dim3(n_reps), dim3(512), 0, st, d, mode);
+hipError_t launch_finalize(const DevState& d, int mode, int n_reps, int slices, hipStream_t st) {
+    if (mode == M_PRIME) slices = 1;
+    hipLaunchKernelGGL((k_finalize<512>), dim3(slices, n_reps), dim3(512), 0, st, d, mode, slices);
     return hipGetLastError();
 }
 
