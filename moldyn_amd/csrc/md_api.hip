@@ -231,6 +231,8 @@ static void make_layout(md_ctx* c) {
             c->groups.push_back(G);
         }
         R.end = slot;
+        R.first_block = c->groups[R.first_group].first_block;
+        R.nblocks = block - R.first_block;
         const double want = ceil(std::max(0.1, c->opt.cells_per_atom) * (double)R.n_atoms);
         R.cell_budget = std::max<int64_t>(27, (int64_t)want);
         R.cell_base = cell_base;
@@ -301,6 +303,10 @@ static int upload_pairs(md_ctx* c) {
     return MD_OK;
 }
 
+static size_t rec_size(const md_ctx* c) {
+    return c->opt.force_mode == MD_FORCE_FP32_LIST ? sizeof(Rec<float>) : sizeof(Rec<double>);
+}
+
 // Allocate everything for the current layout; state buffers are NaN / zero.
 static int alloc_base(md_ctx* c) {
     free_all(c);
@@ -313,28 +319,14 @@ static int alloc_base(md_ctx* c) {
     const size_t nd = sizeof(double) * (size_t)std::max<int64_t>(N, 1);
     void* p;
     int rc;
-    double** dptrs[6] = {d.x, d.y, d.z, d.vx, d.vy, d.vz};
-    for (int k = 0; k < 6; ++k)
-        for (int b = 0; b < 2; ++b) {
-            if ((rc = dalloc(c, c->base_allocs, &p, nd))) return rc;
-            dptrs[k][b] = (double*)p;
-        }
-    void** fptrs[3] = {d.fx, d.fy, d.fz};
-    for (int k = 0; k < 3; ++k)
-        for (int b = 0; b < 2; ++b) {
-            if ((rc = dalloc(c, c->base_allocs, &p, nd))) return rc;
-            fptrs[k][b] = p;
-        }
+    for (int b = 0; b < 2; ++b) {
+        if ((rc = dalloc(c, c->base_allocs, &p, rec_size(c) * (size_t)std::max<int64_t>(N, 1)))) return rc;
+        d.rec[b] = p;
+    }
     if ((rc = dalloc(c, c->base_allocs, &p, nd))) return rc;
     d.pot = (double*)p;
     if ((rc = dalloc(c, c->base_allocs, &p, nd))) return rc;
     d.vir = (double*)p;
-    std::vector<uint8_t> st((size_t)std::max<int64_t>(N, 1), 0);
-    for (const GroupInfo& G : c->groups)
-        for (int64_t i = 0; i < (int64_t)G.nblocks * c->bs; ++i) st[(size_t)(G.start + i)] = (uint8_t)G.type;
-    if ((rc = dalloc(c, c->base_allocs, &p, st.size()))) return rc;
-    d.slot_type = (const uint8_t*)p;
-    HIPCHK(c, hipMemcpy(p, st.data(), st.size(), hipMemcpyHostToDevice));
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(int32_t) * std::max<size_t>(c->blk_group.size(), 1))))
         return rc;
     d.blk_group = (const int32_t*)p;
@@ -374,7 +366,6 @@ static int alloc_base(md_ctx* c) {
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * c->groups.size()))) return rc;
     d.sums = (double*)p;
     HIPCHK(c, hipMemset(p, 0, sizeof(double) * NS * c->groups.size()));
-    for (int t = 0; t < c->T && t < kMaxTypes; ++t) d.mass_t[t] = c->masses[t];
     d.n_types = c->T;
     d.n_blocks = c->n_blocks;
     d.n_slots = N;
@@ -526,7 +517,7 @@ static int ensure_graph(md_ctx* c, int len) {
 // ------------------------------------------------------------------ snapshot
 
 static size_t state_bytes(md_ctx* c) {
-    return sizeof(double) * (size_t)c->n_slots * 11;  // x y z vx vy vz fx fy fz pot vir
+    return (rec_size(c) + 2 * sizeof(double)) * (size_t)c->n_slots;  // record, pot, vir
 }
 
 static int snapshot(md_ctx* c, bool save, int buf) {
@@ -537,13 +528,15 @@ static int snapshot(md_ctx* c, bool save, int buf) {
         HIPCHK(c, hipMalloc((void**)&c->snap_ctrl, sizeof(RepCtrl) * c->R));
     }
     char* s = (char*)c->snap;
-    void* arrs[11] = {c->d.x[buf], c->d.y[buf], c->d.z[buf], c->d.vx[buf], c->d.vy[buf], c->d.vz[buf],
-                      c->d.fx[buf], c->d.fy[buf], c->d.fz[buf], c->d.pot, c->d.vir};
-    for (int k = 0; k < 11; ++k) {
+    void* arrs[3] = {c->d.rec[buf], c->d.pot, c->d.vir};
+    const size_t sz[3] = {rec_size(c) * (size_t)c->n_slots, n, n};
+    size_t off = 0;
+    for (int k = 0; k < 3; ++k) {
         if (save)
-            HIPCHK(c, hipMemcpyAsync(s + k * n, arrs[k], n, hipMemcpyDeviceToDevice, c->st));
+            HIPCHK(c, hipMemcpyAsync(s + off, arrs[k], sz[k], hipMemcpyDeviceToDevice, c->st));
         else
-            HIPCHK(c, hipMemcpyAsync(arrs[k], s + k * n, n, hipMemcpyDeviceToDevice, c->st));
+            HIPCHK(c, hipMemcpyAsync(arrs[k], s + off, sz[k], hipMemcpyDeviceToDevice, c->st));
+        off += sz[k];
     }
     if (save)
         HIPCHK(c, hipMemcpyAsync(c->snap_ctrl, c->d.ctrl, sizeof(RepCtrl) * c->R, hipMemcpyDeviceToDevice, c->st));
@@ -590,14 +583,18 @@ static int list_force_eval(md_ctx* c) {
             const int b = X.ins_epoch & 1;
             for (int k = 0; k < 3; ++k) X.gbox[b][k] = X.box[k];
             grid_dims(X.gbox[b], c->rlist, c->reps[r].cell_budget, X.gdim[b]);
+            brick_dims(X.gdim[b], c->reps[r].n_atoms, X.bedge[b], &X.bwhole[b]);
             X.fill_epoch = X.ins_epoch;
+            X.build_src = X.cur;
+            X.flags = F_BUILD;
         }
         if ((rc = write_ctrl(c, C))) return rc;
         std::vector<RepParams> P((size_t)c->R, base_params(c));
         if ((rc = write_params(c, P))) return rc;
         if ((rc = reset_stats(c))) return rc;
         if (c->n_blocks > 0) {
-            HIPCHK(c, launch_insert_all(c->d, c->bs, c->st));
+            HIPCHK(c, launch_insert_all(c->d, c->fp32, c->bs, c->st));
+            HIPCHK(c, launch_step(c->d, c->fp32, c->bs, c->st));  // F_BUILD: lists only
             HIPCHK(c, launch_force_eval(c->d, c->fp32, c->bs, c->st));
         }
         HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->d.fin_slices, c->st));
@@ -608,8 +605,7 @@ static int list_force_eval(md_ctx* c) {
         if (grown) continue;
         if ((rc = read_ctrl(c, C))) return rc;
         for (int r = 0; r < c->R; ++r) {
-            C[r].flags = (C[r].flags & ~(F_REBUILD | F_INSERT | F_NULL | F_EXIT | F_WRITE_PV)) |
-                         F_HAVE_LIST | F_GRID_FRESH;
+            C[r].flags = F_HAVE_LIST | F_GRID_FRESH;
             C[r].D = 0.0;
             C[r].s = 1.0;
         }
@@ -868,32 +864,120 @@ int md_get_options(const md_ctx* c, md_options* out) {
     return MD_OK;
 }
 
-int md_upload_state(md_ctx* c, const double* x, const double* v, const double* box) {
-    if (!c || !x || !v || !box) return fail(c, MD_ERR_ARG, "NULL argument");
-    const int64_t N = c->n_slots;
-    std::vector<double> h[6];
-    for (int k = 0; k < 6; ++k) h[k].assign((size_t)std::max<int64_t>(N, 1), k < 3 ? NAN : 0.0);
+}  // extern "C"
+
+// Host <-> device record transfer.  Host arrays are N x 3 (x, y, z per
+// particle) in the caller's order; slots add NaN-positioned padding.
+template <typename FT>
+static int put_records(md_ctx* c, int buf, const double* x, const double* v, const double* f,
+                       bool keep_forces) {
+    const size_t N = (size_t)std::max<int64_t>(c->n_slots, 1);
+    std::vector<Rec<FT>> h(N);
+    if (keep_forces)
+        HIPCHK(c, hipMemcpy(h.data(), c->d.rec[buf], sizeof(Rec<FT>) * N, hipMemcpyDeviceToHost));
+    for (const GroupInfo& G : c->groups)
+        for (int64_t i = 0; i < (int64_t)G.nblocks * c->bs; ++i) {
+            Rec<FT>& r = h[(size_t)(G.start + i)];
+            r.type = (uint32_t)G.type;
+            if (i >= G.count && !keep_forces) {
+                r.x = r.y = r.z = NAN;
+                r.vx = r.vy = r.vz = 0.0;
+                r.fx = r.fy = r.fz = (FT)0;
+            }
+        }
     int64_t a = 0;
     for (const GroupInfo& G : c->groups)
         for (int64_t i = 0; i < G.count; ++i, ++a) {
-            const size_t s = (size_t)(G.start + i);
-            for (int k = 0; k < 3; ++k) {
-                h[k][s] = x[3 * a + k];
-                h[3 + k][s] = v[3 * a + k];
+            Rec<FT>& r = h[(size_t)(G.start + i)];
+            if (x) {
+                r.x = x[3 * a];
+                r.y = x[3 * a + 1];
+                r.z = x[3 * a + 2];
+            }
+            if (v) {
+                r.vx = v[3 * a];
+                r.vy = v[3 * a + 1];
+                r.vz = v[3 * a + 2];
+            }
+            if (f) {
+                r.fx = (FT)f[3 * a];
+                r.fy = (FT)f[3 * a + 1];
+                r.fz = (FT)f[3 * a + 2];
+            } else if (!keep_forces) {
+                r.fx = r.fy = r.fz = (FT)0;
             }
         }
-    const size_t nb = sizeof(double) * (size_t)std::max<int64_t>(N, 1);
-    double* dp[6][2] = {{c->d.x[0], c->d.x[1]}, {c->d.y[0], c->d.y[1]}, {c->d.z[0], c->d.z[1]},
-                        {c->d.vx[0], c->d.vx[1]}, {c->d.vy[0], c->d.vy[1]}, {c->d.vz[0], c->d.vz[1]}};
-    for (int k = 0; k < 6; ++k)
-        for (int b = 0; b < 2; ++b) HIPCHK(c, hipMemcpyAsync(dp[k][b], h[k].data(), nb, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpy(c->d.rec[buf], h.data(), sizeof(Rec<FT>) * N, hipMemcpyHostToDevice));
+    return MD_OK;
+}
+
+template <typename FT>
+static int get_records(md_ctx* c, int buf, double* x, double* v, double* f) {
+    const size_t N = (size_t)std::max<int64_t>(c->n_slots, 1);
+    std::vector<Rec<FT>> h(N);
+    HIPCHK(c, hipMemcpy(h.data(), c->d.rec[buf], sizeof(Rec<FT>) * N, hipMemcpyDeviceToHost));
+    int64_t a = 0;
+    for (const GroupInfo& G : c->groups)
+        for (int64_t i = 0; i < G.count; ++i, ++a) {
+            const Rec<FT>& r = h[(size_t)(G.start + i)];
+            if (x) {
+                x[3 * a] = r.x;
+                x[3 * a + 1] = r.y;
+                x[3 * a + 2] = r.z;
+            }
+            if (v) {
+                v[3 * a] = r.vx;
+                v[3 * a + 1] = r.vy;
+                v[3 * a + 2] = r.vz;
+            }
+            if (f) {
+                f[3 * a] = (double)r.fx;
+                f[3 * a + 1] = (double)r.fy;
+                f[3 * a + 2] = (double)r.fz;
+            }
+        }
+    return MD_OK;
+}
+
+static int put_pv(md_ctx* c, const double* pot, const double* vir) {
+    const size_t N = (size_t)std::max<int64_t>(c->n_slots, 1);
+    std::vector<double> hp(N, 0.0), hv(N, 0.0);
+    int64_t a = 0;
+    for (const GroupInfo& G : c->groups)
+        for (int64_t i = 0; i < G.count; ++i, ++a) {
+            if (pot) hp[(size_t)(G.start + i)] = pot[a];
+            if (vir) hv[(size_t)(G.start + i)] = vir[a];
+        }
+    HIPCHK(c, hipMemcpy(c->d.pot, hp.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d.vir, hv.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    return MD_OK;
+}
+
+static int get_pv(md_ctx* c, double* pot, double* vir) {
+    const size_t N = (size_t)std::max<int64_t>(c->n_slots, 1);
+    std::vector<double> hp(N), hv(N);
+    HIPCHK(c, hipMemcpy(hp.data(), c->d.pot, sizeof(double) * N, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(hv.data(), c->d.vir, sizeof(double) * N, hipMemcpyDeviceToHost));
+    int64_t a = 0;
+    for (const GroupInfo& G : c->groups)
+        for (int64_t i = 0; i < G.count; ++i, ++a) {
+            if (pot) pot[a] = hp[(size_t)(G.start + i)];
+            if (vir) vir[a] = hv[(size_t)(G.start + i)];
+        }
+    return MD_OK;
+}
+
+extern "C" {
+
+int md_upload_state(md_ctx* c, const double* x, const double* v, const double* box) {
+    if (!c || !x || !v || !box) return fail(c, MD_ERR_ARG, "NULL argument");
+    int rc;
     for (int b = 0; b < 2; ++b) {
-        HIPCHK(c, hipMemsetAsync(c->d.fx[b], 0, nb, c->st));
-        HIPCHK(c, hipMemsetAsync(c->d.fy[b], 0, nb, c->st));
-        HIPCHK(c, hipMemsetAsync(c->d.fz[b], 0, nb, c->st));
+        rc = c->fp32 ? put_records<float>(c, b, x, v, nullptr, false)
+                     : put_records<double>(c, b, x, v, nullptr, false);
+        if (rc) return rc;
     }
-    HIPCHK(c, hipMemsetAsync(c->d.pot, 0, nb, c->st));
-    HIPCHK(c, hipMemsetAsync(c->d.vir, 0, nb, c->st));
+    if ((rc = put_pv(c, nullptr, nullptr))) return rc;
     std::vector<RepCtrl> C((size_t)c->R);
     for (int r = 0; r < c->R; ++r) {
         RepCtrl X{};
@@ -907,7 +991,7 @@ int md_upload_state(md_ctx* c, const double* x, const double* v, const double* b
         C[r] = X;
         for (int k = 0; k < 3; ++k) c->box0[(size_t)r * 3 + k] = box[3 * r + k];
     }
-    int rc = write_ctrl(c, C);
+    rc = write_ctrl(c, C);
     if (rc) return rc;
     c->sums_valid = false;
     c->stats.rebuilds = 0;
@@ -921,30 +1005,15 @@ int md_upload_forces(md_ctx* c, const double* force, const double* pot, const do
     int rc = read_ctrl(c, C);
     if (rc) return rc;
     const int buf = C[0].cur;
-    const int64_t N = c->n_slots;
-    const size_t n = (size_t)std::max<int64_t>(N, 1);
-    std::vector<double> h[5];
-    for (int k = 0; k < 5; ++k) h[k].assign(n, 0.0);
-    int64_t a = 0;
-    for (const GroupInfo& G : c->groups)
-        for (int64_t i = 0; i < G.count; ++i, ++a) {
-            const size_t s = (size_t)(G.start + i);
-            if (force)
-                for (int k = 0; k < 3; ++k) h[k][s] = force[3 * a + k];
-            if (pot) h[3][s] = pot[a];
-            if (vir) h[4][s] = vir[a];
-        }
-    void* fd[3] = {c->d.fx[buf], c->d.fy[buf], c->d.fz[buf]};
-    for (int k = 0; k < 3; ++k) {
-        if (c->fp32) {
-            std::vector<float> t(h[k].begin(), h[k].end());
-            HIPCHK(c, hipMemcpy(fd[k], t.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-        } else {
-            HIPCHK(c, hipMemcpy(fd[k], h[k].data(), sizeof(double) * n, hipMemcpyHostToDevice));
-        }
+    std::vector<double> zero;
+    if (!force) {
+        zero.assign((size_t)c->n_atoms * 3 + 1, 0.0);
+        force = zero.data();
     }
-    HIPCHK(c, hipMemcpy(c->d.pot, h[3].data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d.vir, h[4].data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    rc = c->fp32 ? put_records<float>(c, buf, nullptr, nullptr, force, true)
+                 : put_records<double>(c, buf, nullptr, nullptr, force, true);
+    if (rc) return rc;
+    if ((rc = put_pv(c, pot, vir))) return rc;
     c->sums_valid = false;
     return MD_OK;
 }
@@ -955,52 +1024,13 @@ int md_download_state(md_ctx* c, double* x, double* v, double* force, double* po
     std::vector<RepCtrl> C;
     int rc = read_ctrl(c, C);
     if (rc) return rc;
-    const int64_t N = c->n_slots;
-    const size_t nb = sizeof(double) * (size_t)std::max<int64_t>(N, 1);
     const int buf = C[0].cur;
-    auto fetch = [&](const void* src, std::vector<double>& dst, bool f32) -> int {
-        dst.assign((size_t)std::max<int64_t>(N, 1), 0.0);
-        if (f32) {
-            std::vector<float> tmp(dst.size());
-            HIPCHK(c, hipMemcpy(tmp.data(), src, sizeof(float) * tmp.size(), hipMemcpyDeviceToHost));
-            for (size_t i = 0; i < tmp.size(); ++i) dst[i] = (double)tmp[i];
-        } else {
-            HIPCHK(c, hipMemcpy(dst.data(), src, nb, hipMemcpyDeviceToHost));
-        }
-        return MD_OK;
-    };
-    std::vector<double> a[3];
-    auto scatter = [&](double* out, int comp_stride, int comp) {
-        int64_t k = 0;
-        for (const GroupInfo& G : c->groups)
-            for (int64_t i = 0; i < G.count; ++i, ++k) out[comp_stride * k + comp] = a[comp][(size_t)(G.start + i)];
-    };
-    if (x) {
-        if ((rc = fetch(c->d.x[buf], a[0], false)) || (rc = fetch(c->d.y[buf], a[1], false)) ||
-            (rc = fetch(c->d.z[buf], a[2], false)))
-            return rc;
-        for (int k = 0; k < 3; ++k) scatter(x, 3, k);
+    if (x || v || force) {
+        rc = c->fp32 ? get_records<float>(c, buf, x, v, force) : get_records<double>(c, buf, x, v, force);
+        if (rc) return rc;
     }
-    if (v) {
-        if ((rc = fetch(c->d.vx[buf], a[0], false)) || (rc = fetch(c->d.vy[buf], a[1], false)) ||
-            (rc = fetch(c->d.vz[buf], a[2], false)))
-            return rc;
-        for (int k = 0; k < 3; ++k) scatter(v, 3, k);
-    }
-    if (force) {
-        if ((rc = fetch(c->d.fx[buf], a[0], c->fp32)) || (rc = fetch(c->d.fy[buf], a[1], c->fp32)) ||
-            (rc = fetch(c->d.fz[buf], a[2], c->fp32)))
-            return rc;
-        for (int k = 0; k < 3; ++k) scatter(force, 3, k);
-    }
-    if (pot) {
-        if ((rc = fetch(c->d.pot, a[0], false))) return rc;
-        scatter(pot, 1, 0);
-    }
-    if (vir) {
-        if ((rc = fetch(c->d.vir, a[0], false))) return rc;
-        scatter(vir, 1, 0);
-    }
+    if (pot || vir)
+        if ((rc = get_pv(c, pot, vir))) return rc;
     if (box)
         for (int r = 0; r < c->R; ++r)
             for (int k = 0; k < 3; ++k) box[3 * r + k] = C[r].box[k];
@@ -1010,8 +1040,8 @@ int md_download_state(md_ctx* c, double* x, double* v, double* force, double* po
 // Relayout keeping the state (used when block size / force path change).
 static int relayout(md_ctx* c, const md_options& nopt) {
     const int64_t n = c->n_atoms;
-    std::vector<double> x((size_t)n * 3), v((size_t)n * 3), f((size_t)n * 3), pot((size_t)n),
-        vir((size_t)n), box((size_t)c->R * 3);
+    std::vector<double> x((size_t)n * 3 + 1), v((size_t)n * 3 + 1), f((size_t)n * 3 + 1),
+        pot((size_t)n + 1), vir((size_t)n + 1), box((size_t)c->R * 3);
     int rc = md_download_state(c, x.data(), v.data(), f.data(), pot.data(), vir.data(), box.data());
     if (rc) return rc;
     std::vector<RepCtrl> C;
@@ -1023,28 +1053,7 @@ static int relayout(md_ctx* c, const md_options& nopt) {
     if ((rc = alloc_base(c))) return rc;
     if ((rc = md_upload_state(c, x.data(), v.data(), box.data()))) return rc;
     // restore forces, potentials and virials in the new precision
-    const int64_t N = c->n_slots;
-    std::vector<double> h[5];
-    for (int k = 0; k < 5; ++k) h[k].assign((size_t)std::max<int64_t>(N, 1), 0.0);
-    int64_t a = 0;
-    for (const GroupInfo& G : c->groups)
-        for (int64_t i = 0; i < G.count; ++i, ++a) {
-            const size_t s = (size_t)(G.start + i);
-            for (int k = 0; k < 3; ++k) h[k][s] = f[(size_t)(3 * a + k)];
-            h[3][s] = pot[(size_t)a];
-            h[4][s] = vir[(size_t)a];
-        }
-    void* fd[3] = {c->d.fx[0], c->d.fy[0], c->d.fz[0]};
-    for (int k = 0; k < 3; ++k) {
-        if (c->fp32) {
-            std::vector<float> t(h[k].begin(), h[k].end());
-            HIPCHK(c, hipMemcpy(fd[k], t.data(), sizeof(float) * t.size(), hipMemcpyHostToDevice));
-        } else {
-            HIPCHK(c, hipMemcpy(fd[k], h[k].data(), sizeof(double) * h[k].size(), hipMemcpyHostToDevice));
-        }
-    }
-    HIPCHK(c, hipMemcpy(c->d.pot, h[3].data(), sizeof(double) * h[3].size(), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->d.vir, h[4].data(), sizeof(double) * h[4].size(), hipMemcpyHostToDevice));
+    if ((rc = md_upload_forces(c, f.data(), pot.data(), vir.data()))) return rc;
     std::vector<RepCtrl> C2;
     if ((rc = read_ctrl(c, C2))) return rc;
     for (int r = 0; r < c->R; ++r) {
@@ -1141,7 +1150,7 @@ int md_update_force(md_ctx* c) {
 
 int md_apply_boundary_conditions(md_ctx* c) {
     if (!c) return MD_ERR_ARG;
-    if (c->n_blocks > 0) HIPCHK(c, launch_wrap(c->d, c->bs, c->st));
+    if (c->n_blocks > 0) HIPCHK(c, launch_wrap(c->d, c->fp32, c->bs, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return MD_OK;
 }
@@ -1224,7 +1233,7 @@ int md_thermostat_update(md_ctx* c, md_thermostat* th, double dt, uint16_t type_
     if (rc) return rc;
     if (th->kind == 1 || th->kind == 2) {
         const GroupInfo& G = c->groups[c->reps[0].first_group + type_id];
-        HIPCHK(c, launch_scale(c->d, c->bs, G.start, G.count, th->lambda, 0, c->st));
+        HIPCHK(c, launch_scale(c->d, c->fp32, G.start, G.count, th->lambda, 0, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         c->sums_valid = false;
     }
@@ -1250,7 +1259,7 @@ int md_barostat_update(md_ctx* c, md_barostat* ba, uint16_t type_id) {
     if (ba->kind == 2) return fail(c, MD_ERR_UNIMPLEMENTED, "Barostat::Custom is todo!() in the reference");
     if (ba->kind != 1) return MD_OK;
     const GroupInfo& G = c->groups[c->reps[0].first_group + type_id];
-    HIPCHK(c, launch_scale(c->d, c->bs, G.start, G.count, ba->myu, 1, c->st));
+    HIPCHK(c, launch_scale(c->d, c->fp32, G.start, G.count, ba->myu, 1, c->st));
     std::vector<RepCtrl> C;
     int rc = read_ctrl(c, C);
     if (rc) return rc;

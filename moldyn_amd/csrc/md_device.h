@@ -7,14 +7,17 @@
 //    multiple of the per-atom block size, so every block belongs to exactly
 //    one group and the per-group FP64 reductions need no segmenting.
 //    Padding slots hold NaN positions, which fail every distance test.
-//  * state is SoA and double-buffered: x/y/z and vx/vy/vz in FP64, forces
-//    FP64 (exact paths) or FP32 (fast path).  The fused step kernel reads
-//    buffer `cur` and writes `cur ^ 1`.
+//  * state is an array of per-particle records (position and velocity FP64,
+//    force FP32 on the fast path or FP64 on the exact paths), double-buffered:
+//    a step reads buffer `cur` and writes `cur ^ 1`.  A neighbour's whole
+//    record is one 64-byte gather (80 bytes with FP64 forces).
 //  * Verlet lists are column-major (nbr[k * n_slots + slot]) so that the k-th
 //    neighbour of 64 consecutive slots is one coalesced load.
 //  * the cell grid is two epoch-tagged header arrays (count | epoch << 32)
 //    with fixed-capacity slot arrays, so filling it never needs a clear and a
-//    fill can run while the other buffer is being read.
+//    fill can run while the other buffer is being read.  Lists are built from
+//    it brick by brick: a block stages a brick of cells plus a one-cell halo
+//    (ids and positions) in LDS and builds the lists of the brick's atoms.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,18 +29,37 @@ namespace md {
 constexpr int kMaxTypes = 8;
 constexpr double kKB = 1.380648528;  // core/src/lib.rs:15
 
+// per-particle state record
+template <typename FT>
+struct Rec;
+template <>
+struct alignas(16) Rec<float> {
+    double x, y, z, vx, vy, vz;
+    float fx, fy, fz;
+    uint32_t type;
+};
+template <>
+struct alignas(16) Rec<double> {
+    double x, y, z, vx, vy, vz;
+    double fx, fy, fz;
+    uint32_t type, pad;
+};
+static_assert(sizeof(Rec<float>) == 64, "record size");
+static_assert(sizeof(Rec<double>) == 80, "record size");
+
 // per-launch control flags of a replica
 enum : uint32_t {
-    F_REBUILD = 1u << 0,    // rebuild Verlet lists from grid `fill_epoch` at launch start
     F_INSERT = 1u << 1,     // insert this launch's output positions into grid `ins_epoch`
-    F_NULL = 1u << 2,       // no time step: insert the input positions, rebuild next launch
+    F_NULL = 1u << 2,       // no time step: insert the input positions
     F_EXIT = 1u << 3,       // target step reached: launch does nothing
     F_WRITE_PV = 1u << 4,   // store per-atom potential and virial in this launch
     F_HAVE_LIST = 1u << 5,  // lists exist (built at some fill)
     F_GRID_FRESH = 1u << 6, // grid `fill_epoch` holds the current input positions
     F_BAD = 1u << 7,        // non-finite T/P or mu^3 <= 0 seen
     F_SPLIT_A = 1u << 8,    // drift exceeds the skin: kick + drift + grid fill only
-    F_SPLIT_B = 1u << 9,    // ... then rebuild at the drifted positions, force, kick
+    F_SPLIT_B = 1u << 9,    // ... then force and kick at the drifted positions
+    F_BUILD = 1u << 10,     // no time step: build the lists from grid `fill_epoch`
+    F_SPLIT_PENDING = 1u << 11,  // a BUILD between the halves of a split step
 };
 
 // per-block partial / per-group sum columns
@@ -72,6 +94,7 @@ struct GroupInfo {
 struct RepInfo {
     int64_t start, end;  // slot range
     int32_t first_group, n_types;
+    int32_t first_block, nblocks;
     int64_t cell_base;   // first cell of this replica in each grid buffer
     int64_t cell_budget; // cells available
     int64_t n_atoms;
@@ -87,6 +110,7 @@ struct RepParams {
     double rlist, rc_max, margin, insert_growth;
     double psi_in;        // Nose-Hoover psi handed in by the caller
     int32_t use_psi_in;   // M_PRIME starts from psi_in instead of psi_used
+    int32_t pad0;
     double cm[kMaxTypes];     // dt / (2 m_t), the half-kick factor (integrator.rs:29)
     double inv_m2[kMaxTypes]; // 1 / m_t^2 (drift-bound bookkeeping only)
 };
@@ -97,11 +121,15 @@ struct RepCtrl {
     double lambda_used, myu_used, psi_used;  // of the last completed step (caller-visible)
     double s, D;               // scale since list build, drift bound in build units
     double gbox[2][3];         // box of grid buffer 0/1
-    int32_t gdim[2][3];
+    int32_t gdim[2][3];        // cells per axis of grid buffer 0/1
+    int32_t bedge[2][3];       // brick edge in cells (= gdim on whole-axis bricks)
+    int32_t bwhole[2];         // axes whose brick spans the whole grid (bit mask)
     uint32_t flags;
     uint32_t ins_epoch;        // epoch this launch inserts with
     uint32_t fill_epoch;       // epoch of the last completed fill
     int32_t cur;               // state buffer holding the input
+    int32_t build_src;         // state buffer a BUILD launch reads
+    int32_t pad1;
     int64_t step;              // completed real steps
     int64_t rebuilds, nulls, launches, splits;
 };
@@ -109,23 +137,13 @@ struct RepCtrl {
 struct RepStats {
     unsigned int list_overflow, cell_overflow;
     int max_list, max_cell;
-    unsigned long long list_sum;
-    unsigned long long list_atoms;
+    unsigned int brick_fallback, pad;
 };
 
 struct DevState {
-    double* x[2];
-    double* y[2];
-    double* z[2];
-    double* vx[2];
-    double* vy[2];
-    double* vz[2];
-    void* fx[2];
-    void* fy[2];
-    void* fz[2];
+    void* rec[2];            // Rec<float> or Rec<double>, n_slots each
     double* pot;
     double* vir;
-    const uint8_t* slot_type;
     int32_t* nbr;
     int32_t* ncount;
     int32_t list_cap;
@@ -143,12 +161,25 @@ struct DevState {
     double* part2;           // n_groups x fin_slices x NS, finalize slice sums
     unsigned int* fin_count; // per replica, finalize slices done (last one resets)
     double* sums;            // n_groups x NS
-    double mass_t[kMaxTypes];
     int32_t n_types;
     int32_t n_blocks;
     int32_t fin_slices;
     int64_t n_slots;
 };
+
+enum FinalizeMode { M_STEP = 0, M_REDUCE = 1, M_PRIME = 2 };
+
+// the finalize stages these through LDS word by word with 128 lanes each
+static_assert(sizeof(RepCtrl) % 4 == 0 && sizeof(RepCtrl) / 4 <= 128, "RepCtrl staging");
+static_assert(sizeof(RepParams) % 4 == 0 && sizeof(RepParams) / 4 <= 128, "RepParams staging");
+static_assert(sizeof(GroupInfo) % 4 == 0 && sizeof(GroupInfo) / 4 * kMaxTypes <= 128,
+              "GroupInfo staging");
+static_assert(kMaxTypes * NS <= 128, "sums staging");
+
+// LDS capacity of one brick: halo cells and staged particles
+constexpr int kBrickCells = 1000;  // (8 + 2)^3
+constexpr int kBrickAtoms = 448;
+constexpr int kBrickTarget = 224;  // expected staged particles a brick is sized for
 
 // Cell grid of a fill: per axis floor(L / (r_list (1 + 1e-9))) cells in
 // [1, 1024], coarsened until the replica's cell budget holds.  Cell edges stay
@@ -168,11 +199,27 @@ __host__ __device__ inline void grid_dims(const double* gb, double rlist, int64_
     }
 }
 
-enum FinalizeMode { M_STEP = 0, M_REDUCE = 1, M_PRIME = 2 };
-
-// host-side launchers (md_kernels.hip)
-hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st);
-hipError_t launch_finalize(const DevState& d, int mode, int n_reps, int slices, hipStream_t st);
+// Brick shape of a grid: the largest cubic edge E <= 8 cells whose halo
+// ((E + 2)^3 cells, or the whole axis where that is shorter) holds about
+// kBrickTarget particles at the grid's mean occupancy.
+__host__ __device__ inline void brick_dims(const int* gd, int64_t n_atoms, int* edge, int* whole) {
+    const double per_cell = (double)n_atoms / ((double)gd[0] * gd[1] * gd[2]);
+    int best = 1;
+    for (int e = 1; e <= 8; ++e) {
+        double halo = 1.0;
+        for (int k = 0; k < 3; ++k) halo *= (gd[k] <= e + 2) ? gd[k] : (e + 2);
+        if (halo * per_cell <= (double)kBrickTarget) best = e;
+    }
+    *whole = 0;
+    for (int k = 0; k < 3; ++k) {
+        if (gd[k] <= best + 2) {
+            edge[k] = gd[k];
+            *whole |= 1 << k;
+        } else {
+            edge[k] = best;
+        }
+    }
+}
 
 // column masks for block partial stores
 constexpr uint32_t kColsKick = (1u << S_KVX) | (1u << S_KVY) | (1u << S_KVZ) | (1u << S_KV2);
@@ -180,13 +227,17 @@ constexpr uint32_t kColsState = (1u << S_MVX) | (1u << S_MVY) | (1u << S_MVZ) | 
                                 (1u << S_POT) | (1u << S_VIR) | (1u << S_WV) | (1u << S_WA);
 constexpr uint32_t kColsDrift = 1u << S_VMAX;
 constexpr uint32_t kColsAll = (1u << NS) - 1u;
+
+// host-side launchers (md_kernels.hip)
+hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st);
+hipError_t launch_finalize(const DevState& d, int mode, int n_reps, int slices, hipStream_t st);
 hipError_t launch_drift(const DevState& d, int bs, hipStream_t st);
 hipError_t launch_allpairs(const DevState& d, int bs, bool force_only, hipStream_t st);
-hipError_t launch_insert_all(const DevState& d, int bs, hipStream_t st);
+hipError_t launch_insert_all(const DevState& d, bool fp32, int bs, hipStream_t st);
 hipError_t launch_force_eval(const DevState& d, bool fp32, int bs, hipStream_t st);
 hipError_t launch_reduce_state(const DevState& d, bool fp32, int bs, hipStream_t st);
-hipError_t launch_scale(const DevState& d, int bs, int64_t start, int64_t count, double fac,
+hipError_t launch_scale(const DevState& d, bool fp32, int64_t start, int64_t count, double fac,
                         int which /*0 velocity, 1 position*/, hipStream_t st);
-hipError_t launch_wrap(const DevState& d, int bs, hipStream_t st);
+hipError_t launch_wrap(const DevState& d, bool fp32, int bs, hipStream_t st);
 
 }  // namespace md

@@ -3,18 +3,19 @@
 // Per time step of a replica the engine runs two launches:
 //   k_step      one thread per particle slot: half kick, thermostat scale,
 //               drift, wrap (integrator.rs:28-45), Verlet-list force
-//               (potential.rs:158-216) with the neighbours' drift recomputed
-//               from the input buffer, second half kick and barostat scale
-//               (integrator.rs:47-58), per-block FP64 partial sums.  When the
-//               list needs refreshing the same launch rebuilds it at entry
-//               from the cell grid filled by the previous launch, and fills
-//               the other grid buffer at exit.
-//   k_finalize  one block per replica: fixed-order reduction of the block
-//               partials, then the Berendsen / Nose-Hoover lambda, psi and
-//               Berendsen mu of the next step (thermostat.rs:24-72,
-//               barostat.rs:21-56), box rescale and the list-validity
-//               decision.  Everything stays on the device; the host only
-//               replays a captured graph.
+//               (potential.rs:158-216) with each neighbour's drift recomputed
+//               from its input record, second half kick and barostat scale
+//               (integrator.rs:47-58), per-block FP64 partial sums, and when
+//               asked a fill of the cell grid with the output positions.
+//               The same kernel also serves the launches that do not advance
+//               time: filling the grid with the input positions (F_NULL),
+//               building the Verlet lists brick by brick (F_BUILD), and the
+//               two halves of a step whose drift exceeds the skin (F_SPLIT_*).
+//   k_finalize  fixed-order reduction of the block partials, then the
+//               Berendsen / Nose-Hoover lambda, psi and Berendsen mu of the
+//               next step (thermostat.rs:24-72, barostat.rs:21-56), box
+//               rescale and the list-validity decision for the next launch.
+//               Everything stays on the device; the host replays a graph.
 // The O(N^2) FP64 path (k_drift + k_allpairs) restates update_force exactly:
 // same pair criterion, same operation order, j ascending.
 //
@@ -80,6 +81,37 @@ __device__ __forceinline__ void block_store(const double (&v)[NS], double* out, 
             acc = k >= kFirstMax ? fmax(acc, red[i * NS + k]) : acc + red[i * NS + k];
         out[k * stride] = acc;
     }
+    __syncthreads();
+}
+
+// In-place exclusive scan of a[0..n) (n <= kBrickCells); a[n] receives the total.
+template <int BS>
+__device__ void block_exclusive_scan(int* a, int n) {
+    constexpr int NW = BS / 64;
+    __shared__ int wsum[NW];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int chunk = (n + BS - 1) / BS;
+    const int lo = t * chunk < n ? t * chunk : n;
+    const int hi = lo + chunk < n ? lo + chunk : n;
+    int local = 0;
+    for (int i = lo; i < hi; ++i) local += a[i];
+    int incl = local;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int before = 0;
+    for (int i = 0; i < w; ++i) before += wsum[i];
+    int run = before + incl - local;
+    for (int i = lo; i < hi; ++i) {
+        const int c = a[i];
+        a[i] = run;
+        run += c;
+    }
+    if (t == BS - 1) a[n] = before + incl;
     __syncthreads();
 }
 
@@ -183,26 +215,47 @@ __device__ __forceinline__ void grid_insert(const DevState& d, const RepCtrl& C,
     }
 }
 
-// Verlet list of `slot` from grid `fill_epoch`: every j of the same replica,
-// type >= own type (potential.rs:171-180), j != slot, within rlist under the
-// reference minimum image, sorted ascending so that sums follow the
-// reference's j order.
-template <bool MULTI>
-__device__ void build_list(const DevState& d, const RepCtrl& C, const RepParams& P,
-                           const RepInfo& R, int64_t slot, int ti, const double* xs,
-                           const double* ys, const double* zs, double x, double y, double z) {
+// Sorted insertion of j into the list column of slot i (column-major).
+__device__ __forceinline__ void list_push(int32_t* __restrict__ col, int64_t N, int n, int cap,
+                                          int32_t j) {
+    if (n < cap) {
+        int p = n;
+        while (p > 0 && col[(int64_t)(p - 1) * N] > j) {
+            col[(int64_t)p * N] = col[(int64_t)(p - 1) * N];
+            --p;
+        }
+        col[(int64_t)p * N] = j;
+    }
+}
+
+__device__ __forceinline__ void list_close(const DevState& d, int64_t slot, int n) {
+    d.ncount[slot] = n < d.list_cap ? n : d.list_cap;
+    if (n > d.list_cap) {
+        atomicOr(&d.stats->list_overflow, 1u);
+        atomicMax(&d.stats->max_list, n);
+    }
+}
+
+// Verlet list of one particle straight from the grid (fallback when a brick
+// does not fit in LDS): every j of the same replica, type >= own type
+// (potential.rs:171-180), j != slot, within rlist under the reference
+// minimum image, sorted ascending so that sums follow the reference's j order.
+template <typename FT, bool MULTI>
+__device__ void build_list_global(const DevState& d, const RepCtrl& C, const RepParams& P,
+                                  const RepInfo& R, const Rec<FT>* __restrict__ recs,
+                                  int64_t slot) {
     const uint32_t ep = C.fill_epoch;
     const int buf = ep & 1;
     const int* gd = C.gdim[buf];
     const double* gb = C.gbox[buf];
-    const int cx = cell_coord(x, gb[0], gd[0]);
-    const int cy = cell_coord(y, gb[1], gd[1]);
-    const int cz = cell_coord(z, gb[2], gd[2]);
+    const Rec<FT> me = recs[slot];
+    const int ti = (int)me.type;
+    const int cx = cell_coord(me.x, gb[0], gd[0]);
+    const int cy = cell_coord(me.y, gb[1], gd[1]);
+    const int cz = cell_coord(me.z, gb[2], gd[2]);
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double rl2 = P.rlist * P.rlist;
-    const int cap = d.list_cap;
     const int cc = d.cell_cap;
-    const int64_t N = d.n_slots;
     int32_t* __restrict__ col = d.nbr + slot;
     const unsigned long long* __restrict__ hdr = d.cell_hdr[buf];
     const int32_t* __restrict__ cs = d.cell_slots[buf];
@@ -224,29 +277,156 @@ __device__ void build_list(const DevState& d, const RepCtrl& C, const RepParams&
                 for (int k = 0; k < cnt; ++k) {
                     const int32_t j = cs[cell * cc + k];
                     if ((int64_t)j == slot) continue;
-                    if (MULTI && (int)d.slot_type[j] < ti) continue;
-                    const double dx = min_image(xs[j] - x, bx);
-                    const double dy = min_image(ys[j] - y, by);
-                    const double dz = min_image(zs[j] - z, bz);
+                    const Rec<FT> o = recs[j];
+                    if (MULTI && (int)o.type < ti) continue;
+                    const double dx = min_image(o.x - me.x, bx);
+                    const double dy = min_image(o.y - me.y, by);
+                    const double dz = min_image(o.z - me.z, bz);
                     const double r2 = (dx * dx + dy * dy) + dz * dz;
                     if (!(r2 <= rl2)) continue;
-                    if (n < cap) {
-                        int p = n;
-                        while (p > 0 && col[(int64_t)(p - 1) * N] > j) {
-                            col[(int64_t)p * N] = col[(int64_t)(p - 1) * N];
-                            --p;
-                        }
-                        col[(int64_t)p * N] = j;
-                    }
+                    list_push(col, d.n_slots, n, d.list_cap, j);
                     ++n;
                 }
             }
         }
     }
-    d.ncount[slot] = n < cap ? n : cap;
-    if (n > cap) {
-        atomicOr(&d.stats->list_overflow, 1u);
-        atomicMax(&d.stats->max_list, n);
+    list_close(d, slot, n);
+}
+
+// Lists of every particle of a replica, one brick of cells at a time: the
+// brick and a one-cell halo are staged in LDS (slot ids, positions, types),
+// then each particle of the brick scans its 27 neighbour cells there.
+// Bricks are shared out among the replica's blocks.
+template <typename FT, int BS, bool MULTI>
+__device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams& P,
+                            const RepInfo& R, int block_in_rep) {
+    const uint32_t ep = C.fill_epoch;
+    const int buf = ep & 1;
+    const int* gd = C.gdim[buf];
+    const int* be = C.bedge[buf];
+    const int whole = C.bwhole[buf];
+    const Rec<FT>* __restrict__ recs = (const Rec<FT>*)d.rec[C.build_src];
+    const unsigned long long* __restrict__ hdr = d.cell_hdr[buf];
+    const int32_t* __restrict__ cs = d.cell_slots[buf];
+    const int cc = d.cell_cap;
+    const double bx = C.box[0], by = C.box[1], bz = C.box[2];
+    const double rl2 = P.rlist * P.rlist;
+    const int64_t N = d.n_slots;
+    int nbk[3], hd[3];
+    for (int k = 0; k < 3; ++k) {
+        const bool w = (whole >> k) & 1;
+        nbk[k] = w ? 1 : (gd[k] + be[k] - 1) / be[k];
+        hd[k] = w ? gd[k] : be[k] + 2;
+    }
+    const int nbricks = nbk[0] * nbk[1] * nbk[2];
+    const int H = hd[0] * hd[1] * hd[2];
+    __shared__ int s_off[kBrickCells + 1];
+    __shared__ double s_x[kBrickAtoms], s_y[kBrickAtoms], s_z[kBrickAtoms];
+    __shared__ int32_t s_id[kBrickAtoms];
+    __shared__ uint16_t s_hc[kBrickAtoms];
+    __shared__ uint8_t s_ty[MULTI ? kBrickAtoms : 1];
+    for (int bi = block_in_rep; bi < nbricks; bi += R.nblocks) {
+        const int bc[3] = {bi / (nbk[1] * nbk[2]), (bi / nbk[2]) % nbk[1], bi % nbk[2]};
+        int org[3], ncen[3];
+        for (int k = 0; k < 3; ++k) {
+            const bool w = (whole >> k) & 1;
+            org[k] = w ? 0 : bc[k] * be[k] - 1;
+            const int rest = gd[k] - bc[k] * be[k];
+            ncen[k] = w ? gd[k] : (rest < be[k] ? rest : be[k]);
+        }
+        // halo cell h -> grid cell (halo coordinates wrap periodically)
+        auto cell_of = [&](int h) -> int64_t {
+            const int h0 = h / (hd[1] * hd[2]), h1 = (h / hd[2]) % hd[1], h2 = h % hd[2];
+            const int c0 = ((org[0] + h0) % gd[0] + gd[0]) % gd[0];
+            const int c1 = ((org[1] + h1) % gd[1] + gd[1]) % gd[1];
+            const int c2 = ((org[2] + h2) % gd[2] + gd[2]) % gd[2];
+            return R.cell_base + ((int64_t)c0 * gd[1] + c1) * gd[2] + c2;
+        };
+        for (int h = threadIdx.x; h < H; h += BS) {
+            const unsigned long long w = hdr[cell_of(h)];
+            int cnt = (uint32_t)(w >> 32) == ep ? (int)(uint32_t)w : 0;
+            s_off[h] = cnt < cc ? cnt : cc;
+        }
+        __syncthreads();
+        block_exclusive_scan<BS>(s_off, H);
+        const int total = s_off[H];
+        if (total > kBrickAtoms) {
+            // too dense for LDS: lists of the brick's particles straight from the grid
+            if (threadIdx.x == 0) atomicOr(&d.stats->brick_fallback, 1u);
+            const int ncell = ncen[0] * ncen[1] * ncen[2];
+            for (int q = threadIdx.x; q < ncell; q += BS) {
+                const int q0 = q / (ncen[1] * ncen[2]), q1 = (q / ncen[2]) % ncen[1], q2 = q % ncen[2];
+                const int h = (((whole & 1) ? q0 : q0 + 1) * hd[1] + ((whole & 2) ? q1 : q1 + 1)) * hd[2] +
+                              ((whole & 4) ? q2 : q2 + 1);
+                const int64_t cell = cell_of(h);
+                const int cnt = s_off[h + 1] - s_off[h];
+                for (int k = 0; k < cnt; ++k)
+                    build_list_global<FT, MULTI>(d, C, P, R, recs, cs[cell * cc + k]);
+            }
+            __syncthreads();
+            continue;
+        }
+        for (int h = threadIdx.x; h < H; h += BS) {
+            const int64_t cell = cell_of(h);
+            const int base = s_off[h], cnt = s_off[h + 1] - base;
+            for (int k = 0; k < cnt; ++k) {
+                const int32_t j = cs[cell * cc + k];
+                const Rec<FT>& o = recs[j];
+                s_x[base + k] = o.x;
+                s_y[base + k] = o.y;
+                s_z[base + k] = o.z;
+                s_id[base + k] = j;
+                s_hc[base + k] = (uint16_t)h;
+                if (MULTI) s_ty[base + k] = (uint8_t)o.type;
+            }
+        }
+        __syncthreads();
+        for (int a = threadIdx.x; a < total; a += BS) {
+            const int h = s_hc[a];
+            const int hc[3] = {h / (hd[1] * hd[2]), (h / hd[2]) % hd[1], h % hd[2]};
+            bool center = true;
+            int lo[3], hi[3];
+            for (int k = 0; k < 3; ++k) {
+                if ((whole >> k) & 1) {
+                    lo[k] = gd[k] >= 3 ? -1 : 0;
+                    hi[k] = gd[k] >= 2 ? 1 : 0;
+                } else {
+                    center = center && hc[k] >= 1 && hc[k] <= ncen[k];
+                    lo[k] = -1;
+                    hi[k] = 1;
+                }
+            }
+            if (!center) continue;
+            const int32_t i = s_id[a];
+            const int ti = MULTI ? (int)s_ty[a] : 0;
+            const double xi = s_x[a], yi = s_y[a], zi = s_z[a];
+            int32_t* __restrict__ col = d.nbr + i;
+            int n = 0;
+            for (int o0 = lo[0]; o0 <= hi[0]; ++o0) {
+                const int g0 = ((whole & 1) ? (hc[0] + o0 + gd[0]) % gd[0] : hc[0] + o0);
+                for (int o1 = lo[1]; o1 <= hi[1]; ++o1) {
+                    const int g1 = ((whole & 2) ? (hc[1] + o1 + gd[1]) % gd[1] : hc[1] + o1);
+                    for (int o2 = lo[2]; o2 <= hi[2]; ++o2) {
+                        const int g2 = ((whole & 4) ? (hc[2] + o2 + gd[2]) % gd[2] : hc[2] + o2);
+                        const int h2 = (g0 * hd[1] + g1) * hd[2] + g2;
+                        for (int b2 = s_off[h2]; b2 < s_off[h2 + 1]; ++b2) {
+                            const int32_t j = s_id[b2];
+                            if (j == i) continue;
+                            if (MULTI && (int)s_ty[b2] < ti) continue;
+                            const double dx = min_image(s_x[b2] - xi, bx);
+                            const double dy = min_image(s_y[b2] - yi, by);
+                            const double dz = min_image(s_z[b2] - zi, bz);
+                            const double r2 = (dx * dx + dy * dy) + dz * dz;
+                            if (!(r2 <= rl2)) continue;
+                            list_push(col, N, n, d.list_cap, j);
+                            ++n;
+                        }
+                    }
+                }
+            }
+            list_close(d, i, n);
+        }
+        __syncthreads();
     }
 }
 
@@ -265,10 +445,16 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
     const int64_t s = (int64_t)b * BS + threadIdx.x;
     const bool valid = (s - G.start) < G.count;
     const int cur = C.cur, nx = cur ^ 1;
+    const Rec<FT>* __restrict__ rin = (const Rec<FT>*)d.rec[cur];
+    Rec<FT>* __restrict__ rnx = (Rec<FT>*)d.rec[nx];
     double* const part_out = d.partial + b;
     const int64_t pstride = d.n_blocks;
     if (flags & F_NULL) {
-        if (valid) grid_insert(d, C, R, s, d.x[cur][s], d.y[cur][s], d.z[cur][s]);
+        if (valid) grid_insert(d, C, R, s, rin[s].x, rin[s].y, rin[s].z);
+        return;
+    }
+    if (flags & F_BUILD) {
+        brick_build<FT, BS, MULTI>(d, C, P, R, b - R.first_block);
         return;
     }
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
@@ -284,59 +470,46 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
         // first half of a split step: kick, scale, drift, wrap into cur ^ 1 and
         // fill the grid with the drifted positions (the drift exceeds the skin)
         if (valid) {
-            const double vkx = d.vx[cur][s] + (double)((const FT*)d.fx[cur])[s] * c;
-            const double vky = d.vy[cur][s] + (double)((const FT*)d.fy[cur])[s] * c;
-            const double vkz = d.vz[cur][s] + (double)((const FT*)d.fz[cur])[s] * c;
+            Rec<FT> a = rin[s];
+            const double vkx = a.vx + (double)a.fx * c;
+            const double vky = a.vy + (double)a.fy * c;
+            const double vkz = a.vz + (double)a.fz * c;
             part[S_KVX] = vkx * m;
             part[S_KVY] = vky * m;
             part[S_KVZ] = vkz * m;
             part[S_KV2] = m * ((vkx * vkx + vky * vky) + vkz * vkz);
-            const double v1x = vkx * lam, v1y = vky * lam, v1z = vkz * lam;
-            const double x1 = wrap1(d.x[cur][s] + v1x * dt, bx);
-            const double y1 = wrap1(d.y[cur][s] + v1y * dt, by);
-            const double z1 = wrap1(d.z[cur][s] + v1z * dt, bz);
-            d.x[nx][s] = x1;
-            d.y[nx][s] = y1;
-            d.z[nx][s] = z1;
-            d.vx[nx][s] = v1x;
-            d.vy[nx][s] = v1y;
-            d.vz[nx][s] = v1z;
-            part[S_VMAX] = (v1x * v1x + v1y * v1y) + v1z * v1z;
-            grid_insert(d, C, R, s, x1, y1, z1);
+            a.vx = vkx * lam;
+            a.vy = vky * lam;
+            a.vz = vkz * lam;
+            a.x = wrap1(a.x + a.vx * dt, bx);
+            a.y = wrap1(a.y + a.vy * dt, by);
+            a.z = wrap1(a.z + a.vz * dt, bz);
+            rnx[s] = a;
+            part[S_VMAX] = (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz;
+            grid_insert(d, C, R, s, a.x, a.y, a.z);
         }
         block_store<BS>(part, part_out, pstride, kick_cols | kColsDrift);
         return;
     }
     // second half of a split step: drifted state in cur ^ 1, results to cur
     const bool split = (flags & F_SPLIT_B) != 0;
-    const int src = split ? nx : cur;
-    const int dst = split ? cur : nx;
-    const double* xs = d.x[src];
-    const double* ys = d.y[src];
-    const double* zs = d.z[src];
-    const double* vxs = d.vx[src];
-    const double* vys = d.vy[src];
-    const double* vzs = d.vz[src];
-    const FT* fxs = (const FT*)d.fx[cur];
-    const FT* fys = (const FT*)d.fy[cur];
-    const FT* fzs = (const FT*)d.fz[cur];
+    const Rec<FT>* __restrict__ src = split ? rnx : rin;
+    Rec<FT>* __restrict__ dst = split ? (Rec<FT>*)d.rec[cur] : rnx;
     if (valid) {
+        const Rec<FT> a = src[s];
         double x1, y1, z1, v1x, v1y, v1z;
         if (split) {
-            x1 = xs[s];
-            y1 = ys[s];
-            z1 = zs[s];
-            v1x = vxs[s];
-            v1y = vys[s];
-            v1z = vzs[s];
-            build_list<MULTI>(d, C, P, R, s, ti, xs, ys, zs, x1, y1, z1);
+            x1 = a.x;
+            y1 = a.y;
+            z1 = a.z;
+            v1x = a.vx;
+            v1y = a.vy;
+            v1z = a.vz;
         } else {
-            const double x = xs[s], y = ys[s], z = zs[s];
-            if (flags & F_REBUILD) build_list<MULTI>(d, C, P, R, s, ti, xs, ys, zs, x, y, z);
             // integrator.rs:28-34 first half kick
-            const double vkx = vxs[s] + (double)fxs[s] * c;
-            const double vky = vys[s] + (double)fys[s] * c;
-            const double vkz = vzs[s] + (double)fzs[s] * c;
+            const double vkx = a.vx + (double)a.fx * c;
+            const double vky = a.vy + (double)a.fy * c;
+            const double vkz = a.vz + (double)a.fz * c;
             part[S_KVX] = vkx * m;
             part[S_KVY] = vky * m;
             part[S_KVZ] = vkz * m;
@@ -345,34 +518,27 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
             v1x = vkx * lam;
             v1y = vky * lam;
             v1z = vkz * lam;
-            x1 = wrap1(x + v1x * dt, bx);
-            y1 = wrap1(y + v1y * dt, by);
-            z1 = wrap1(z + v1z * dt, bz);
+            x1 = wrap1(a.x + v1x * dt, bx);
+            y1 = wrap1(a.y + v1y * dt, by);
+            z1 = wrap1(a.z + v1z * dt, bz);
             part[S_VMAX] = (v1x * v1x + v1y * v1y) + v1z * v1z;
         }
         // potential.rs:158-216 over the Verlet list at the drifted positions;
-        // a neighbour's drift is recomputed from the input buffer with the
+        // a neighbour's drift is recomputed from its input record with the
         // same operations its own thread performs, so it is bitwise the same
         Acc<FT> acc;
         const int nn = d.ncount[s];
         const PairParam p0 = d.pairs[0];
         for (int k = 0; k < nn; ++k) {
             const int32_t j = d.nbr[(int64_t)k * d.n_slots + s];
-            int tj = 0;
-            double cj = c;
-            if (MULTI) {
-                tj = d.slot_type[j];
-                cj = P.cm[tj];
-            }
-            double xj, yj, zj;
-            if (split) {
-                xj = xs[j];
-                yj = ys[j];
-                zj = zs[j];
-            } else {
-                xj = wrap1(xs[j] + ((vxs[j] + (double)fxs[j] * cj) * lam) * dt, bx);
-                yj = wrap1(ys[j] + ((vys[j] + (double)fys[j] * cj) * lam) * dt, by);
-                zj = wrap1(zs[j] + ((vzs[j] + (double)fzs[j] * cj) * lam) * dt, bz);
+            const Rec<FT> o = src[j];
+            const int tj = MULTI ? (int)o.type : 0;
+            double xj = o.x, yj = o.y, zj = o.z;
+            if (!split) {
+                const double cj = MULTI ? P.cm[tj] : c;
+                xj = wrap1(xj + ((o.vx + (double)o.fx * cj) * lam) * dt, bx);
+                yj = wrap1(yj + ((o.vy + (double)o.fy * cj) * lam) * dt, by);
+                zj = wrap1(zj + ((o.vz + (double)o.fz * cj) * lam) * dt, bz);
             }
             const double dx = min_image(xj - x1, bx);
             const double dy = min_image(yj - y1, by);
@@ -382,54 +548,48 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
             if (r2 <= pp.r2max) acc.add(dx, dy, dz, r2, pp);
         }
         // integrator.rs:47-53 second half kick, barostat.rs:45-48 scale
-        const double v2x = v1x + (double)acc.fx * c;
-        const double v2y = v1y + (double)acc.fy * c;
-        const double v2z = v1z + (double)acc.fz * c;
-        const double x2 = x1 * mu, y2 = y1 * mu, z2 = z1 * mu;
-        d.x[dst][s] = x2;
-        d.y[dst][s] = y2;
-        d.z[dst][s] = z2;
-        d.vx[dst][s] = v2x;
-        d.vy[dst][s] = v2y;
-        d.vz[dst][s] = v2z;
-        ((FT*)d.fx[dst])[s] = acc.fx;
-        ((FT*)d.fy[dst])[s] = acc.fy;
-        ((FT*)d.fz[dst])[s] = acc.fz;
+        Rec<FT> out;
+        out.vx = v1x + (double)acc.fx * c;
+        out.vy = v1y + (double)acc.fy * c;
+        out.vz = v1z + (double)acc.fz * c;
+        out.x = x1 * mu;
+        out.y = y1 * mu;
+        out.z = z1 * mu;
+        out.fx = acc.fx;
+        out.fy = acc.fy;
+        out.fz = acc.fz;
+        out.type = a.type;
+        dst[s] = out;
         if (flags & F_WRITE_PV) {
             d.pot[s] = (double)acc.pot;
             d.vir[s] = (double)acc.vir;
         }
-        part[S_MVX] = v2x * m;
-        part[S_MVY] = v2y * m;
-        part[S_MVZ] = v2z * m;
-        part[S_MV2] = m * ((v2x * v2x + v2y * v2y) + v2z * v2z);
+        part[S_MVX] = out.vx * m;
+        part[S_MVY] = out.vy * m;
+        part[S_MVZ] = out.vz * m;
+        part[S_MV2] = m * ((out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz);
         part[S_POT] = (double)acc.pot;
         part[S_VIR] = (double)acc.vir;
-        part[S_WV] = (v2x * v2x + v2y * v2y) + v2z * v2z;
+        part[S_WV] = (out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz;
         part[S_WA] = (((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
                       (double)acc.fz * (double)acc.fz) * P.inv_m2[ti];
-        if (flags & F_INSERT) grid_insert(d, C, R, s, x2, y2, z2);
+        if (flags & F_INSERT) grid_insert(d, C, R, s, out.x, out.y, out.z);
     }
     // a split step stored its kick / drift columns in the first half
     block_store<BS>(part, part_out, pstride, split ? kColsState : (kColsState | kick_cols | kColsDrift));
 }
 
-// Force evaluation at the current positions through a freshly built list
-// (update_force for the list paths).  Grid `fill_epoch` must hold x[cur].
+// Force evaluation at the current positions through the current lists
+// (update_force for the list paths).
 template <typename FT, int BS, bool MULTI>
 __global__ __launch_bounds__(BS) void k_force_eval(DevState d) {
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
     const int r = G.rep;
     const RepCtrl& C = d.ctrl[r];
-    const RepParams& P = d.params[r];
-    const RepInfo R = d.reps[r];
     const int64_t s = (int64_t)b * BS + threadIdx.x;
     const bool valid = (s - G.start) < G.count;
-    const int cur = C.cur;
-    const double* xs = d.x[cur];
-    const double* ys = d.y[cur];
-    const double* zs = d.z[cur];
+    Rec<FT>* __restrict__ rc = (Rec<FT>*)d.rec[C.cur];
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double m = G.mass;
     const int ti = G.type;
@@ -437,48 +597,48 @@ __global__ __launch_bounds__(BS) void k_force_eval(DevState d) {
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
     if (valid) {
-        const double x = xs[s], y = ys[s], z = zs[s];
-        build_list<MULTI>(d, C, P, R, s, ti, xs, ys, zs, x, y, z);
+        Rec<FT> a = rc[s];
         Acc<FT> acc;
         const int nn = d.ncount[s];
         for (int k = 0; k < nn; ++k) {
             const int32_t j = d.nbr[(int64_t)k * d.n_slots + s];
-            const int tj = MULTI ? (int)d.slot_type[j] : 0;
-            const double dx = min_image(xs[j] - x, bx);
-            const double dy = min_image(ys[j] - y, by);
-            const double dz = min_image(zs[j] - z, bz);
+            const Rec<FT> o = rc[j];
+            const int tj = MULTI ? (int)o.type : 0;
+            const double dx = min_image(o.x - a.x, bx);
+            const double dy = min_image(o.y - a.y, by);
+            const double dz = min_image(o.z - a.z, bz);
             const double r2 = (dx * dx + dy * dy) + dz * dz;
             const PairParam& pp = d.pairs[ti * d.n_types + tj];
             if (r2 <= pp.r2max) acc.add(dx, dy, dz, r2, pp);
         }
-        ((FT*)d.fx[cur])[s] = acc.fx;
-        ((FT*)d.fy[cur])[s] = acc.fy;
-        ((FT*)d.fz[cur])[s] = acc.fz;
+        a.fx = acc.fx;
+        a.fy = acc.fy;
+        a.fz = acc.fz;
+        rc[s] = a;
         d.pot[s] = (double)acc.pot;
         d.vir[s] = (double)acc.vir;
-        const double vx = d.vx[cur][s], vy = d.vy[cur][s], vz = d.vz[cur][s];
-        part[S_MVX] = vx * m;
-        part[S_MVY] = vy * m;
-        part[S_MVZ] = vz * m;
-        part[S_MV2] = m * ((vx * vx + vy * vy) + vz * vz);
+        part[S_MVX] = a.vx * m;
+        part[S_MVY] = a.vy * m;
+        part[S_MVZ] = a.vz * m;
+        part[S_MV2] = m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
         part[S_POT] = (double)acc.pot;
         part[S_VIR] = (double)acc.vir;
-        part[S_WV] = (vx * vx + vy * vy) + vz * vz;
+        part[S_WV] = (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz;
         part[S_WA] = (((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
                       (double)acc.fz * (double)acc.fz) / (m * m);
     }
     block_store<BS>(part, d.partial + b, d.n_blocks, kColsAll);
 }
 
-template <int BS>
+template <typename FT, int BS>
 __global__ __launch_bounds__(BS) void k_insert_all(DevState d) {
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
     const RepCtrl& C = d.ctrl[G.rep];
     const int64_t s = (int64_t)b * BS + threadIdx.x;
     if ((s - G.start) >= G.count) return;
-    const int cur = C.cur;
-    grid_insert(d, C, d.reps[G.rep], s, d.x[cur][s], d.y[cur][s], d.z[cur][s]);
+    const Rec<FT>& a = ((const Rec<FT>*)d.rec[C.cur])[s];
+    grid_insert(d, C, d.reps[G.rep], s, a.x, a.y, a.z);
 }
 
 // Partial sums of the current state (after an upload or a host-side change).
@@ -489,23 +649,20 @@ __global__ __launch_bounds__(BS) void k_reduce_state(DevState d) {
     const RepCtrl& C = d.ctrl[G.rep];
     const int64_t s = (int64_t)b * BS + threadIdx.x;
     const bool valid = (s - G.start) < G.count;
-    const int cur = C.cur;
     double part[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
     if (valid) {
+        const Rec<FT> a = ((const Rec<FT>*)d.rec[C.cur])[s];
         const double m = G.mass;
-        const double vx = d.vx[cur][s], vy = d.vy[cur][s], vz = d.vz[cur][s];
-        part[S_MVX] = vx * m;
-        part[S_MVY] = vy * m;
-        part[S_MVZ] = vz * m;
-        part[S_MV2] = m * ((vx * vx + vy * vy) + vz * vz);
+        part[S_MVX] = a.vx * m;
+        part[S_MVY] = a.vy * m;
+        part[S_MVZ] = a.vz * m;
+        part[S_MV2] = m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
         part[S_POT] = d.pot[s];
         part[S_VIR] = d.vir[s];
-        const double fx = (double)((const FT*)d.fx[cur])[s];
-        const double fy = (double)((const FT*)d.fy[cur])[s];
-        const double fz = (double)((const FT*)d.fz[cur])[s];
-        part[S_WV] = (vx * vx + vy * vy) + vz * vz;
+        const double fx = (double)a.fx, fy = (double)a.fy, fz = (double)a.fz;
+        part[S_WV] = (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz;
         part[S_WA] = ((fx * fx + fy * fy) + fz * fz) / (m * m);
     }
     block_store<BS>(part, d.partial + b, d.n_blocks, kColsAll);
@@ -530,24 +687,22 @@ __global__ __launch_bounds__(BS) void k_drift(DevState d) {
     if (valid) {
         const double dt = P.dt, lam = C.lambda, m = G.mass;
         const double c = P.cm[G.type];
-        const double* fx = (const double*)d.fx[cur];
-        const double* fy = (const double*)d.fy[cur];
-        const double* fz = (const double*)d.fz[cur];
-        const double vkx = d.vx[cur][s] + fx[s] * c;
-        const double vky = d.vy[cur][s] + fy[s] * c;
-        const double vkz = d.vz[cur][s] + fz[s] * c;
+        Rec<double> a = ((const Rec<double>*)d.rec[cur])[s];
+        const double vkx = a.vx + a.fx * c;
+        const double vky = a.vy + a.fy * c;
+        const double vkz = a.vz + a.fz * c;
         part[S_KVX] = vkx * m;
         part[S_KVY] = vky * m;
         part[S_KVZ] = vkz * m;
         part[S_KV2] = m * ((vkx * vkx + vky * vky) + vkz * vkz);
-        const double v1x = vkx * lam, v1y = vky * lam, v1z = vkz * lam;
-        d.vx[nx][s] = v1x;
-        d.vy[nx][s] = v1y;
-        d.vz[nx][s] = v1z;
-        d.x[nx][s] = wrap1(d.x[cur][s] + v1x * dt, C.box[0]);
-        d.y[nx][s] = wrap1(d.y[cur][s] + v1y * dt, C.box[1]);
-        d.z[nx][s] = wrap1(d.z[cur][s] + v1z * dt, C.box[2]);
-        part[S_VMAX] = (v1x * v1x + v1y * v1y) + v1z * v1z;
+        a.vx = vkx * lam;
+        a.vy = vky * lam;
+        a.vz = vkz * lam;
+        a.x = wrap1(a.x + a.vx * dt, C.box[0]);
+        a.y = wrap1(a.y + a.vy * dt, C.box[1]);
+        a.z = wrap1(a.z + a.vz * dt, C.box[2]);
+        ((Rec<double>*)d.rec[nx])[s] = a;
+        part[S_VMAX] = (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz;
     }
     block_store<BS>(part, d.partial + b, d.n_blocks,
                     (P.th_kind == 2 ? kColsKick : 0u) | kColsDrift);
@@ -555,9 +710,10 @@ __global__ __launch_bounds__(BS) void k_drift(DevState d) {
 
 // update_force by all pairs: thread per i, j tiles staged through LDS, j
 // ascending from the start of i's type to the end of the replica
-// (potential.rs:171-177), i == j skipped.  FORCE_ONLY evaluates at x[cur]
-// and stores F; otherwise positions come from the drift buffer and the
-// second half kick and barostat scale follow (integrator.rs:46-58).
+// (potential.rs:171-177), i == j skipped.  FORCE_ONLY evaluates at the
+// current positions and stores F; otherwise positions come from the drift
+// buffer and the second half kick and barostat scale follow
+// (integrator.rs:46-58).
 template <int BS, bool MULTI, bool FORCE_ONLY>
 __global__ __launch_bounds__(BS) void k_allpairs(DevState d) {
     const int b = blockIdx.x;
@@ -570,37 +726,35 @@ __global__ __launch_bounds__(BS) void k_allpairs(DevState d) {
     const int64_t s = (int64_t)b * BS + threadIdx.x;
     const bool valid = (s - G.start) < G.count;
     const int cur = C.cur;
-    const int src = FORCE_ONLY ? cur : (cur ^ 1);
-    const double* xs = d.x[src];
-    const double* ys = d.y[src];
-    const double* zs = d.z[src];
+    const Rec<double>* __restrict__ src = (const Rec<double>*)d.rec[FORCE_ONLY ? cur : (cur ^ 1)];
+    Rec<double>* __restrict__ dst = (Rec<double>*)d.rec[cur];
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const int ti = G.type;
     const int T = d.n_types;
     __shared__ double tx[BS], ty[BS], tz[BS];
     __shared__ uint8_t tt[BS];
-    const double xi = valid ? xs[s] : 0.0;
-    const double yi = valid ? ys[s] : 0.0;
-    const double zi = valid ? zs[s] : 0.0;
+    Rec<double> a;
+    if (valid) a = src[s];
     double fx = 0.0, fy = 0.0, fz = 0.0, pot = 0.0, vir = 0.0;
     const PairParam p0 = d.pairs[ti * T + ti];
     for (int64_t t0 = G.start; t0 < R.end; t0 += BS) {
         const int64_t jl = t0 + threadIdx.x;
         __syncthreads();
         if (jl < R.end) {
-            tx[threadIdx.x] = xs[jl];
-            ty[threadIdx.x] = ys[jl];
-            tz[threadIdx.x] = zs[jl];
-            if (MULTI) tt[threadIdx.x] = d.slot_type[jl];
+            const Rec<double>& o = src[jl];
+            tx[threadIdx.x] = o.x;
+            ty[threadIdx.x] = o.y;
+            tz[threadIdx.x] = o.z;
+            if (MULTI) tt[threadIdx.x] = (uint8_t)o.type;
         }
         __syncthreads();
         const int n = (R.end - t0) < BS ? (int)(R.end - t0) : BS;
         if (valid) {
             for (int k = 0; k < n; ++k) {
                 if (t0 + k == s) continue;
-                const double dx = min_image(tx[k] - xi, bx);
-                const double dy = min_image(ty[k] - yi, by);
-                const double dz = min_image(tz[k] - zi, bz);
+                const double dx = min_image(tx[k] - a.x, bx);
+                const double dy = min_image(ty[k] - a.y, by);
+                const double dz = min_image(tz[k] - a.z, bz);
                 const double r2 = (dx * dx + dy * dy) + dz * dz;
                 const PairParam& pp = MULTI ? d.pairs[ti * T + tt[k]] : p0;
                 if (r2 <= pp.r2max) pair_exact(dx, dy, dz, r2, pp, fx, fy, fz, pot, vir);
@@ -613,42 +767,31 @@ __global__ __launch_bounds__(BS) void k_allpairs(DevState d) {
     if (valid) {
         const double m = G.mass;
         const double c = P.cm[ti];
-        double vx, vy, vz;
-        if (FORCE_ONLY) {
-            vx = d.vx[cur][s];
-            vy = d.vy[cur][s];
-            vz = d.vz[cur][s];
-        } else {
-            vx = d.vx[src][s] + fx * c;
-            vy = d.vy[src][s] + fy * c;
-            vz = d.vz[src][s] + fz * c;
+        if (!FORCE_ONLY) {
+            a.vx = a.vx + fx * c;
+            a.vy = a.vy + fy * c;
+            a.vz = a.vz + fz * c;
             const double mu = C.myu;
-            d.x[cur][s] = xi * mu;
-            d.y[cur][s] = yi * mu;
-            d.z[cur][s] = zi * mu;
-            d.vx[cur][s] = vx;
-            d.vy[cur][s] = vy;
-            d.vz[cur][s] = vz;
+            a.x = a.x * mu;
+            a.y = a.y * mu;
+            a.z = a.z * mu;
         }
-        ((double*)d.fx[cur])[s] = fx;
-        ((double*)d.fy[cur])[s] = fy;
-        ((double*)d.fz[cur])[s] = fz;
+        a.fx = fx;
+        a.fy = fy;
+        a.fz = fz;
+        dst[s] = a;
         d.pot[s] = pot;
         d.vir[s] = vir;
-        part[S_MVX] = vx * m;
-        part[S_MVY] = vy * m;
-        part[S_MVZ] = vz * m;
-        part[S_MV2] = m * ((vx * vx + vy * vy) + vz * vz);
+        part[S_MVX] = a.vx * m;
+        part[S_MVY] = a.vy * m;
+        part[S_MVZ] = a.vz * m;
+        part[S_MV2] = m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
         part[S_POT] = pot;
         part[S_VIR] = vir;
-        part[S_WV] = (vx * vx + vy * vy) + vz * vz;
+        part[S_WV] = (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz;
         part[S_WA] = ((fx * fx + fy * fy) + fz * fz) / (m * m);
     }
-    if (FORCE_ONLY) {
-        block_store<BS>(part, d.partial + b, d.n_blocks, kColsAll);
-    } else {
-        block_store<BS>(part, d.partial + b, d.n_blocks, kColsState);
-    }
+    block_store<BS>(part, d.partial + b, d.n_blocks, FORCE_ONLY ? kColsAll : kColsState);
 }
 
 // ---------------------------------------------------------------- finalize
@@ -669,6 +812,13 @@ __device__ __forceinline__ void thermo(const double* S, const GroupInfo& G, cons
     pressure = (th2 + (-S[S_VIR]) * 0.5) / volume / 3.0;
 }
 
+__device__ void set_fill(RepCtrl& C, const RepParams& P, const RepInfo& R, const double* gb) {
+    C.ins_epoch = C.fill_epoch + 1u;
+    const int g = C.ins_epoch & 1;
+    for (int k = 0; k < 3; ++k) C.gbox[g][k] = gb[k];
+    grid_dims(C.gbox[g], P.rlist, R.cell_budget, C.gdim[g]);
+    brick_dims(C.gdim[g], R.n_atoms, C.bedge[g], &C.bwhole[g]);
+}
 
 // Advance a replica's controls after a launch (M_STEP) or prepare the first
 // step of a call (M_PRIME).  S holds the group sums of the replica's types.
@@ -688,12 +838,26 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
             C.nulls++;
             controls = false;
         } else if (f & F_SPLIT_A) {
-            // drifted state and its grid are ready; the second half follows
+            // drifted state and its grid are ready: build lists there, then the second half
             C.fill_epoch = C.ins_epoch;
-            C.flags = F_HAVE_LIST | F_SPLIT_B;
+            C.build_src = C.cur ^ 1;
+            C.flags = F_BUILD | F_SPLIT_PENDING;
             C.splits++;
             controls = false;
             decide = false;
+        } else if (f & F_BUILD) {
+            // lists now describe the build positions
+            C.D = 0.0;
+            C.s = 1.0;
+            C.rebuilds++;
+            const uint32_t keep = f & F_GRID_FRESH;
+            if (f & F_SPLIT_PENDING) {
+                C.flags = F_HAVE_LIST | F_SPLIT_B;
+                decide = false;
+            } else {
+                C.flags = F_HAVE_LIST | keep;
+            }
+            controls = false;
         } else {
             C.step++;
             C.lambda_used = C.lambda;
@@ -718,12 +882,6 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
             }
             uint32_t nf = f & F_HAVE_LIST;
             if (P.list_mode) {
-                if (f & (F_REBUILD | F_SPLIT_B)) {
-                    C.D = 0.0;
-                    C.s = 1.0;
-                    C.rebuilds++;
-                    nf |= F_HAVE_LIST;
-                }
                 if (!(f & F_SPLIT_B)) {
                     // the list of a split step was built after this step's drift
                     double vmax = 0.0;
@@ -780,44 +938,33 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
                                   (1.0 + 1e-12));
         const double w = C.lambda * wmax * dt / C.s;
         const double thr = P.rlist - P.rc_max / C.s - P.margin;
-        bool need = !(C.flags & F_HAVE_LIST) || P.rebuild_every_step || !(2.0 * (C.D + w) < thr);
+        const bool need =
+            !(C.flags & F_HAVE_LIST) || P.rebuild_every_step || !(2.0 * (C.D + w) < thr);
         if (need) {
             const double w0 = C.lambda * wmax * dt;
             if (!(2.0 * w0 < P.rlist - P.rc_max - P.margin)) {
                 // one drift exceeds what any list built before it can cover
                 C.flags |= F_SPLIT_A;
-                C.ins_epoch = C.fill_epoch + 1u;
-                const int gbf = C.ins_epoch & 1;
-                for (int k = 0; k < 3; ++k) C.gbox[gbf][k] = C.box[k];
-                grid_dims(C.gbox[gbf], P.rlist, R.cell_budget, C.gdim[gbf]);
+                set_fill(C, P, R, C.box);
             } else if (C.flags & F_GRID_FRESH) {
-                C.flags |= F_REBUILD;
+                C.flags |= F_BUILD;
+                C.build_src = C.cur;
             } else {
                 C.flags |= F_NULL;
-                C.ins_epoch = C.fill_epoch + 1u;
-                const int gbf = C.ins_epoch & 1;
-                for (int k = 0; k < 3; ++k) C.gbox[gbf][k] = C.box[k];
-                grid_dims(C.gbox[gbf], P.rlist, R.cell_budget, C.gdim[gbf]);
+                set_fill(C, P, R, C.box);
             }
-        }
-        if (!(C.flags & (F_NULL | F_SPLIT_A))) {
-            const bool rb = (C.flags & F_REBUILD) != 0;
-            const double d1 = rb ? 0.0 : C.D;
-            const double s1 = rb ? 1.0 : C.s;
-            const double w1 = C.lambda * wmax * dt / s1;
-            const double s2 = s1 * (baro ? C.myu : 1.0);
+        } else {
+            // fill the grid during this step when the list may expire within
+            // the next two, so that the rebuild needs no separate fill
+            const double s2 = C.s * (baro ? C.myu : 1.0);
             const double thr2 = P.rlist - P.rc_max / s2 - P.margin;
-            if (P.rebuild_every_step ||
-                !(2.0 * (d1 + w1 * (2.0 + P.insert_growth)) < thr2)) {
+            if (P.rebuild_every_step || !(2.0 * (C.D + w * (2.0 + P.insert_growth)) < thr2)) {
                 C.flags |= F_INSERT;
-                C.ins_epoch = C.fill_epoch + 1u;
-                const int gbf = C.ins_epoch & 1;
                 double gb[3] = {C.box[0], C.box[1], C.box[2]};
                 if (baro)
                     for (int t = 0; t < T; ++t)
                         for (int k = 0; k < 3; ++k) gb[k] *= C.myu;
-                for (int k = 0; k < 3; ++k) C.gbox[gbf][k] = gb[k];
-                grid_dims(C.gbox[gbf], P.rlist, R.cell_budget, C.gdim[gbf]);
+                set_fill(C, P, R, gb);
             }
         }
     }
@@ -832,7 +979,7 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
 // partials in a fixed order; with several slices the last block to finish
 // (release / acquire on a per-replica counter) adds the slice sums in slice
 // order, so the result does not depend on which block finishes last.  Then one
-// thread advances the replica's controls.
+// thread advances the replica's controls from LDS-staged inputs.
 template <int BS>
 __global__ __launch_bounds__(BS) void k_finalize(DevState d, int mode, int S) {
     const int sl = blockIdx.x;
@@ -841,7 +988,8 @@ __global__ __launch_bounds__(BS) void k_finalize(DevState d, int mode, int S) {
     RepCtrl* Cp = d.ctrl + r;
     const uint32_t f0 = Cp->flags;
     if (mode == M_STEP && (f0 & F_EXIT)) return;
-    const bool reduce = mode == M_REDUCE || (mode == M_STEP && !(f0 & (F_NULL | F_SPLIT_A)));
+    const bool reduce =
+        mode == M_REDUCE || (mode == M_STEP && !(f0 & (F_NULL | F_SPLIT_A | F_BUILD)));
     __shared__ int s_last;
     if (!reduce && sl != 0) return;
     if (reduce) {
@@ -900,68 +1048,91 @@ __global__ __launch_bounds__(BS) void k_finalize(DevState d, int mode, int S) {
         }
     }
     if (mode == M_REDUCE) return;
+    // Stage everything the control update reads into LDS with one round of
+    // parallel loads: the update itself is a serial chain on one lane and
+    // would otherwise wait on each global load in turn.
+    __shared__ RepCtrl s_ctrl;
+    __shared__ RepParams s_par;
+    __shared__ GroupInfo s_grp[kMaxTypes];
+    __shared__ double s_sums[kMaxTypes * NS];
     __syncthreads();
-    if (threadIdx.x == 0) {
-        RepCtrl C = *Cp;
-        advance(C, d.params[r], R, d.groups + R.first_group, d.sums + (int64_t)R.first_group * NS,
-                mode);
-        *Cp = C;
+    {
+        const int T = R.n_types;
+        const int nc = (int)(sizeof(RepCtrl) / 4), np = (int)(sizeof(RepParams) / 4);
+        const int ng = (int)(sizeof(GroupInfo) / 4) * T;
+        const int tid = threadIdx.x;
+        if (tid < nc) ((uint32_t*)&s_ctrl)[tid] = ((const uint32_t*)Cp)[tid];
+        if (tid >= 128 && tid - 128 < np)
+            ((uint32_t*)&s_par)[tid - 128] = ((const uint32_t*)(d.params + r))[tid - 128];
+        if (tid >= 256 && tid - 256 < ng)
+            ((uint32_t*)s_grp)[tid - 256] = ((const uint32_t*)(d.groups + R.first_group))[tid - 256];
+        if (tid >= 384 && tid - 384 < T * NS)
+            s_sums[tid - 384] = d.sums[(int64_t)R.first_group * NS + (tid - 384)];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) advance(s_ctrl, s_par, R, s_grp, s_sums, mode);
+    __syncthreads();
+    {
+        const int nc = (int)(sizeof(RepCtrl) / 4);
+        if ((int)threadIdx.x < nc) ((uint32_t*)Cp)[threadIdx.x] = ((const uint32_t*)&s_ctrl)[threadIdx.x];
     }
 }
 
 // ---------------------------------------------------------------- small helpers
 
-template <int BS>
-__global__ __launch_bounds__(BS) void k_scale(DevState d, int64_t start, int64_t count,
-                                              double fac, int which) {
-    const int64_t i = start + (int64_t)blockIdx.x * BS + threadIdx.x;
+template <typename FT>
+__global__ __launch_bounds__(256) void k_scale(DevState d, int64_t start, int64_t count,
+                                               double fac, int which) {
+    const int64_t i = start + (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= start + count) return;
-    const int cur = d.ctrl[0].cur;
+    Rec<FT>& a = ((Rec<FT>*)d.rec[d.ctrl[0].cur])[i];
     if (which == 0) {
-        d.vx[cur][i] *= fac;
-        d.vy[cur][i] *= fac;
-        d.vz[cur][i] *= fac;
+        a.vx *= fac;
+        a.vy *= fac;
+        a.vz *= fac;
     } else {
-        d.x[cur][i] *= fac;
-        d.y[cur][i] *= fac;
-        d.z[cur][i] *= fac;
+        a.x *= fac;
+        a.y *= fac;
+        a.z *= fac;
     }
 }
 
-template <int BS>
+template <typename FT, int BS>
 __global__ __launch_bounds__(BS) void k_wrap(DevState d) {
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
     const RepCtrl& C = d.ctrl[G.rep];
     const int64_t s = (int64_t)b * BS + threadIdx.x;
     if ((s - G.start) >= G.count) return;
-    const int cur = C.cur;
-    d.x[cur][s] = wrap1(d.x[cur][s], C.box[0]);
-    d.y[cur][s] = wrap1(d.y[cur][s], C.box[1]);
-    d.z[cur][s] = wrap1(d.z[cur][s], C.box[2]);
+    Rec<FT>& a = ((Rec<FT>*)d.rec[C.cur])[s];
+    a.x = wrap1(a.x, C.box[0]);
+    a.y = wrap1(a.y, C.box[1]);
+    a.z = wrap1(a.z, C.box[2]);
 }
 
 // ---------------------------------------------------------------- launchers
 
-#define MD_BS_SWITCH(bs, F)                 \
-    switch (bs) {                           \
-        case 64: F(64); break;              \
-        case 128: F(128); break;            \
-        case 256: F(256); break;            \
-        case 512: F(512); break;            \
+#define MD_BS_SWITCH(bs, F)                   \
+    switch (bs) {                             \
+        case 64: F(64); break;                \
+        case 128: F(128); break;              \
+        case 256: F(256); break;              \
+        case 512: F(512); break;              \
         default: return hipErrorInvalidValue; \
+    }
+
+#define MD_LAUNCH_FT_MULTI(K, B)                                                               \
+    if (fp32) {                                                                                \
+        if (multi) hipLaunchKernelGGL((K<float, B, true>), dim3(d.n_blocks), dim3(B), 0, st, d); \
+        else hipLaunchKernelGGL((K<float, B, false>), dim3(d.n_blocks), dim3(B), 0, st, d);      \
+    } else {                                                                                   \
+        if (multi) hipLaunchKernelGGL((K<double, B, true>), dim3(d.n_blocks), dim3(B), 0, st, d); \
+        else hipLaunchKernelGGL((K<double, B, false>), dim3(d.n_blocks), dim3(B), 0, st, d);      \
     }
 
 hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st) {
     const bool multi = d.n_types > 1;
-#define L(B)                                                                                  \
-    if (fp32) {                                                                               \
-        if (multi) hipLaunchKernelGGL((k_step<float, B, true>), dim3(d.n_blocks), dim3(B), 0, st, d); \
-        else hipLaunchKernelGGL((k_step<float, B, false>), dim3(d.n_blocks), dim3(B), 0, st, d);      \
-    } else {                                                                                  \
-        if (multi) hipLaunchKernelGGL((k_step<double, B, true>), dim3(d.n_blocks), dim3(B), 0, st, d); \
-        else hipLaunchKernelGGL((k_step<double, B, false>), dim3(d.n_blocks), dim3(B), 0, st, d);      \
-    }
+#define L(B) MD_LAUNCH_FT_MULTI(k_step, B)
     MD_BS_SWITCH(bs, L)
 #undef L
     return hipGetLastError();
@@ -969,14 +1140,7 @@ hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st) {
 
 hipError_t launch_force_eval(const DevState& d, bool fp32, int bs, hipStream_t st) {
     const bool multi = d.n_types > 1;
-#define L(B)                                                                                        \
-    if (fp32) {                                                                                     \
-        if (multi) hipLaunchKernelGGL((k_force_eval<float, B, true>), dim3(d.n_blocks), dim3(B), 0, st, d); \
-        else hipLaunchKernelGGL((k_force_eval<float, B, false>), dim3(d.n_blocks), dim3(B), 0, st, d);      \
-    } else {                                                                                        \
-        if (multi) hipLaunchKernelGGL((k_force_eval<double, B, true>), dim3(d.n_blocks), dim3(B), 0, st, d); \
-        else hipLaunchKernelGGL((k_force_eval<double, B, false>), dim3(d.n_blocks), dim3(B), 0, st, d);      \
-    }
+#define L(B) MD_LAUNCH_FT_MULTI(k_force_eval, B)
     MD_BS_SWITCH(bs, L)
 #undef L
     return hipGetLastError();
@@ -1010,15 +1174,17 @@ hipError_t launch_allpairs(const DevState& d, int bs, bool force_only, hipStream
     return hipGetLastError();
 }
 
-hipError_t launch_insert_all(const DevState& d, int bs, hipStream_t st) {
-#define L(B) hipLaunchKernelGGL((k_insert_all<B>), dim3(d.n_blocks), dim3(B), 0, st, d);
+hipError_t launch_insert_all(const DevState& d, bool fp32, int bs, hipStream_t st) {
+#define L(B)                                                                               \
+    if (fp32) hipLaunchKernelGGL((k_insert_all<float, B>), dim3(d.n_blocks), dim3(B), 0, st, d); \
+    else hipLaunchKernelGGL((k_insert_all<double, B>), dim3(d.n_blocks), dim3(B), 0, st, d);
     MD_BS_SWITCH(bs, L)
 #undef L
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_state(const DevState& d, bool fp32, int bs, hipStream_t st) {
-#define L(B)                                                                            \
+#define L(B)                                                                                 \
     if (fp32) hipLaunchKernelGGL((k_reduce_state<float, B>), dim3(d.n_blocks), dim3(B), 0, st, d); \
     else hipLaunchKernelGGL((k_reduce_state<double, B>), dim3(d.n_blocks), dim3(B), 0, st, d);
     MD_BS_SWITCH(bs, L)
@@ -1026,17 +1192,19 @@ hipError_t launch_reduce_state(const DevState& d, bool fp32, int bs, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_scale(const DevState& d, int bs, int64_t start, int64_t count, double fac,
+hipError_t launch_scale(const DevState& d, bool fp32, int64_t start, int64_t count, double fac,
                         int which, hipStream_t st) {
     if (count <= 0) return hipSuccess;
     const unsigned nb = (unsigned)((count + 255) / 256);
-    hipLaunchKernelGGL((k_scale<256>), dim3(nb), dim3(256), 0, st, d, start, count, fac, which);
-    (void)bs;
+    if (fp32) hipLaunchKernelGGL((k_scale<float>), dim3(nb), dim3(256), 0, st, d, start, count, fac, which);
+    else hipLaunchKernelGGL((k_scale<double>), dim3(nb), dim3(256), 0, st, d, start, count, fac, which);
     return hipGetLastError();
 }
 
-hipError_t launch_wrap(const DevState& d, int bs, hipStream_t st) {
-#define L(B) hipLaunchKernelGGL((k_wrap<B>), dim3(d.n_blocks), dim3(B), 0, st, d);
+hipError_t launch_wrap(const DevState& d, bool fp32, int bs, hipStream_t st) {
+#define L(B)                                                                         \
+    if (fp32) hipLaunchKernelGGL((k_wrap<float, B>), dim3(d.n_blocks), dim3(B), 0, st, d); \
+    else hipLaunchKernelGGL((k_wrap<double, B>), dim3(d.n_blocks), dim3(B), 0, st, d);
     MD_BS_SWITCH(bs, L)
 #undef L
     return hipGetLastError();
