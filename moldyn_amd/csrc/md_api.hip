@@ -26,7 +26,7 @@ static thread_local std::string g_error;
 
 namespace {
 
-enum { K_STEP = 0, K_FINALIZE = 1, K_ALLPAIRS = 2, K_DRIFT = 3, K_NUM = 4 };
+enum { K_STEP = 0, K_FINALIZE = 1, K_ALLPAIRS = 2, K_DRIFT = 3, K_BUILD = 4, K_NUM = 5 };
 
 struct EvPair {
     hipEvent_t a, b;
@@ -72,10 +72,11 @@ struct md_ctx {
     bool prof = false;
     std::vector<EvPair> ev_pool;
     size_t ev_used = 0;
-    double ktime[K_NUM] = {0, 0, 0, 0};
-    int64_t kcount[K_NUM] = {0, 0, 0, 0};
+    double ktime[K_NUM] = {0, 0, 0, 0, 0};
+    int64_t kcount[K_NUM] = {0, 0, 0, 0, 0};
     md_stats stats{};
     int64_t last_rebuilds = 0, last_refresh = 0;
+    bool ctrl_valid = false;  // ctrl holds epochs of fills into the current grid buffers
     std::string err;
 };
 
@@ -269,6 +270,7 @@ static int auto_capacities(md_ctx* c) {
 static int alloc_lists(md_ctx* c) {
     free_list(c->list_allocs);
     drop_graph(c);
+    c->ctrl_valid = false;  // cleared headers: no old epoch can match
     void* p;
     int rc;
     if (!c->list_mode) {
@@ -478,6 +480,9 @@ static hipError_t launch_once(md_ctx* c) {
     hipError_t e;
     EvPair* ev;
     if (c->list_mode) {
+        if ((e = ev_begin(c, K_BUILD, &ev))) return e;
+        if ((e = launch_build(c->d, c->fp32, c->bs, c->st))) return e;
+        if ((e = ev_end(c, ev))) return e;
         if ((e = ev_begin(c, K_STEP, &ev))) return e;
         if ((e = launch_step(c->d, c->fp32, c->bs, c->st))) return e;
         if ((e = ev_end(c, ev))) return e;
@@ -594,7 +599,7 @@ static int list_force_eval(md_ctx* c) {
         if ((rc = reset_stats(c))) return rc;
         if (c->n_blocks > 0) {
             HIPCHK(c, launch_insert_all(c->d, c->fp32, c->bs, c->st));
-            HIPCHK(c, launch_step(c->d, c->fp32, c->bs, c->st));  // F_BUILD: lists only
+            HIPCHK(c, launch_build(c->d, c->fp32, c->bs, c->st));
             HIPCHK(c, launch_force_eval(c->d, c->fp32, c->bs, c->st));
         }
         HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->d.fin_slices, c->st));
@@ -978,9 +983,19 @@ int md_upload_state(md_ctx* c, const double* x, const double* v, const double* b
         if (rc) return rc;
     }
     if ((rc = put_pv(c, nullptr, nullptr))) return rc;
+    // the grid buffers keep the headers of earlier fills: continue the epoch
+    // count past every epoch used so far so that none of them can match again
+    uint32_t epoch0 = 0;
+    if (c->ctrl_valid) {
+        std::vector<RepCtrl> old;
+        if ((rc = read_ctrl(c, old))) return rc;
+        for (const RepCtrl& X : old) epoch0 = std::max(epoch0, std::max(X.fill_epoch, X.ins_epoch));
+        epoch0 += 2u;
+    }
     std::vector<RepCtrl> C((size_t)c->R);
     for (int r = 0; r < c->R; ++r) {
         RepCtrl X{};
+        X.fill_epoch = X.ins_epoch = epoch0;
         for (int k = 0; k < 3; ++k) X.box[k] = box[3 * r + k];
         X.lambda = X.myu = 1.0;
         X.lambda_used = X.myu_used = 1.0;
@@ -993,6 +1008,7 @@ int md_upload_state(md_ctx* c, const double* x, const double* v, const double* b
     }
     rc = write_ctrl(c, C);
     if (rc) return rc;
+    c->ctrl_valid = true;
     c->sums_valid = false;
     c->stats.rebuilds = 0;
     c->stats.refresh_steps = 0;

@@ -230,6 +230,7 @@ constexpr uint32_t kColsAll = (1u << NS) - 1u;
 
 // host-side launchers (md_kernels.hip)
 hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st);
+hipError_t launch_build(const DevState& d, bool fp32, int bs, hipStream_t st);
 hipError_t launch_finalize(const DevState& d, int mode, int n_reps, int slices, hipStream_t st);
 hipError_t launch_drift(const DevState& d, int bs, hipStream_t st);
 hipError_t launch_allpairs(const DevState& d, int bs, bool force_only, hipStream_t st);

@@ -453,10 +453,7 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
         if (valid) grid_insert(d, C, R, s, rin[s].x, rin[s].y, rin[s].z);
         return;
     }
-    if (flags & F_BUILD) {
-        brick_build<FT, BS, MULTI>(d, C, P, R, b - R.first_block);
-        return;
-    }
+    if (flags & F_BUILD) return;  // k_build's launch
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double dt = P.dt, lam = C.lambda, mu = C.myu;
     const double m = G.mass;
@@ -577,6 +574,19 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
     }
     // a split step stored its kick / drift columns in the first half
     block_store<BS>(part, part_out, pstride, split ? kColsState : (kColsState | kick_cols | kColsDrift));
+}
+
+// List build launch: runs before k_step every step and returns at once unless
+// its replica is flagged F_BUILD (a separate kernel so that the brick build's
+// registers and LDS do not lower the occupancy of k_step).
+template <typename FT, int BS, bool MULTI>
+__global__ __launch_bounds__(BS) void k_build(DevState d) {
+    const int b = blockIdx.x;
+    const int r = d.groups[d.blk_group[b]].rep;
+    const RepCtrl& C = d.ctrl[r];
+    if ((C.flags & (F_BUILD | F_EXIT)) != F_BUILD) return;
+    const RepInfo R = d.reps[r];
+    brick_build<FT, BS, MULTI>(d, C, d.params[r], R, b - R.first_block);
 }
 
 // Force evaluation at the current positions through the current lists
@@ -1133,6 +1143,14 @@ __global__ __launch_bounds__(BS) void k_wrap(DevState d) {
 hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st) {
     const bool multi = d.n_types > 1;
 #define L(B) MD_LAUNCH_FT_MULTI(k_step, B)
+    MD_BS_SWITCH(bs, L)
+#undef L
+    return hipGetLastError();
+}
+
+hipError_t launch_build(const DevState& d, bool fp32, int bs, hipStream_t st) {
+    const bool multi = d.n_types > 1;
+#define L(B) MD_LAUNCH_FT_MULTI(k_build, B)
     MD_BS_SWITCH(bs, L)
 #undef L
     return hipGetLastError();

@@ -382,3 +382,57 @@ def test_standalone_thermostat_barostat_api():
     O.barostat_update(oba, s, 0)
     d = e.download()
     assert np.array_equal(d["x"], s.pos) and np.array_equal(d["box"], s.box)
+
+
+@pytest.mark.parametrize("mode", ["fp64_list", "fp32_list"])
+def test_large_gas_interior_bricks(mode):
+    """A box much wider than r_list: cell bricks with halos that wrap the
+    periodic faces (the small fixtures have single-brick grids)."""
+    s = gas_state(6000, 30.0, 21)
+    e = engine_for(s, mode)
+    e.update_force()
+    d = e.download()
+    O.update_force_cells(s)
+    assert np.count_nonzero(s.force) > 1000
+    if mode == "fp64_list":
+        assert np.array_equal(d["force"], s.force)
+        assert np.array_equal(d["virial"], s.temp)
+    else:
+        assert rms(d["force"] - s.force) <= 1e-5 * rms(s.force)
+        assert np.array_equal(d["virial"] != 0, s.temp != 0)
+
+
+def test_dense_bricks_fall_back_to_grid_scan():
+    """Liquid argon with ~32 atoms per cell: brick halos exceed the LDS
+    staging capacity and lists are built straight from the grid."""
+    s = liquid_state(10)
+    e = engine_for(s, "fp64_list")
+    e.update_force()
+    d = e.download()
+    O.update_force_cells(s)
+    assert np.array_equal(d["force"], s.force)
+
+
+@pytest.mark.parametrize("mode", ["fp64_list", "fp32_list"])
+def test_reupload_into_used_engine(mode):
+    """Grid buffers keep the headers of earlier fills: a new state uploaded
+    into a used engine must not see any of them."""
+    a = gas_state(3000, 12.0, 31)
+    b = gas_state(3000, 12.0, 32)
+    e = engine_for(a, mode)
+    e.update_force()
+    e.step(0.002, 300)
+    e.upload(b.pos, b.vel, b.box)
+    e.update_force()
+    d = e.download()
+    O.update_force_cells(b)
+    if mode == "fp64_list":
+        assert np.array_equal(d["force"], b.force)
+    else:
+        assert rms(d["force"] - b.force) <= 1e-5 * rms(b.force)
+    # and a trajectory from the new state equals a fresh engine's
+    e.step(0.002, 200)
+    f = engine_for(b, mode)
+    f.update_force()
+    f.step(0.002, 200)
+    assert np.array_equal(e.download()["x"], f.download()["x"])
