@@ -54,6 +54,7 @@ struct md_ctx {
     std::vector<GroupInfo> groups;
     std::vector<RepInfo> reps;
     std::vector<int32_t> blk_group;
+    std::vector<int32_t> bld_rep;
     // device
     DevState d{};
     std::vector<void*> base_allocs;
@@ -244,6 +245,16 @@ static void make_layout(md_ctx* c) {
     c->n_slots = slot;
     c->n_blocks = block;
     c->total_cells = cell_base;
+    // k_build grid: about four blocks per CU in total, shared out by atom count
+    const int want = std::max(1, std::min(block, 1024));
+    c->bld_rep.clear();
+    for (int r = 0; r < c->R; ++r) {
+        RepInfo& R = c->reps[r];
+        const double share = c->n_atoms ? (double)R.n_atoms / (double)c->n_atoms : 1.0 / c->R;
+        R.bld_first = (int32_t)c->bld_rep.size();
+        R.bld_count = std::max(1, (int)(share * want));
+        for (int k = 0; k < R.bld_count; ++k) c->bld_rep.push_back(r);
+    }
 }
 
 static int auto_capacities(md_ctx* c) {
@@ -335,6 +346,10 @@ static int alloc_base(md_ctx* c) {
     if (!c->blk_group.empty())
         HIPCHK(c, hipMemcpy(p, c->blk_group.data(), sizeof(int32_t) * c->blk_group.size(),
                             hipMemcpyHostToDevice));
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(int32_t) * c->bld_rep.size()))) return rc;
+    d.bld_rep = (const int32_t*)p;
+    HIPCHK(c, hipMemcpy(p, c->bld_rep.data(), sizeof(int32_t) * c->bld_rep.size(), hipMemcpyHostToDevice));
+    d.n_bld = (int32_t)c->bld_rep.size();
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(GroupInfo) * c->groups.size()))) return rc;
     d.groups = (const GroupInfo*)p;
     HIPCHK(c, hipMemcpy(p, c->groups.data(), sizeof(GroupInfo) * c->groups.size(),
@@ -362,9 +377,6 @@ static int alloc_base(md_ctx* c) {
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * c->groups.size() * d.fin_slices)))
         return rc;
     d.part2 = (double*)p;
-    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(unsigned int) * c->R))) return rc;
-    d.fin_count = (unsigned int*)p;
-    HIPCHK(c, hipMemset(p, 0, sizeof(unsigned int) * c->R));
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * c->groups.size()))) return rc;
     d.sums = (double*)p;
     HIPCHK(c, hipMemset(p, 0, sizeof(double) * NS * c->groups.size()));

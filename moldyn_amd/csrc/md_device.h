@@ -95,6 +95,7 @@ struct RepInfo {
     int64_t start, end;  // slot range
     int32_t first_group, n_types;
     int32_t first_block, nblocks;
+    int32_t bld_first, bld_count;  // k_build blocks of this replica
     int64_t cell_base;   // first cell of this replica in each grid buffer
     int64_t cell_budget; // cells available
     int64_t n_atoms;
@@ -151,6 +152,7 @@ struct DevState {
     int32_t* cell_slots[2];
     int32_t cell_cap;
     const int32_t* blk_group;
+    const int32_t* bld_rep;  // k_build block -> replica
     const GroupInfo* groups;
     const RepInfo* reps;
     RepCtrl* ctrl;
@@ -159,11 +161,11 @@ struct DevState {
     const PairParam* pairs;  // n_types x n_types
     double* partial;         // NS x n_blocks (column-major)
     double* part2;           // n_groups x fin_slices x NS, finalize slice sums
-    unsigned int* fin_count; // per replica, finalize slices done (last one resets)
     double* sums;            // n_groups x NS
     int32_t n_types;
     int32_t n_blocks;
     int32_t fin_slices;
+    int32_t n_bld;           // k_build grid size
     int64_t n_slots;
 };
 
@@ -178,8 +180,9 @@ static_assert(kMaxTypes * NS <= 128, "sums staging");
 
 // LDS capacity of one brick: halo cells and staged particles
 constexpr int kBrickCells = 1000;  // (8 + 2)^3
-constexpr int kBrickAtoms = 448;
-constexpr int kBrickTarget = 224;  // expected staged particles a brick is sized for
+constexpr int kBrickAtoms = 1024;
+constexpr int kBrickTarget = 600;  // expected staged particles a brick is sized for
+constexpr int kListReg = 16;       // list entries kept (sorted) in registers while building
 
 // Cell grid of a fill: per axis floor(L / (r_list (1 + 1e-9))) cells in
 // [1, 1024], coarsened until the replica's cell budget holds.  Cell edges stay

@@ -236,6 +236,46 @@ __device__ __forceinline__ void list_close(const DevState& d, int64_t slot, int 
     }
 }
 
+// A list under construction: the first kListReg entries stay sorted in
+// registers (compile-time indexed insertion), longer lists continue in global
+// memory.  close() stores the register part and the count.
+struct RegList {
+    int32_t v[kListReg];
+    int n;
+    int32_t* __restrict__ col;
+    int64_t N;
+    int cap;
+    __device__ __forceinline__ RegList(int32_t* c, int64_t stride, int capacity)
+        : n(0), col(c), N(stride), cap(capacity) {}
+    __device__ __forceinline__ void push(int32_t j) {
+        if (n < kListReg) {
+#pragma unroll
+            for (int p = kListReg - 1; p >= 0; --p) {
+                if (p <= n) {
+                    const bool keep = (p < n) && (v[p] < j);
+                    const bool here = p > 0 ? (v[p > 0 ? p - 1 : 0] < j) : true;
+                    v[p] = keep ? v[p] : (here ? j : v[p > 0 ? p - 1 : 0]);
+                }
+            }
+        } else {
+            if (n == kListReg) {
+#pragma unroll
+                for (int p = 0; p < kListReg; ++p)
+                    if (p < cap) col[(int64_t)p * N] = v[p];
+            }
+            list_push(col, N, n, cap, j);
+        }
+        ++n;
+    }
+    __device__ __forceinline__ void store() {
+        if (n <= kListReg) {
+#pragma unroll
+            for (int p = 0; p < kListReg; ++p)
+                if (p < n && p < cap) col[(int64_t)p * N] = v[p];
+        }
+    }
+};
+
 // Verlet list of one particle straight from the grid (fallback when a brick
 // does not fit in LDS): every j of the same replica, type >= own type
 // (potential.rs:171-180), j != slot, within rlist under the reference
@@ -325,7 +365,7 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
     __shared__ int32_t s_id[kBrickAtoms];
     __shared__ uint16_t s_hc[kBrickAtoms];
     __shared__ uint8_t s_ty[MULTI ? kBrickAtoms : 1];
-    for (int bi = block_in_rep; bi < nbricks; bi += R.nblocks) {
+    for (int bi = block_in_rep; bi < nbricks; bi += R.bld_count) {
         const int bc[3] = {bi / (nbk[1] * nbk[2]), (bi / nbk[2]) % nbk[1], bi % nbk[2]};
         int org[3], ncen[3];
         for (int k = 0; k < 3; ++k) {
@@ -400,8 +440,7 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             const int32_t i = s_id[a];
             const int ti = MULTI ? (int)s_ty[a] : 0;
             const double xi = s_x[a], yi = s_y[a], zi = s_z[a];
-            int32_t* __restrict__ col = d.nbr + i;
-            int n = 0;
+            RegList lst(d.nbr + i, N, d.list_cap);
             for (int o0 = lo[0]; o0 <= hi[0]; ++o0) {
                 const int g0 = ((whole & 1) ? (hc[0] + o0 + gd[0]) % gd[0] : hc[0] + o0);
                 for (int o1 = lo[1]; o1 <= hi[1]; ++o1) {
@@ -418,13 +457,13 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
                             const double dz = min_image(s_z[b2] - zi, bz);
                             const double r2 = (dx * dx + dy * dy) + dz * dz;
                             if (!(r2 <= rl2)) continue;
-                            list_push(col, N, n, d.list_cap, j);
-                            ++n;
+                            lst.push(j);
                         }
                     }
                 }
             }
-            list_close(d, i, n);
+            lst.store();
+            list_close(d, i, lst.n);
         }
         __syncthreads();
     }
@@ -582,11 +621,11 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
 template <typename FT, int BS, bool MULTI>
 __global__ __launch_bounds__(BS) void k_build(DevState d) {
     const int b = blockIdx.x;
-    const int r = d.groups[d.blk_group[b]].rep;
+    const int r = d.bld_rep[b];
     const RepCtrl& C = d.ctrl[r];
     if ((C.flags & (F_BUILD | F_EXIT)) != F_BUILD) return;
     const RepInfo R = d.reps[r];
-    brick_build<FT, BS, MULTI>(d, C, d.params[r], R, b - R.first_block);
+    brick_build<FT, BS, MULTI>(d, C, d.params[r], R, b - R.bld_first);
 }
 
 // Force evaluation at the current positions through the current lists
@@ -985,76 +1024,81 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
     }
 }
 
-// grid (slices, replicas).  Each block reduces its slice of a group's block
-// partials in a fixed order; with several slices the last block to finish
-// (release / acquire on a per-replica counter) adds the slice sums in slice
-// order, so the result does not depend on which block finishes last.  Then one
-// thread advances the replica's controls from LDS-staged inputs.
+// Which launches produced block partials to reduce.
+__device__ __forceinline__ bool finalize_reduces(int mode, uint32_t f0) {
+    return mode == M_REDUCE || (mode == M_STEP && !(f0 & (F_EXIT | F_NULL | F_SPLIT_A | F_BUILD)));
+}
+
+// Sum rows [lo, hi) of a group's column-major block partials in a fixed order
+// (thread-strided, four rows in flight per thread, then the block tree).
 template <int BS>
-__global__ __launch_bounds__(BS) void k_finalize(DevState d, int mode, int S) {
+__device__ void reduce_rows(const DevState& d, const GroupInfo& G, int lo, int hi, double* out) {
+    const int64_t nb = d.n_blocks;
+    double acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+    for (int b0 = lo + threadIdx.x; b0 < hi; b0 += 4 * BS) {
+        double v[4][NS];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int bb = b0 + u * BS;
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+                v[u][k] = bb < hi ? d.partial[k * nb + G.first_block + bb] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+                acc[k] = k >= kFirstMax ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
+    }
+    block_store<BS>(acc, out, 1, kColsAll);
+}
+
+// First level of a sliced reduction: grid (slices, replicas); slice sums go
+// to part2 and the finalize adds them in slice order (the kernel boundary
+// orders the two, so no fences are needed).
+template <int BS>
+__global__ __launch_bounds__(BS) void k_reduce_slices(DevState d, int mode, int S) {
     const int sl = blockIdx.x;
     const int r = blockIdx.y;
+    const RepInfo R = d.reps[r];
+    if (!finalize_reduces(mode, d.ctrl[r].flags)) return;
+    for (int t = 0; t < R.n_types; ++t) {
+        const int g = R.first_group + t;
+        const GroupInfo G = d.groups[g];
+        const int lo = (int)((int64_t)G.nblocks * sl / S);
+        const int hi = (int)((int64_t)G.nblocks * (sl + 1) / S);
+        reduce_rows<BS>(d, G, lo, hi, d.part2 + ((int64_t)g * S + sl) * NS);
+    }
+}
+
+// grid (replicas).  Group sums (directly from the block partials, or from the
+// slice sums of k_reduce_slices), then one thread advances the replica's
+// controls from LDS-staged inputs.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_finalize(DevState d, int mode, int S) {
+    const int r = blockIdx.x;
     const RepInfo R = d.reps[r];
     RepCtrl* Cp = d.ctrl + r;
     const uint32_t f0 = Cp->flags;
     if (mode == M_STEP && (f0 & F_EXIT)) return;
-    const bool reduce =
-        mode == M_REDUCE || (mode == M_STEP && !(f0 & (F_NULL | F_SPLIT_A | F_BUILD)));
-    __shared__ int s_last;
-    if (!reduce && sl != 0) return;
-    if (reduce) {
-        const int64_t nb = d.n_blocks;
+    if (finalize_reduces(mode, f0)) {
         for (int t = 0; t < R.n_types; ++t) {
             const int g = R.first_group + t;
-            const GroupInfo G = d.groups[g];
-            const int lo = (int)((int64_t)G.nblocks * sl / S);
-            const int hi = (int)((int64_t)G.nblocks * (sl + 1) / S);
-            double acc[NS];
-#pragma unroll
-            for (int k = 0; k < NS; ++k) acc[k] = 0.0;
-            // column-major partials: consecutive threads read consecutive blocks;
-            // four rows per thread are loaded before any is summed
-            for (int b0 = lo + threadIdx.x; b0 < hi; b0 += 4 * BS) {
-                double v[4][NS];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int bb = b0 + u * BS;
-#pragma unroll
-                    for (int k = 0; k < NS; ++k)
-                        v[u][k] = bb < hi ? d.partial[k * nb + G.first_block + bb] : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int k = 0; k < NS; ++k)
-                        acc[k] = k >= kFirstMax ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
-            }
-            if (S == 1)
-                block_store<BS>(acc, d.sums + (int64_t)g * NS, 1, kColsAll);
-            else
-                block_store<BS>(acc, d.part2 + ((int64_t)g * S + sl) * NS, 1, kColsAll);
-        }
-        if (S > 1) {
-            if (threadIdx.x == 0) {
-                __threadfence();
-                const unsigned int done = atomicAdd(&d.fin_count[r], 1u);
-                s_last = done == (unsigned int)(S - 1);
-            }
-            __syncthreads();
-            if (!s_last) return;
-            __threadfence();
-            const int k = threadIdx.x;
-            if (k < NS) {
-                for (int t = 0; t < R.n_types; ++t) {
-                    const int64_t g = R.first_group + t;
-                    const double* row = d.part2 + g * S * NS;
+            if (S == 1) {
+                const GroupInfo G = d.groups[g];
+                reduce_rows<BS>(d, G, 0, G.nblocks, d.sums + (int64_t)g * NS);
+            } else {
+                const int k = threadIdx.x;
+                if (k < NS) {
+                    const double* row = d.part2 + (int64_t)g * S * NS;
                     double a = row[k];
                     for (int q = 1; q < S; ++q)
                         a = k >= kFirstMax ? fmax(a, row[q * NS + k]) : a + row[q * NS + k];
-                    d.sums[g * NS + k] = a;
+                    d.sums[(int64_t)g * NS + k] = a;
                 }
             }
-            if (threadIdx.x == 0) d.fin_count[r] = 0u;
         }
     }
     if (mode == M_REDUCE) return;
@@ -1131,6 +1175,15 @@ __global__ __launch_bounds__(BS) void k_wrap(DevState d) {
         default: return hipErrorInvalidValue; \
     }
 
+#define MD_LAUNCH_FT_MULTI_G(K, B, G)                                                     \
+    if (fp32) {                                                                            \
+        if (multi) hipLaunchKernelGGL((K<float, B, true>), dim3(G), dim3(B), 0, st, d);      \
+        else hipLaunchKernelGGL((K<float, B, false>), dim3(G), dim3(B), 0, st, d);           \
+    } else {                                                                               \
+        if (multi) hipLaunchKernelGGL((K<double, B, true>), dim3(G), dim3(B), 0, st, d);     \
+        else hipLaunchKernelGGL((K<double, B, false>), dim3(G), dim3(B), 0, st, d);          \
+    }
+
 #define MD_LAUNCH_FT_MULTI(K, B)                                                               \
     if (fp32) {                                                                                \
         if (multi) hipLaunchKernelGGL((K<float, B, true>), dim3(d.n_blocks), dim3(B), 0, st, d); \
@@ -1150,7 +1203,7 @@ hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st) {
 
 hipError_t launch_build(const DevState& d, bool fp32, int bs, hipStream_t st) {
     const bool multi = d.n_types > 1;
-#define L(B) MD_LAUNCH_FT_MULTI(k_build, B)
+#define L(B) MD_LAUNCH_FT_MULTI_G(k_build, B, d.n_bld)
     MD_BS_SWITCH(bs, L)
 #undef L
     return hipGetLastError();
@@ -1166,7 +1219,9 @@ hipError_t launch_force_eval(const DevState& d, bool fp32, int bs, hipStream_t s
 
 hipError_t launch_finalize(const DevState& d, int mode, int n_reps, int slices, hipStream_t st) {
     if (mode == M_PRIME) slices = 1;
-    hipLaunchKernelGGL((k_finalize<512>), dim3(slices, n_reps), dim3(512), 0, st, d, mode, slices);
+    if (slices > 1)
+        hipLaunchKernelGGL((k_reduce_slices<512>), dim3(slices, n_reps), dim3(512), 0, st, d, mode, slices);
+    hipLaunchKernelGGL((k_finalize<512>), dim3(n_reps), dim3(512), 0, st, d, mode, slices);
     return hipGetLastError();
 }
 
