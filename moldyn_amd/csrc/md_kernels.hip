@@ -1030,24 +1030,24 @@ __device__ __forceinline__ bool finalize_reduces(int mode, uint32_t f0) {
 }
 
 // Sum rows [lo, hi) of a group's column-major block partials in a fixed order
-// (thread-strided, four rows in flight per thread, then the block tree).
+// (thread-strided, two rows in flight per thread, then the block tree).
 template <int BS>
 __device__ void reduce_rows(const DevState& d, const GroupInfo& G, int lo, int hi, double* out) {
     const int64_t nb = d.n_blocks;
     double acc[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) acc[k] = 0.0;
-    for (int b0 = lo + threadIdx.x; b0 < hi; b0 += 4 * BS) {
-        double v[4][NS];
+    for (int b0 = lo + threadIdx.x; b0 < hi; b0 += 2 * BS) {
+        double v[2][NS];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 2; ++u) {
             const int bb = b0 + u * BS;
 #pragma unroll
             for (int k = 0; k < NS; ++k)
                 v[u][k] = bb < hi ? d.partial[k * nb + G.first_block + bb] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 2; ++u)
 #pragma unroll
             for (int k = 0; k < NS; ++k)
                 acc[k] = k >= kFirstMax ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
