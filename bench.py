@@ -62,7 +62,8 @@ def main():
     ap.add_argument("--cells-per-atom", type=float, default=2.0)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-steps", type=int, default=200)
-    ap.add_argument("--traffic-json", default=os.path.join(ROOT, "profiles", "step_kernel_traffic.json"))
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_step_1M_npt_traffic.txt"),
+                    help="rocprof_summary traffic file holding bytes_per_launch of the step kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -109,10 +110,8 @@ def main():
     achieved = bytes_per_launch / avg_step_s / 1e9
     log(f"[bench] k_step avg {avg_step_s * 1e6:.2f} us over {step_n} launches, "
         f"k_finalize avg {fin_ms / max(fin_n, 1) * 1e3:.2f} us; <n_l> {n_l:.4f}")
-    traffic = None
-    if os.path.exists(args.traffic_json):
-        with open(args.traffic_json) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+    from moldyn_amd.rocprof_summary import read_traffic
+    traffic = read_traffic(args.traffic)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": f"k_step<float,{opts['block_size']}>", "avg_us": round(avg_step_s * 1e6, 3),

@@ -1,0 +1,176 @@
+"""Text summaries of rocprofv3 output for the measurement records.
+
+Two commands:
+
+  python -m moldyn_amd.rocprof_summary stats DIR --command "..." --out FILE
+      per-kernel duration statistics of a ``rocprofv3 --kernel-trace`` run
+      (SQLite ``*_results.db`` or ``*kernel_trace.csv`` under DIR).
+
+  python -m moldyn_amd.rocprof_summary traffic --kernel REGEX \
+      --fetch DIR --write DIR --copy-fetch DIR --copy-write DIR \
+      --copy-kernel REGEX --copy-bytes B --command "..." --out FILE
+      HBM bytes per launch of the kernels matching REGEX from two ``--pmc``
+      passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by
+      what the same counter reads for a plain copy kernel that moves B bytes in
+      and B bytes out.  Writes ``bytes_per_launch: <int>`` among its lines.
+
+Only the standard library is used; the module reads what rocprofv3 wrote and
+never runs anything.
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import glob
+import os
+import re
+import sqlite3
+import statistics
+
+
+def _kernel_rows(path):
+    """[(name, start_ns, end_ns)] from a kernel-trace directory."""
+    dbs = sorted(glob.glob(os.path.join(path, "**", "*results.db"), recursive=True))
+    rows = []
+    if dbs:
+        for db in dbs:
+            con = sqlite3.connect(db)
+            try:
+                cols = [r[1] for r in con.execute("pragma table_info(kernels)")]
+                name = "name" if "name" in cols else "kernel_name"
+                rows += con.execute(f"select {name}, start, end from kernels").fetchall()
+            finally:
+                con.close()
+        return rows
+    for f in sorted(glob.glob(os.path.join(path, "**", "*kernel_trace.csv"), recursive=True)):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                rows.append((r["Kernel_Name"], int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    if not rows:
+        raise SystemExit(f"no kernel trace found under {path}")
+    return rows
+
+
+def _pmc_rows(path, counter):
+    """{kernel_name: [value per dispatch]} of one counter from a --pmc run."""
+    out = {}
+    files = sorted(glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True))
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {path} (run rocprofv3 with -f csv)")
+    for f in files:
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] != counter:
+                    continue
+                out.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return out
+
+
+def _q(ds, p):
+    return ds[min(len(ds) - 1, int(p * len(ds)))]
+
+
+def stats(args):
+    rows = _kernel_rows(args.dir)
+    by = {}
+    for name, s, e in rows:
+        by.setdefault(name, []).append((e - s) / 1000.0)
+    lines = [f"# rocprofv3 kernel-trace summary: {args.title}", "",
+             "command:", "", f"    {args.command}", "",
+             "durations in microseconds (end - start of each dispatch)", "",
+             f"{'kernel':62s} {'n':>7s} {'total_ms':>10s} {'mean':>9s} {'p10':>9s} {'p50':>9s} "
+             f"{'p90':>9s} {'p99':>9s} {'max':>9s}"]
+    total = sum(sum(v) for v in by.values())
+    for name, ds in sorted(by.items(), key=lambda kv: -sum(kv[1]))[: args.top]:
+        ds.sort()
+        lines.append(f"{name[:62]:62s} {len(ds):7d} {sum(ds) / 1e3:10.3f} {statistics.mean(ds):9.2f} "
+                     f"{_q(ds, .1):9.2f} {_q(ds, .5):9.2f} {_q(ds, .9):9.2f} {_q(ds, .99):9.2f} "
+                     f"{ds[-1]:9.2f}")
+    lines += ["", f"all kernels: {sum(len(v) for v in by.values())} dispatches, {total / 1e3:.3f} ms busy"]
+    text = "\n".join(lines) + "\n"
+    _emit(text, args.out)
+
+
+def _per_dispatch(rows, regex):
+    pat = re.compile(regex)
+    vals = [v for name, vs in rows.items() if pat.search(name) for v in vs]
+    if not vals:
+        raise SystemExit(f"no dispatch of a kernel matching {regex!r}")
+    vals.sort()
+    return vals
+
+
+def traffic(args):
+    fetch = _per_dispatch(_pmc_rows(args.fetch, "FETCH_SIZE"), args.kernel)
+    write = _per_dispatch(_pmc_rows(args.write, "WRITE_SIZE"), args.kernel)
+    cf = _per_dispatch(_pmc_rows(args.copy_fetch, "FETCH_SIZE"), args.copy_kernel)
+    cw = _per_dispatch(_pmc_rows(args.copy_write, "WRITE_SIZE"), args.copy_kernel)
+    kib = 1024.0
+    # counter scale: bytes the copy really moved / bytes the counter reports
+    f_scale = args.copy_bytes / (statistics.median(cf) * kib)
+    w_scale = args.copy_bytes / (statistics.median(cw) * kib)
+    fm, wm = statistics.median(fetch), statistics.median(write)
+    rd = fm * kib * f_scale
+    wr = wm * kib * w_scale
+    lines = [f"# rocprofv3 --pmc HBM traffic: {args.title}", "",
+             "commands:", ""] + [f"    {c}" for c in args.command] + ["",
+             f"kernel regex: {args.kernel}",
+             f"dispatches: FETCH_SIZE {len(fetch)}, WRITE_SIZE {len(write)}",
+             f"FETCH_SIZE KiB per dispatch: median {fm:.1f} (p10 {_q(fetch, .1):.1f}, p90 {_q(fetch, .9):.1f})",
+             f"WRITE_SIZE KiB per dispatch: median {wm:.1f} (p10 {_q(write, .1):.1f}, p90 {_q(write, .9):.1f})",
+             "",
+             f"copy calibration ({args.copy_kernel}, {args.copy_bytes} bytes read and written):",
+             f"  FETCH_SIZE {statistics.median(cf):.1f} KiB -> scale {f_scale:.4f}",
+             f"  WRITE_SIZE {statistics.median(cw):.1f} KiB -> scale {w_scale:.4f}",
+             "",
+             f"read_bytes_per_launch: {rd:.0f}",
+             f"write_bytes_per_launch: {wr:.0f}",
+             f"bytes_per_launch: {rd + wr:.0f}"]
+    _emit("\n".join(lines) + "\n", args.out)
+
+
+def read_traffic(path):
+    """bytes_per_launch from a summary written by ``traffic`` (None if absent)."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    return None
+
+
+def _emit(text, out):
+    if out:
+        os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+        with open(out, "w") as fh:
+            fh.write(text)
+    print(text, end="")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m moldyn_amd.rocprof_summary")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    s.add_argument("--command", required=True)
+    s.add_argument("--title", default="")
+    s.add_argument("--top", type=int, default=25)
+    s.add_argument("--out")
+    t = sub.add_parser("traffic")
+    t.add_argument("--kernel", required=True)
+    t.add_argument("--fetch", required=True)
+    t.add_argument("--write", required=True)
+    t.add_argument("--copy-fetch", required=True)
+    t.add_argument("--copy-write", required=True)
+    t.add_argument("--copy-kernel", required=True)
+    t.add_argument("--copy-bytes", type=float, required=True)
+    t.add_argument("--command", action="append", required=True)
+    t.add_argument("--title", default="")
+    t.add_argument("--out")
+    a = ap.parse_args(argv)
+    (stats if a.cmd == "stats" else traffic)(a)
+
+
+if __name__ == "__main__":
+    main()
