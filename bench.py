@@ -60,8 +60,12 @@ def main():
     ap.add_argument("--skin", type=float, default=None)
     ap.add_argument("--block-size", type=int, default=256)
     ap.add_argument("--cells-per-atom", type=float, default=2.0)
+    ap.add_argument("--tiles-per-block", type=int, default=0)
+    ap.add_argument("--graph-chunk", type=int, default=32, help="launches per captured graph (1: none)")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="no per-launch device timestamps in the timed region (no roofline)")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--tune-steps", type=int, default=200)
+    ap.add_argument("--tune-steps", type=int, default=300)
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_step_1M_npt_traffic.txt"),
                     help="rocprof_summary traffic file holding bytes_per_launch of the step kernel")
     args = ap.parse_args()
@@ -75,19 +79,36 @@ def main():
     n, pos, vel, box = initial_state(M, args.n_side)
     log(f"[bench] state ready: {n} atoms, box {box[0]:.4f}")
     opts = {"skin": args.skin if args.skin is not None else 0.3 * 0.3418,
-            "block_size": args.block_size, "cells_per_atom": args.cells_per_atom}
+            "block_size": args.block_size, "cells_per_atom": args.cells_per_atom,
+            "tiles_per_block": args.tiles_per_block}
     if not args.no_tune:
         from moldyn_amd.autotune import autotune
-        opts, _ = autotune([n], [MASS], box, pos, vel, DT, lambda: controls(M),
+        # tune on the state half-way through the timed run: the barostat
+        # compresses the gas several-fold over the run, which lengthens the
+        # lists and shortens the rebuild interval
+        pre = args.warmup + args.steps // 2
+        e0 = M.Engine([n], [MASS], box, force_mode="fp32_list", skin=0.5, cells_per_atom=4.0)
+        e0.upload(pos, vel, box)
+        e0.update_force()
+        th0, ba0 = controls(M)
+        e0.step(DT, pre, th0, ba0)
+        mid = e0.download(("x", "v", "box"))
+        e0.close()
+        log(f"[bench] tuning at step {pre}: box {mid['box'][0]:.4f}")
+        opts, _ = autotune([n], [MASS], mid["box"], mid["x"], mid["v"], DT, lambda: controls(M),
                            steps=args.tune_steps, warmup=20, log=log)
         log(f"[bench] autotuned options: {opts}")
-    e = M.Engine([n], [MASS], box, force_mode="fp32_list", **opts)
+    e = M.Engine([n], [MASS], box, force_mode="fp32_list", graph_chunk=args.graph_chunk, **opts)
     e.upload(pos, vel, box)
     e.update_force()
     th, ba = controls(M)
     e.step(DT, args.warmup, th, ba)
     log(f"[bench] warmup {args.warmup} steps done")
 
+    # per-launch device timestamps of every kernel in the timed region (the
+    # graphs keep running; each launch is timed from its first block's start
+    # to its last block's end on the GPU clock)
+    e.set_profiling(not args.no_profile)
     t0 = time.perf_counter()
     e.step(DT, args.steps, th, ba)  # returns after the device finished
     t1 = time.perf_counter()
@@ -95,26 +116,23 @@ def main():
     value = n * args.steps / (t1 - t0)
     ms_per_step = 1e3 * (t1 - t0) / args.steps
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value:.4e} atom-steps/s; stats {st}")
-
-    # per-kernel device time with hipEvents around every launch, same workload continued
-    e.set_profiling(True)
-    prof_steps = min(args.steps, 1000)
-    e.step(DT, prof_steps, th, ba)
     step_ms, step_n = e.kernel_time("step")
     fin_ms, fin_n = e.kernel_time("finalize")
     e.set_profiling(False)
-    st2 = e.stats()
     avg_step_s = step_ms / max(step_n, 1) * 1e-3
-    n_l = st2["mean_list_len"]
+    # mean list length over the timed launches (entries read / atoms / launches)
+    n_l = st["list_visits"] / (n * max(step_n, 1)) if step_n else st["mean_list_len"]
     bytes_per_launch = n * (124.0 + 28.0 * n_l)
-    achieved = bytes_per_launch / avg_step_s / 1e9
+    achieved = bytes_per_launch / avg_step_s / 1e9 if step_n else 0.0
     log(f"[bench] k_step avg {avg_step_s * 1e6:.2f} us over {step_n} launches, "
-        f"k_finalize avg {fin_ms / max(fin_n, 1) * 1e3:.2f} us; <n_l> {n_l:.4f}")
+        f"k_fin avg {fin_ms / max(fin_n, 1) * 1e3:.2f} us over {fin_n}; <n_l> {n_l:.4f}")
     from moldyn_amd.rocprof_summary import read_traffic
     traffic = read_traffic(args.traffic)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": f"k_step<float,{opts['block_size']}>", "avg_us": round(avg_step_s * 1e6, 3),
+                "launches_timed": step_n, "mean_list_len_timed": round(n_l, 5),
+                "k_fin_avg_us": round(fin_ms / max(fin_n, 1) * 1e3, 3),
                 "algorithmic_bytes_per_launch": round(bytes_per_launch)}
 
     # FP32 force error on the final state, against the FP64 path (bitwise the

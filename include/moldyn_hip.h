@@ -81,7 +81,9 @@ typedef struct {
     int list_capacity;     /* initial neighbours per atom; 0 = automatic; grown on overflow */
     int cell_capacity;     /* initial atoms per cell slot array; 0 = automatic; grown on overflow */
     double cells_per_atom; /* cell-grid budget per replica, cells per atom; default 2 */
-    int graph_chunk;       /* steps per captured hipGraph; 0 = default 32, 1 = no graphs */
+    int graph_chunk;       /* launches per captured hipGraph (rounded up to even); 0 = default 32, 1 = no graphs */
+    int tiles_per_block;   /* particle tiles (block_size slots each) per block of the per-particle
+                              kernels; 0 = automatic (about four blocks per CU on the list paths) */
 } md_options;
 
 typedef struct md_ctx md_ctx;
@@ -187,11 +189,17 @@ typedef struct {
     double mean_list_len;   /* at the last rebuild */
     int32_t max_list_len;
     int32_t n_slots;        /* padded particle slots on the device */
+    int64_t grows;          /* capacity increases between chunks (no replay) */
+    int64_t list_visits;    /* neighbour-list entries read by the step kernel while profiling */
 } md_stats;
 int md_get_stats(md_ctx* ctx, md_stats* out);
-/* Per-kernel device timing with hipEvents around every launch of the step
- * kernel (kernel 0), the finalize kernel (1), the all-pairs force kernel (2)
- * and the drift kernel (3).  Enabling it disables graph capture. */
+/* Per-kernel device time of every launch of md_step: the step kernel
+ * (kernel 0), the finalize / list-build kernel (1), the all-pairs force kernel
+ * (2) and the drift kernel (3) (kernel 4, the former separate list build,
+ * reports nothing).  Each launch of a kernel is timed on the device from the
+ * start of its first block to the end of its last block with the GPU wall
+ * clock (s_memrealtime), so the captured graphs keep running unchanged.
+ * md_set_profiling(ctx, 1) also clears the totals. */
 int md_set_profiling(md_ctx* ctx, int on);
 int md_kernel_time(md_ctx* ctx, int kernel, double* total_ms, int64_t* launches);
 

@@ -49,7 +49,8 @@ class MdBarostat(C.Structure):
 class MdOptions(C.Structure):
     _fields_ = [("force_mode", C.c_int), ("skin", C.c_double), ("block_size", C.c_int),
                 ("list_capacity", C.c_int), ("cell_capacity", C.c_int),
-                ("cells_per_atom", C.c_double), ("graph_chunk", C.c_int)]
+                ("cells_per_atom", C.c_double), ("graph_chunk", C.c_int),
+                ("tiles_per_block", C.c_int)]
 
 
 class MdStats(C.Structure):
@@ -57,7 +58,7 @@ class MdStats(C.Structure):
                 ("refresh_steps", C.c_int64), ("retries", C.c_int64),
                 ("list_capacity", C.c_int32), ("cell_capacity", C.c_int32),
                 ("mean_list_len", C.c_double), ("max_list_len", C.c_int32),
-                ("n_slots", C.c_int32)]
+                ("n_slots", C.c_int32), ("grows", C.c_int64), ("list_visits", C.c_int64)]
 
 
 class MdError(RuntimeError):

@@ -1,13 +1,15 @@
 // md_api.hip -- host runtime behind include/moldyn_hip.h.
 //
-// A context owns the slot layout, device buffers, one HIP stream and a
-// captured hipGraph of `graph_chunk` step launches.  md_step runs in chunks:
-// snapshot the state (list paths), prime the controls on the device, replay
-// the graph, let replicas that spent launches on list refreshes catch up,
-// then check the device's overflow flags; on overflow the chunk is replayed
-// from the snapshot with larger capacities (results do not depend on when
-// lists are rebuilt, because every list is sorted and pairs outside r_cut
-// contribute nothing).
+// A context owns the slot layout, device buffers, one HIP stream and two
+// captured hipGraphs of `graph_chunk` step launches (one per parity of the
+// double-buffered control blocks).  md_step runs in chunks: snapshot the
+// state (list paths), prime the controls on the device, replay the graph,
+// let replicas that spent launches on list refreshes catch up, then check
+// the device's overflow flags; on overflow the chunk is replayed from the
+// snapshot with larger capacities (results do not depend on when lists are
+// rebuilt, because every list is sorted and pairs outside r_cut contribute
+// nothing).  Capacities that come close to overflowing are grown between
+// chunks without a replay.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -27,11 +29,6 @@ static thread_local std::string g_error;
 namespace {
 
 enum { K_STEP = 0, K_FINALIZE = 1, K_ALLPAIRS = 2, K_DRIFT = 3, K_BUILD = 4, K_NUM = 5 };
-
-struct EvPair {
-    hipEvent_t a, b;
-    int kernel;
-};
 
 }  // namespace
 
@@ -66,15 +63,21 @@ struct md_ctx {
     bool sums_valid = false;
     double last_dt = 0.002;
     double rlist = 0.0, rc_max = 0.0;
-    // graphs
-    hipGraphExec_t gexec = nullptr;
-    int g_len = 0;
-    // profiling
+    int par = 0;       // control buffer the next launch reads
+    int snap_par = 0;  // ... at the snapshot
+    int n_cu = 256;
+    int tpb = 1;
+    // graphs, by the parity they start at
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int g_len[2] = {0, 0};
+    // profiling: per-launch device timestamps
     bool prof = false;
-    std::vector<EvPair> ev_pool;
-    size_t ev_used = 0;
-    double ktime[K_NUM] = {0, 0, 0, 0, 0};
-    int64_t kcount[K_NUM] = {0, 0, 0, 0, 0};
+    unsigned long long* prof_ring = nullptr;
+    unsigned long long* prof_blk = nullptr;
+    double wall_hz = 1e8;
+    double ktime[P_NUM] = {0, 0, 0};
+    int64_t kcount[P_NUM] = {0, 0, 0};
+    int64_t list_visits = 0;
     md_stats stats{};
     int64_t last_rebuilds = 0, last_refresh = 0;
     bool ctrl_valid = false;  // ctrl holds epochs of fills into the current grid buffers
@@ -186,9 +189,11 @@ static void free_list(std::vector<void*>& v) {
 }
 
 static void drop_graph(md_ctx* c) {
-    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
-    c->g_len = 0;
+    for (int k = 0; k < 2; ++k) {
+        if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+        c->gexec[k] = nullptr;
+        c->g_len[k] = 0;
+    }
 }
 
 static void free_all(md_ctx* c) {
@@ -200,10 +205,31 @@ static void free_all(md_ctx* c) {
     c->snap_bytes = 0;
     if (c->snap_ctrl) (void)hipFree(c->snap_ctrl);
     c->snap_ctrl = nullptr;
+    if (c->prof_ring) (void)hipFree(c->prof_ring);
+    c->prof_ring = nullptr;
+    if (c->prof_blk) (void)hipFree(c->prof_blk);
+    c->prof_blk = nullptr;
+    c->d.prof = nullptr;
+    c->d.prof_blk = nullptr;
+    c->prof = false;  // a new layout starts unprofiled
+}
+
+// Tiles per block of the per-slot kernels: the option, else 1 on the
+// all-pairs path (compute-bound, wants every block it can get) and on the
+// list paths about four blocks per CU, at most 16 tiles each.
+static int tiles_per_block(md_ctx* c) {
+    if (c->opt.tiles_per_block > 0) return c->opt.tiles_per_block;
+    if (c->opt.force_mode == MD_FORCE_FP64_ALLPAIRS) return 1;
+    int64_t tiles = 0;
+    for (int r = 0; r < c->R; ++r)
+        for (int t = 0; t < c->T; ++t) tiles += (c->counts[(size_t)r * c->T + t] + c->bs - 1) / c->bs;
+    const int64_t want = 4 * (int64_t)c->n_cu;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(16, (tiles + want - 1) / want));
 }
 
 static void make_layout(md_ctx* c) {
     const int bs = c->bs;
+    c->tpb = tiles_per_block(c);
     c->groups.clear();
     c->reps.clear();
     c->blk_group.clear();
@@ -220,14 +246,15 @@ static void make_layout(md_ctx* c) {
             GroupInfo G{};
             G.start = slot;
             G.count = c->counts[(size_t)r * c->T + t];
-            G.nblocks = (int32_t)((G.count + bs - 1) / bs);
+            const int64_t tiles = (G.count + bs - 1) / bs;
+            G.nblocks = (int32_t)((tiles + c->tpb - 1) / c->tpb);
             G.first_block = block;
             G.rep = r;
             G.type = t;
             G.mass = c->masses[t];
             G.sum_m = fold_mass(G.mass, G.count);
             for (int b = 0; b < G.nblocks; ++b) c->blk_group.push_back((int32_t)c->groups.size());
-            slot += (int64_t)G.nblocks * bs;
+            slot += tiles * bs;
             block += G.nblocks;
             R.n_atoms += G.count;
             c->groups.push_back(G);
@@ -245,8 +272,10 @@ static void make_layout(md_ctx* c) {
     c->n_slots = slot;
     c->n_blocks = block;
     c->total_cells = cell_base;
-    // k_build grid: about four blocks per CU in total, shared out by atom count
-    const int want = std::max(1, std::min(block, 1024));
+    // list-build blocks: one per CU in total (one 512-thread block fits a CU), shared out by atom count
+    int64_t tiles = 0;
+    for (const GroupInfo& G : c->groups) tiles += (G.count + bs - 1) / bs;
+    const int want = (int)std::max<int64_t>(1, std::min<int64_t>(tiles, (int64_t)c->n_cu));
     c->bld_rep.clear();
     for (int r = 0; r < c->R; ++r) {
         RepInfo& R = c->reps[r];
@@ -357,8 +386,10 @@ static int alloc_base(md_ctx* c) {
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(RepInfo) * c->reps.size()))) return rc;
     d.reps = (const RepInfo*)p;
     HIPCHK(c, hipMemcpy(p, c->reps.data(), sizeof(RepInfo) * c->reps.size(), hipMemcpyHostToDevice));
-    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(RepCtrl) * c->R))) return rc;
+    if ((rc = dalloc(c, c->base_allocs, &p, 2 * sizeof(RepCtrl) * c->R))) return rc;
     d.ctrl = (RepCtrl*)p;
+    HIPCHK(c, hipMemset(p, 0, 2 * sizeof(RepCtrl) * c->R));
+    c->par = 0;
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(RepParams) * c->R))) return rc;
     d.params = (const RepParams*)p;
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(RepStats)))) return rc;
@@ -369,19 +400,14 @@ static int alloc_base(md_ctx* c) {
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * (size_t)std::max(c->n_blocks, 1))))
         return rc;
     d.partial = (double*)p;
-    {
-        int maxnb = 1;
-        for (const GroupInfo& G : c->groups) maxnb = std::max(maxnb, (int)G.nblocks);
-        d.fin_slices = std::min(64, std::max(1, (maxnb + 255) / 256));
-    }
-    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * c->groups.size() * d.fin_slices)))
-        return rc;
-    d.part2 = (double*)p;
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * c->groups.size()))) return rc;
     d.sums = (double*)p;
     HIPCHK(c, hipMemset(p, 0, sizeof(double) * NS * c->groups.size()));
     d.n_types = c->T;
     d.n_blocks = c->n_blocks;
+    d.n_reps = c->R;
+    d.tpb = c->tpb;
+    d.par = 0;
     d.n_slots = N;
     d.list_cap = c->opt.list_capacity > 0 ? c->opt.list_capacity : 8;
     d.cell_cap = c->opt.cell_capacity > 0 ? c->opt.cell_capacity : 8;
@@ -393,17 +419,28 @@ static int alloc_base(md_ctx* c) {
     return MD_OK;
 }
 
+// Control blocks of the buffer the next launch reads.
+static RepCtrl* ctrl_cur(md_ctx* c) { return c->d.ctrl + (size_t)c->par * c->R; }
+
 static int read_ctrl(md_ctx* c, std::vector<RepCtrl>& out) {
     out.resize(c->R);
-    HIPCHK(c, hipMemcpyAsync(out.data(), c->d.ctrl, sizeof(RepCtrl) * c->R, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(out.data(), ctrl_cur(c), sizeof(RepCtrl) * c->R, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return MD_OK;
 }
 
 static int write_ctrl(md_ctx* c, const std::vector<RepCtrl>& in) {
-    HIPCHK(c, hipMemcpyAsync(c->d.ctrl, in.data(), sizeof(RepCtrl) * c->R, hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(ctrl_cur(c), in.data(), sizeof(RepCtrl) * c->R, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return MD_OK;
+}
+
+// Finalize launch; M_STEP and M_PRIME write the other control buffer.
+static hipError_t finalize(md_ctx* c, int mode) {
+    c->d.par = c->par;
+    hipError_t e = launch_finalize(c->d, mode, c->list_mode, c->fp32, c->st);
+    if (e == hipSuccess && mode != M_REDUCE) c->par ^= 1;
+    return e;
 }
 
 static void set_kick(md_ctx* c, RepParams& P, double dt) {
@@ -437,8 +474,9 @@ static int ensure_sums(md_ctx* c) {
     std::vector<RepParams> P((size_t)c->R, base_params(c));
     int rc;
     if ((rc = write_params(c, P))) return rc;
+    c->d.par = c->par;
     if (c->n_blocks > 0) HIPCHK(c, launch_reduce_state(c->d, c->fp32, c->bs, c->st));
-    HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->d.fin_slices, c->st));
+    HIPCHK(c, finalize(c, M_REDUCE));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->sums_valid = true;
     return MD_OK;
@@ -453,81 +491,61 @@ static int read_sums(md_ctx* c, std::vector<double>& S) {
 
 // ------------------------------------------------------------------ profiling
 
-static hipError_t ev_begin(md_ctx* c, int kernel, EvPair** out) {
-    *out = nullptr;
-    if (!c->prof) return hipSuccess;
-    if (c->ev_used == c->ev_pool.size()) {
-        EvPair e;
-        hipError_t r = hipEventCreate(&e.a);
-        if (r != hipSuccess) return r;
-        r = hipEventCreate(&e.b);
-        if (r != hipSuccess) return r;
-        c->ev_pool.push_back(e);
+// Fold the device timestamp ring into the per-kernel totals and reset it.
+static int prof_collect(md_ctx* c) {
+    if (!c->prof || !c->prof_ring) return MD_OK;
+    std::vector<unsigned long long> h((size_t)kProfCount + 1);
+    HIPCHK(c, hipMemcpyAsync(h.data(), c->prof_ring, sizeof(unsigned long long) * h.size(),
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (size_t e = 0; e < (size_t)kProfRing * P_NUM; ++e) {
+        const unsigned long long a = h[2 * e], b = h[2 * e + 1];
+        if (a == ~0ull || b < a) continue;
+        const int k = (int)(e % P_NUM);
+        c->ktime[k] += (double)(b - a) / c->wall_hz * 1e3;
+        c->kcount[k] += 1;
     }
-    EvPair* e = &c->ev_pool[c->ev_used++];
-    e->kernel = kernel;
-    *out = e;
-    return hipEventRecord(e->a, c->st);
+    c->list_visits += (int64_t)h[(size_t)kProfCount];
+    HIPCHK(c, launch_prof_init(c->d, c->st));
+    return MD_OK;
 }
 
-static hipError_t ev_end(md_ctx* c, EvPair* e) {
-    if (!e) return hipSuccess;
-    return hipEventRecord(e->b, c->st);
-}
-
-static void ev_collect(md_ctx* c) {
-    for (size_t i = 0; i < c->ev_used; ++i) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->ev_pool[i].a, c->ev_pool[i].b) == hipSuccess) {
-            c->ktime[c->ev_pool[i].kernel] += ms;
-            c->kcount[c->ev_pool[i].kernel] += 1;
-        }
-    }
-    c->ev_used = 0;
-}
-
-// One step launch sequence: list paths k_step + k_finalize, all-pairs path
-// k_drift + k_allpairs + k_finalize.
+// One step launch: list paths k_step + k_fin (which also builds lists),
+// all-pairs path k_drift + k_allpairs + k_fin.
 static hipError_t launch_once(md_ctx* c) {
     hipError_t e;
-    EvPair* ev;
+    c->d.par = c->par;
     if (c->list_mode) {
-        if ((e = ev_begin(c, K_BUILD, &ev))) return e;
-        if ((e = launch_build(c->d, c->fp32, c->bs, c->st))) return e;
-        if ((e = ev_end(c, ev))) return e;
-        if ((e = ev_begin(c, K_STEP, &ev))) return e;
         if ((e = launch_step(c->d, c->fp32, c->bs, c->st))) return e;
-        if ((e = ev_end(c, ev))) return e;
     } else {
-        if ((e = ev_begin(c, K_DRIFT, &ev))) return e;
         if ((e = launch_drift(c->d, c->bs, c->st))) return e;
-        if ((e = ev_end(c, ev))) return e;
-        if ((e = ev_begin(c, K_ALLPAIRS, &ev))) return e;
         if ((e = launch_allpairs(c->d, c->bs, false, c->st))) return e;
-        if ((e = ev_end(c, ev))) return e;
     }
-    if ((e = ev_begin(c, K_FINALIZE, &ev))) return e;
-    if ((e = launch_finalize(c->d, M_STEP, c->R, c->d.fin_slices, c->st))) return e;
-    return ev_end(c, ev);
+    return finalize(c, M_STEP);
 }
 
+// Graph of `len` launches starting at the current parity (len is even, so a
+// replay ends at the parity it started at).
 static int ensure_graph(md_ctx* c, int len) {
-    if (c->gexec && c->g_len == len) return MD_OK;
-    drop_graph(c);
+    const int p0 = c->par;
+    if (c->gexec[p0] && c->g_len[p0] == len) return MD_OK;
+    if (c->gexec[p0]) (void)hipGraphExecDestroy(c->gexec[p0]);
+    c->gexec[p0] = nullptr;
+    c->g_len[p0] = 0;
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int k = 0; k < len && e == hipSuccess; ++k) e = launch_once(c);
     hipError_t e2 = hipStreamEndCapture(c->st, &g);
+    c->par = p0;
     if (e != hipSuccess) return fail(c, MD_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
     if (e2 != hipSuccess) return fail(c, MD_ERR_HIP, std::string("end capture: ") + hipGetErrorString(e2));
-    e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
-    if (e != hipSuccess) {
-        c->gexec = nullptr;
-        return fail(c, MD_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-    }
-    c->g_len = len;
+    if (e != hipSuccess) return fail(c, MD_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    c->gexec[p0] = x;
+    c->g_len[p0] = len;
     return MD_OK;
 }
 
@@ -555,10 +573,13 @@ static int snapshot(md_ctx* c, bool save, int buf) {
             HIPCHK(c, hipMemcpyAsync(arrs[k], s + off, sz[k], hipMemcpyDeviceToDevice, c->st));
         off += sz[k];
     }
-    if (save)
-        HIPCHK(c, hipMemcpyAsync(c->snap_ctrl, c->d.ctrl, sizeof(RepCtrl) * c->R, hipMemcpyDeviceToDevice, c->st));
-    else
-        HIPCHK(c, hipMemcpyAsync(c->d.ctrl, c->snap_ctrl, sizeof(RepCtrl) * c->R, hipMemcpyDeviceToDevice, c->st));
+    if (save) {
+        c->snap_par = c->par;
+        HIPCHK(c, hipMemcpyAsync(c->snap_ctrl, ctrl_cur(c), sizeof(RepCtrl) * c->R, hipMemcpyDeviceToDevice, c->st));
+    } else {
+        c->par = c->snap_par;
+        HIPCHK(c, hipMemcpyAsync(ctrl_cur(c), c->snap_ctrl, sizeof(RepCtrl) * c->R, hipMemcpyDeviceToDevice, c->st));
+    }
     return MD_OK;
 }
 
@@ -609,12 +630,13 @@ static int list_force_eval(md_ctx* c) {
         std::vector<RepParams> P((size_t)c->R, base_params(c));
         if ((rc = write_params(c, P))) return rc;
         if ((rc = reset_stats(c))) return rc;
+        c->d.par = c->par;
         if (c->n_blocks > 0) {
             HIPCHK(c, launch_insert_all(c->d, c->fp32, c->bs, c->st));
-            HIPCHK(c, launch_build(c->d, c->fp32, c->bs, c->st));
+            HIPCHK(c, launch_build(c->d, c->fp32, c->st));
             HIPCHK(c, launch_force_eval(c->d, c->fp32, c->bs, c->st));
         }
-        HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->d.fin_slices, c->st));
+        HIPCHK(c, finalize(c, M_REDUCE));
         RepStats s;
         if ((rc = read_stats(c, s))) return rc;
         int grown = 0;
@@ -681,7 +703,10 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
         p.rebuild_every_step = (c->list_mode && c->T > 1 && p.ba_kind == 1) ? 1 : 0;
         if (C[r].step != step0) return fail(c, MD_ERR_ARG, "replicas out of step");
     }
-    const int K = c->prof ? 1 : (c->opt.graph_chunk > 0 ? c->opt.graph_chunk : 32);
+    // graphs hold an even number of launches so that a replay ends at the
+    // control buffer it started from
+    int K = c->opt.graph_chunk > 0 ? c->opt.graph_chunk : 32;
+    if (K > 1) K += K & 1;
     const int64_t chunk = c->list_mode ? std::max<int64_t>(512, 16 * (int64_t)K) : 64;
     int64_t done = 0;
     bool first = true, bad = false;
@@ -699,17 +724,16 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
         }
         if ((rc = write_params(c, P))) return rc;
         if ((rc = reset_stats(c))) return rc;
-        HIPCHK(c, launch_finalize(c->d, M_PRIME, c->R, 1, c->st));
+        HIPCHK(c, finalize(c, M_PRIME));
         int64_t k = 0;
-        if (!c->prof && K > 1 && cn >= K) {
+        if (K > 1 && cn >= K) {
             if ((rc = ensure_graph(c, K))) return rc;
-            for (; k + K <= cn; k += K) HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+            for (; k + K <= cn; k += K) HIPCHK(c, hipGraphLaunch(c->gexec[c->par], c->st));
         }
         for (; k < cn; ++k) HIPCHK(c, launch_once(c));
         int64_t launched = cn;
         for (int iter = 0;; ++iter) {
             if ((rc = read_ctrl(c, C))) return rc;
-            if (c->prof) ev_collect(c);
             int64_t behind = 0;
             for (int r = 0; r < c->R; ++r) behind = std::max(behind, P[r].target_step - C[r].step);
             if (behind <= 0) break;
@@ -727,8 +751,24 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
             if ((rc = read_ctrl(c, C))) return rc;
             for (int r = 0; r < c->R; ++r) C[r].flags &= ~(F_HAVE_LIST | F_GRID_FRESH);
             if ((rc = write_ctrl(c, C))) return rc;
+            if (c->prof) HIPCHK(c, launch_prof_init(c->d, c->st));  // the replay reuses launch indices
             st.retries++;
             continue;
+        }
+        if ((rc = prof_collect(c))) return rc;
+        // a list or cell that came within a quarter of its capacity: grow now,
+        // lists are rebuilt at the next launch
+        if (s.max_list * 4 > c->d.list_cap * 3 || s.max_cell * 4 > c->d.cell_cap * 3) {
+            RepStats g{};
+            g.list_overflow = s.max_list * 4 > c->d.list_cap * 3;
+            g.cell_overflow = s.max_cell * 4 > c->d.cell_cap * 3;
+            g.max_list = s.max_list;
+            g.max_cell = s.max_cell;
+            int grown = 0;
+            if ((rc = grow_capacity(c, g, &grown))) return rc;
+            for (int r = 0; r < c->R; ++r) C[r].flags &= ~(F_HAVE_LIST | F_GRID_FRESH);
+            if ((rc = write_ctrl(c, C))) return rc;
+            st.grows++;
         }
         for (int r = 0; r < c->R; ++r)
             if (C[r].flags & F_BAD) bad = true;
@@ -758,6 +798,7 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
     c->stats.steps = st.steps;
     c->stats.launches = st.launches;
     c->stats.retries = st.retries;
+    c->stats.grows = st.grows;
     c->stats.list_capacity = c->d.list_cap;
     c->stats.cell_capacity = c->d.cell_cap;
     c->stats.n_slots = (int32_t)c->n_slots;
@@ -836,7 +877,15 @@ int md_create_replicas(md_ctx** out, int n_replicas, int n_types, const int64_t*
     c->opt.cell_capacity = 0;
     c->opt.cells_per_atom = 2.0;
     c->opt.graph_chunk = 32;
+    c->opt.tiles_per_block = 0;
     c->bs = 256;
+    {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            c->n_cu = v;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeWallClockRate, dev) == hipSuccess && v > 0)
+            c->wall_hz = 1e3 * (double)v;  // attribute is in kHz
+    }
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return fail(nullptr, MD_ERR_HIP, "hipStreamCreate failed");
@@ -867,10 +916,6 @@ void md_destroy(md_ctx* c) {
     if (!c) return;
     (void)hipStreamSynchronize(c->st);
     free_all(c);
-    for (auto& e : c->ev_pool) {
-        (void)hipEventDestroy(e.a);
-        (void)hipEventDestroy(e.b);
-    }
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -1108,9 +1153,12 @@ int md_set_options(md_ctx* c, const md_options* o) {
     if (!(n.cells_per_atom > 0)) n.cells_per_atom = 2.0;
     if (n.graph_chunk < 0) return fail(c, MD_ERR_ARG, "graph_chunk < 0");
     if (n.skin < 0) return fail(c, MD_ERR_ARG, "skin < 0");
+    if (n.tiles_per_block < 0 || n.tiles_per_block > 64)
+        return fail(c, MD_ERR_ARG, "tiles_per_block must be 0 (automatic) or 1..64");
     c->skin_set = n.skin > 0;
     const bool relay = n.block_size != c->opt.block_size || n.force_mode != c->opt.force_mode ||
-                       n.cells_per_atom != c->opt.cells_per_atom;
+                       n.cells_per_atom != c->opt.cells_per_atom ||
+                       n.tiles_per_block != c->opt.tiles_per_block;
     if (relay) {
         int rc = relayout(c, n);
         if (rc) return rc;
@@ -1169,8 +1217,9 @@ int md_update_force(md_ctx* c) {
     std::vector<RepParams> P((size_t)c->R, base_params(c));
     int rc = write_params(c, P);
     if (rc) return rc;
+    c->d.par = c->par;
     if (c->n_blocks > 0) HIPCHK(c, launch_allpairs(c->d, c->bs, true, c->st));
-    HIPCHK(c, launch_finalize(c->d, M_REDUCE, c->R, c->d.fin_slices, c->st));
+    HIPCHK(c, finalize(c, M_REDUCE));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->sums_valid = true;
     return MD_OK;
@@ -1178,6 +1227,7 @@ int md_update_force(md_ctx* c) {
 
 int md_apply_boundary_conditions(md_ctx* c) {
     if (!c) return MD_ERR_ARG;
+    c->d.par = c->par;
     if (c->n_blocks > 0) HIPCHK(c, launch_wrap(c->d, c->fp32, c->bs, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return MD_OK;
@@ -1261,6 +1311,7 @@ int md_thermostat_update(md_ctx* c, md_thermostat* th, double dt, uint16_t type_
     if (rc) return rc;
     if (th->kind == 1 || th->kind == 2) {
         const GroupInfo& G = c->groups[c->reps[0].first_group + type_id];
+        c->d.par = c->par;
         HIPCHK(c, launch_scale(c->d, c->fp32, G.start, G.count, th->lambda, 0, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         c->sums_valid = false;
@@ -1287,6 +1338,7 @@ int md_barostat_update(md_ctx* c, md_barostat* ba, uint16_t type_id) {
     if (ba->kind == 2) return fail(c, MD_ERR_UNIMPLEMENTED, "Barostat::Custom is todo!() in the reference");
     if (ba->kind != 1) return MD_OK;
     const GroupInfo& G = c->groups[c->reps[0].first_group + type_id];
+    c->d.par = c->par;
     HIPCHK(c, launch_scale(c->d, c->fp32, G.start, G.count, ba->myu, 1, c->st));
     std::vector<RepCtrl> C;
     int rc = read_ctrl(c, C);
@@ -1380,6 +1432,7 @@ int md_get_stats(md_ctx* c, md_stats* out) {
     s.n_slots = (int32_t)c->n_slots;
     s.mean_list_len = 0.0;
     s.max_list_len = 0;
+    s.list_visits = c->list_visits;
     if (c->list_mode && c->n_slots > 0) {
         std::vector<int32_t> nc((size_t)c->n_slots);
         if (hipMemcpy(nc.data(), c->d.ncount, sizeof(int32_t) * nc.size(), hipMemcpyDeviceToHost) == hipSuccess) {
@@ -1401,19 +1454,39 @@ int md_get_stats(md_ctx* c, md_stats* out) {
 
 int md_set_profiling(md_ctx* c, int on) {
     if (!c) return MD_ERR_ARG;
-    c->prof = on != 0;
-    for (int k = 0; k < K_NUM; ++k) {
+    for (int k = 0; k < P_NUM; ++k) {
         c->ktime[k] = 0.0;
         c->kcount[k] = 0;
     }
-    c->ev_used = 0;
+    c->list_visits = 0;
+    if (on && !c->prof_ring) {
+        void* p = nullptr;
+        HIPCHK(c, hipMalloc(&p, sizeof(unsigned long long) * ((size_t)kProfCount + 1)));
+        c->prof_ring = (unsigned long long*)p;
+        HIPCHK(c, hipMalloc(&p, sizeof(unsigned long long) * 3 * P_NUM * (size_t)std::max(c->n_blocks, 1)));
+        c->prof_blk = (unsigned long long*)p;
+    }
+    c->prof = on != 0;
+    c->d.prof = c->prof ? c->prof_ring : nullptr;
+    c->d.prof_blk = c->prof ? c->prof_blk : nullptr;
+    drop_graph(c);  // the graphs hold the old kernel arguments
+    if (c->prof) {
+        HIPCHK(c, launch_prof_init(c->d, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+    }
     return MD_OK;
 }
 
 int md_kernel_time(md_ctx* c, int kernel, double* total_ms, int64_t* launches) {
     if (!c || kernel < 0 || kernel >= K_NUM) return MD_ERR_ARG;
-    if (total_ms) *total_ms = c->ktime[kernel];
-    if (launches) *launches = c->kcount[kernel];
+    // the step slot holds k_step on the list paths and k_allpairs on the plain path
+    int slot = -1;
+    if (kernel == K_STEP && c->list_mode) slot = P_STEP;
+    if (kernel == K_ALLPAIRS && !c->list_mode) slot = P_STEP;
+    if (kernel == K_FINALIZE) slot = P_FIN;
+    if (kernel == K_DRIFT && !c->list_mode) slot = P_DRIFT;
+    if (total_ms) *total_ms = slot >= 0 ? c->ktime[slot] : 0.0;
+    if (launches) *launches = slot >= 0 ? c->kcount[slot] : 0;
     return MD_OK;
 }
 

@@ -3,8 +3,9 @@
 // (host runtime behind the C ABI).
 //
 // Layout in HBM (one context, R replicas x T types):
-//  * particles live in "slots"; each (replica, type) group starts at a
-//    multiple of the per-atom block size, so every block belongs to exactly
+//  * particles live in "slots"; each (replica, type) group is a whole number
+//    of tiles (one tile = one slot per thread of a block), and a block owns
+//    `tpb` consecutive tiles of one group, so every block belongs to exactly
 //    one group and the per-group FP64 reductions need no segmenting.
 //    Padding slots hold NaN positions, which fail every distance test.
 //  * state is an array of per-particle records (position and velocity FP64,
@@ -18,6 +19,9 @@
 //    fill can run while the other buffer is being read.  Lists are built from
 //    it brick by brick: a block stages a brick of cells plus a one-cell halo
 //    (ids and positions) in LDS and builds the lists of the brick's atoms.
+//  * the per-replica control blocks are double-buffered by launch parity:
+//    every kernel of a launch reads buffer `par`, the finalize writes
+//    `par ^ 1`, so blocks of the same kernel never see a half-updated block.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -160,26 +164,62 @@ struct DevState {
     RepStats* stats;
     const PairParam* pairs;  // n_types x n_types
     double* partial;         // NS x n_blocks (column-major)
-    double* part2;           // n_groups x fin_slices x NS, finalize slice sums
     double* sums;            // n_groups x NS
+    unsigned long long* prof;  // per-launch kernel timestamps (null: off)
+    unsigned long long* prof_blk;  // per-block {start, end, list entries} of the per-slot kernels
     int32_t n_types;
     int32_t n_blocks;
-    int32_t fin_slices;
-    int32_t n_bld;           // k_build grid size
+    int32_t n_reps;
+    int32_t par;             // control buffer this launch reads
+    int32_t tpb;             // tiles per block of the per-slot kernels
+    int32_t n_bld;           // list-build blocks
     int64_t n_slots;
 };
 
+__host__ __device__ inline const RepCtrl& ctrl_in(const DevState& d, int r) {
+    return d.ctrl[d.par * d.n_reps + r];
+}
+__host__ __device__ inline RepCtrl* ctrl_out(const DevState& d, int r) {
+    return d.ctrl + (d.par ^ 1) * d.n_reps + r;
+}
+
+// Tiles of block b: group G's tiles [t0, t1), slots G.start + t * BS + lane.
+struct TileRange {
+    int64_t t0, t1;
+};
+__host__ __device__ inline TileRange block_tiles(const DevState& d, const GroupInfo& G, int b, int bs) {
+    const int64_t nt = (G.count + bs - 1) / bs;
+    const int64_t t0 = (int64_t)(b - G.first_block) * d.tpb;
+    const int64_t t1 = t0 + d.tpb < nt ? t0 + d.tpb : nt;
+    return TileRange{t0, t1};
+}
+
+// Per-launch timestamp ring (wall clock ticks): for launch l, entry
+// (l % kProfRing) holds {min start, max end} of kernels P_STEP, P_FIN, P_DRIFT;
+// one counter of list entries read follows the ring.  The per-slot kernels
+// store per-block {start, end, entries} with plain stores (slot P_STEP for
+// k_step / k_allpairs, P_DRIFT for k_drift) and one block of the finalize
+// kernel folds them into the ring; the finalize kernel's own blocks (one per
+// replica plus the blocks that build lists) record with atomics.
+constexpr int kProfRing = 8192;
+enum { P_STEP = 0, P_FIN = 1, P_DRIFT = 2, P_NUM = 3 };
+constexpr int64_t kProfCount = (int64_t)kProfRing * P_NUM * 2;  // index of the entry counter
+
 enum FinalizeMode { M_STEP = 0, M_REDUCE = 1, M_PRIME = 2 };
 
-// the finalize stages these through LDS word by word with 128 lanes each
-static_assert(sizeof(RepCtrl) % 4 == 0 && sizeof(RepCtrl) / 4 <= 128, "RepCtrl staging");
-static_assert(sizeof(RepParams) % 4 == 0 && sizeof(RepParams) / 4 <= 128, "RepParams staging");
-static_assert(sizeof(GroupInfo) % 4 == 0 && sizeof(GroupInfo) / 4 * kMaxTypes <= 128,
-              "GroupInfo staging");
-static_assert(kMaxTypes * NS <= 128, "sums staging");
+// the finalize stages these through LDS word by word
+static_assert(sizeof(RepCtrl) % 4 == 0, "RepCtrl staging");
+static_assert(sizeof(RepParams) % 4 == 0, "RepParams staging");
+static_assert(sizeof(GroupInfo) % 4 == 0, "GroupInfo staging");
 
-// LDS capacity of one brick: halo cells and staged particles
-constexpr int kBrickCells = 1000;  // (8 + 2)^3
+// threads per block of the finalize / list-build kernel
+constexpr int kFinBS = 512;
+
+// LDS capacity of one brick: halo cells and staged particles.  Dilute
+// systems get bricks of up to 14^3 cells so that a brick still holds a few
+// hundred particles.
+constexpr int kBrickEdge = 14;
+constexpr int kBrickCells = 4096;  // (14 + 2)^3
 constexpr int kBrickAtoms = 1024;
 constexpr int kBrickTarget = 600;  // expected staged particles a brick is sized for
 constexpr int kListReg = 16;       // list entries kept (sorted) in registers while building
@@ -187,19 +227,32 @@ constexpr int kListReg = 16;       // list entries kept (sorted) in registers wh
 // Cell grid of a fill: per axis floor(L / (r_list (1 + 1e-9))) cells in
 // [1, 1024], coarsened until the replica's cell budget holds.  Cell edges stay
 // >= r_list, so the 27-cell stencil around a particle covers its list sphere.
+// (Written with fixed indices only, so that the device control update keeps
+// its state in registers.)
+__host__ __device__ inline int coarsen(int g) {
+    const int dec = g / 8;
+    const int n = g - (dec < 1 ? 1 : dec);
+    return n < 1 ? 1 : n;
+}
 __host__ __device__ inline void grid_dims(const double* gb, double rlist, int64_t budget, int* gd) {
+    int g[3];
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
         const double q = floor(gb[k] / (rlist * (1.0 + 1e-9)));
-        gd[k] = q < 1.0 ? 1 : (q > 1024.0 ? 1024 : (int)q);
+        g[k] = q < 1.0 ? 1 : (q > 1024.0 ? 1024 : (int)q);
     }
-    while ((int64_t)gd[0] * gd[1] * gd[2] > budget) {
-        int k = 0;
-        if (gd[1] > gd[k]) k = 1;
-        if (gd[2] > gd[k]) k = 2;
-        const int dec = gd[k] / 8;
-        gd[k] = gd[k] - (dec < 1 ? 1 : dec);
-        if (gd[k] < 1) gd[k] = 1;
+    while ((int64_t)g[0] * g[1] * g[2] > budget) {
+        // the largest axis, the first of equals
+        if (g[1] > g[0] && g[1] >= g[2]) {
+            g[1] = coarsen(g[1]);
+        } else if (g[2] > g[0] && g[2] > g[1]) {
+            g[2] = coarsen(g[2]);
+        } else {
+            g[0] = coarsen(g[0]);
+        }
     }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gd[k] = g[k];
 }
 
 // Brick shape of a grid: the largest cubic edge E <= 8 cells whose halo
@@ -208,20 +261,23 @@ __host__ __device__ inline void grid_dims(const double* gb, double rlist, int64_
 __host__ __device__ inline void brick_dims(const int* gd, int64_t n_atoms, int* edge, int* whole) {
     const double per_cell = (double)n_atoms / ((double)gd[0] * gd[1] * gd[2]);
     int best = 1;
-    for (int e = 1; e <= 8; ++e) {
+    for (int e = 1; e <= kBrickEdge; ++e) {
         double halo = 1.0;
+#pragma unroll
         for (int k = 0; k < 3; ++k) halo *= (gd[k] <= e + 2) ? gd[k] : (e + 2);
-        if (halo * per_cell <= (double)kBrickTarget) best = e;
+        if (halo <= (double)kBrickCells && halo * per_cell <= (double)kBrickTarget) best = e;
     }
-    *whole = 0;
+    int w = 0;
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
         if (gd[k] <= best + 2) {
             edge[k] = gd[k];
-            *whole |= 1 << k;
+            w |= 1 << k;
         } else {
             edge[k] = best;
         }
     }
+    *whole = w;
 }
 
 // column masks for block partial stores
@@ -233,8 +289,11 @@ constexpr uint32_t kColsAll = (1u << NS) - 1u;
 
 // host-side launchers (md_kernels.hip)
 hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st);
-hipError_t launch_build(const DevState& d, bool fp32, int bs, hipStream_t st);
-hipError_t launch_finalize(const DevState& d, int mode, int n_reps, int slices, hipStream_t st);
+hipError_t launch_build(const DevState& d, bool fp32, hipStream_t st);
+hipError_t launch_prof_init(const DevState& d, hipStream_t st);
+// finalize (grid n_reps); with `build` the grid also covers the list-build
+// blocks, which build the lists of replicas flagged F_BUILD
+hipError_t launch_finalize(const DevState& d, int mode, bool build, bool fp32, hipStream_t st);
 hipError_t launch_drift(const DevState& d, int bs, hipStream_t st);
 hipError_t launch_allpairs(const DevState& d, int bs, bool force_only, hipStream_t st);
 hipError_t launch_insert_all(const DevState& d, bool fp32, int bs, hipStream_t st);

@@ -1,21 +1,22 @@
 // md_kernels.hip -- gfx950 kernels of the MD step engine.
 //
-// Per time step of a replica the engine runs two launches:
-//   k_step      one thread per particle slot: half kick, thermostat scale,
-//               drift, wrap (integrator.rs:28-45), Verlet-list force
-//               (potential.rs:158-216) with each neighbour's drift recomputed
-//               from its input record, second half kick and barostat scale
-//               (integrator.rs:47-58), per-block FP64 partial sums, and when
-//               asked a fill of the cell grid with the output positions.
-//               The same kernel also serves the launches that do not advance
-//               time: filling the grid with the input positions (F_NULL),
-//               building the Verlet lists brick by brick (F_BUILD), and the
-//               two halves of a step whose drift exceeds the skin (F_SPLIT_*).
-//   k_finalize  fixed-order reduction of the block partials, then the
-//               Berendsen / Nose-Hoover lambda, psi and Berendsen mu of the
-//               next step (thermostat.rs:24-72, barostat.rs:21-56), box
-//               rescale and the list-validity decision for the next launch.
-//               Everything stays on the device; the host replays a graph.
+// One launch per time step of every replica, two kernels:
+//   k_step  one thread per particle slot of each of the block's tiles: half
+//           kick, thermostat scale, drift, wrap (integrator.rs:28-45),
+//           Verlet-list force (potential.rs:158-216) with each neighbour's
+//           drift recomputed from its input record, second half kick and
+//           barostat scale (integrator.rs:47-58), per-block FP64 partial
+//           sums, and when asked a fill of the cell grid with the output
+//           positions.  It also serves the launches that do not advance time:
+//           filling the grid with the input positions (F_NULL) and the first
+//           half of a step whose drift exceeds the skin (F_SPLIT_A).
+//   k_fin   per replica, the fixed-order reduction of the block partials,
+//           then the Berendsen / Nose-Hoover lambda, psi and Berendsen mu of
+//           the next step (thermostat.rs:24-72, barostat.rs:21-56), box
+//           rescale and the list-validity decision for the next launch; its
+//           other blocks build the Verlet lists brick by brick in launches
+//           flagged F_BUILD (k_step does nothing in those).
+// Everything stays on the device; the host replays a graph of launches.
 // The O(N^2) FP64 path (k_drift + k_allpairs) restates update_force exactly:
 // same pair criterion, same operation order, j ascending.
 //
@@ -209,6 +210,8 @@ __device__ __forceinline__ void grid_insert(const DevState& d, const RepCtrl& C,
     } while (old != assumed);
     if ((int)k < d.cell_cap) {
         d.cell_slots[buf][cell * d.cell_cap + k] = (int32_t)slot;
+        // within a quarter of the capacity: report, the host grows it between chunks
+        if (4 * ((int)k + 1) > 3 * d.cell_cap) atomicMax(&d.stats->max_cell, (int)k + 1);
     } else {
         atomicOr(&d.stats->cell_overflow, 1u);
         atomicMax(&d.stats->max_cell, (int)k + 1);
@@ -232,6 +235,8 @@ __device__ __forceinline__ void list_close(const DevState& d, int64_t slot, int 
     d.ncount[slot] = n < d.list_cap ? n : d.list_cap;
     if (n > d.list_cap) {
         atomicOr(&d.stats->list_overflow, 1u);
+        atomicMax(&d.stats->max_list, n);
+    } else if (4 * n > 3 * d.list_cap) {
         atomicMax(&d.stats->max_list, n);
     }
 }
@@ -382,10 +387,21 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             const int c2 = ((org[2] + h2) % gd[2] + gd[2]) % gd[2];
             return R.cell_base + ((int64_t)c0 * gd[1] + c1) * gd[2] + c2;
         };
-        for (int h = threadIdx.x; h < H; h += BS) {
-            const unsigned long long w = hdr[cell_of(h)];
-            int cnt = (uint32_t)(w >> 32) == ep ? (int)(uint32_t)w : 0;
-            s_off[h] = cnt < cc ? cnt : cc;
+        // all header loads of a thread in flight at once
+        constexpr int HP = (kBrickCells + BS - 1) / BS;
+        unsigned long long hw[HP];
+#pragma unroll
+        for (int q = 0; q < HP; ++q) {
+            const int h = threadIdx.x + q * BS;
+            hw[q] = h < H ? hdr[cell_of(h)] : 0ull;
+        }
+#pragma unroll
+        for (int q = 0; q < HP; ++q) {
+            const int h = threadIdx.x + q * BS;
+            if (h < H) {
+                const int cnt = (uint32_t)(hw[q] >> 32) == ep ? (int)(uint32_t)hw[q] : 0;
+                s_off[h] = cnt < cc ? cnt : cc;
+            }
         }
         __syncthreads();
         block_exclusive_scan<BS>(s_off, H);
@@ -406,19 +422,20 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             __syncthreads();
             continue;
         }
-        for (int h = threadIdx.x; h < H; h += BS) {
-            const int64_t cell = cell_of(h);
-            const int base = s_off[h], cnt = s_off[h + 1] - base;
-            for (int k = 0; k < cnt; ++k) {
-                const int32_t j = cs[cell * cc + k];
-                const Rec<FT>& o = recs[j];
-                s_x[base + k] = o.x;
-                s_y[base + k] = o.y;
-                s_z[base + k] = o.z;
-                s_id[base + k] = j;
-                s_hc[base + k] = (uint16_t)h;
-                if (MULTI) s_ty[base + k] = (uint8_t)o.type;
-            }
+        // staged particle -> its halo cell (LDS only), then one particle per
+        // thread: slot id and position loads of all particles go out together
+        for (int h = threadIdx.x; h < H; h += BS)
+            for (int a = s_off[h]; a < s_off[h + 1]; ++a) s_hc[a] = (uint16_t)h;
+        __syncthreads();
+        for (int a = threadIdx.x; a < total; a += BS) {
+            const int h = s_hc[a];
+            const int32_t j = cs[cell_of(h) * cc + (a - s_off[h])];
+            const Rec<FT>& o = recs[j];
+            s_x[a] = o.x;
+            s_y[a] = o.y;
+            s_z[a] = o.z;
+            s_id[a] = j;
+            if (MULTI) s_ty[a] = (uint8_t)o.type;
         }
         __syncthreads();
         for (int a = threadIdx.x; a < total; a += BS) {
@@ -471,28 +488,66 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
 
 // ---------------------------------------------------------------- fused step
 
+// Per-launch kernel timestamps: first block start to last block end, in
+// wall-clock ticks, accumulated per launch with atomics (only when profiling).
+__device__ __forceinline__ uint64_t prof_now(const DevState& d) {
+    return d.prof ? (uint64_t)wall_clock64() : 0ull;
+}
+
+__device__ __forceinline__ void prof_mark(const DevState& d, int64_t launch, int kernel, uint64_t t0) {
+    if (d.prof && threadIdx.x == 0) {
+        const uint64_t t1 = (uint64_t)wall_clock64();
+        unsigned long long* e = d.prof + ((launch % kProfRing) * P_NUM + kernel) * 2;
+        atomicMin(e, (unsigned long long)t0);
+        atomicMax(e + 1, (unsigned long long)t1);
+    }
+}
+
+// Per-block record of a per-slot kernel (thread 0, after the block's last
+// barrier); blocks that did no work store an empty interval.
+__device__ __forceinline__ void prof_block(const DevState& d, int slot, uint64_t t0, bool worked,
+                                           uint64_t entries) {
+    if (d.prof && threadIdx.x == 0) {
+        unsigned long long* e = d.prof_blk + ((int64_t)slot * d.n_blocks + blockIdx.x) * 3;
+        e[0] = worked ? (unsigned long long)t0 : ~0ull;
+        e[1] = worked ? (unsigned long long)wall_clock64() : 0ull;
+        e[2] = entries;
+    }
+}
+
+// One launch of the step kernel over the block's tiles.  The record and list
+// length of the next tile are loaded before the current tile is processed.
 template <typename FT, int BS, bool MULTI>
-__global__ __launch_bounds__(BS) void k_step(DevState d) {
+__global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
     const int r = G.rep;
-    const RepCtrl& C = d.ctrl[r];
+    const RepCtrl& C = ctrl_in(d, r);
     const uint32_t flags = C.flags;
-    if (flags & F_EXIT) return;
+    if (flags & (F_EXIT | F_BUILD)) {  // a build launch builds in the finalize kernel
+        prof_block(d, P_STEP, 0, false, 0);
+        return;
+    }
+    const uint64_t t_start = prof_now(d);
     const RepParams& P = d.params[r];
     const RepInfo R = d.reps[r];
-    const int64_t s = (int64_t)b * BS + threadIdx.x;
-    const bool valid = (s - G.start) < G.count;
+    const TileRange tr = block_tiles(d, G, b, BS);
+    const int lane = threadIdx.x;
+    const int64_t base = G.start + lane;
     const int cur = C.cur, nx = cur ^ 1;
     const Rec<FT>* __restrict__ rin = (const Rec<FT>*)d.rec[cur];
     Rec<FT>* __restrict__ rnx = (Rec<FT>*)d.rec[nx];
     double* const part_out = d.partial + b;
     const int64_t pstride = d.n_blocks;
     if (flags & F_NULL) {
-        if (valid) grid_insert(d, C, R, s, rin[s].x, rin[s].y, rin[s].z);
+        for (int64_t t = tr.t0; t < tr.t1; ++t) {
+            const int64_t s = base + t * BS;
+            if (t * BS + lane < G.count) grid_insert(d, C, R, s, rin[s].x, rin[s].y, rin[s].z);
+        }
+        __syncthreads();
+        prof_block(d, P_STEP, t_start, true, 0);
         return;
     }
-    if (flags & F_BUILD) return;  // k_build's launch
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double dt = P.dt, lam = C.lambda, mu = C.myu;
     const double m = G.mass;
@@ -505,15 +560,17 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
     if (flags & F_SPLIT_A) {
         // first half of a split step: kick, scale, drift, wrap into cur ^ 1 and
         // fill the grid with the drifted positions (the drift exceeds the skin)
-        if (valid) {
+        for (int64_t t = tr.t0; t < tr.t1; ++t) {
+            const int64_t s = base + t * BS;
+            if (t * BS + lane >= G.count) continue;
             Rec<FT> a = rin[s];
             const double vkx = a.vx + (double)a.fx * c;
             const double vky = a.vy + (double)a.fy * c;
             const double vkz = a.vz + (double)a.fz * c;
-            part[S_KVX] = vkx * m;
-            part[S_KVY] = vky * m;
-            part[S_KVZ] = vkz * m;
-            part[S_KV2] = m * ((vkx * vkx + vky * vky) + vkz * vkz);
+            part[S_KVX] += vkx * m;
+            part[S_KVY] += vky * m;
+            part[S_KVZ] += vkz * m;
+            part[S_KV2] += m * ((vkx * vkx + vky * vky) + vkz * vkz);
             a.vx = vkx * lam;
             a.vy = vky * lam;
             a.vz = vkz * lam;
@@ -521,111 +578,163 @@ __global__ __launch_bounds__(BS) void k_step(DevState d) {
             a.y = wrap1(a.y + a.vy * dt, by);
             a.z = wrap1(a.z + a.vz * dt, bz);
             rnx[s] = a;
-            part[S_VMAX] = (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz;
+            part[S_VMAX] = fmax(part[S_VMAX], (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
             grid_insert(d, C, R, s, a.x, a.y, a.z);
         }
         block_store<BS>(part, part_out, pstride, kick_cols | kColsDrift);
+        prof_block(d, P_STEP, t_start, true, 0);
         return;
     }
     // second half of a split step: drifted state in cur ^ 1, results to cur
     const bool split = (flags & F_SPLIT_B) != 0;
     const Rec<FT>* __restrict__ src = split ? rnx : rin;
     Rec<FT>* __restrict__ dst = split ? (Rec<FT>*)d.rec[cur] : rnx;
-    if (valid) {
-        const Rec<FT> a = src[s];
-        double x1, y1, z1, v1x, v1y, v1z;
-        if (split) {
-            x1 = a.x;
-            y1 = a.y;
-            z1 = a.z;
-            v1x = a.vx;
-            v1y = a.vy;
-            v1z = a.vz;
-        } else {
-            // integrator.rs:28-34 first half kick
-            const double vkx = a.vx + (double)a.fx * c;
-            const double vky = a.vy + (double)a.fy * c;
-            const double vkz = a.vz + (double)a.fz * c;
-            part[S_KVX] = vkx * m;
-            part[S_KVY] = vky * m;
-            part[S_KVZ] = vkz * m;
-            part[S_KV2] = m * ((vkx * vkx + vky * vky) + vkz * vkz);
-            // thermostat.rs:55-57 scale, integrator.rs:40-45 drift + wrap
-            v1x = vkx * lam;
-            v1y = vky * lam;
-            v1z = vkz * lam;
-            x1 = wrap1(a.x + v1x * dt, bx);
-            y1 = wrap1(a.y + v1y * dt, by);
-            z1 = wrap1(a.z + v1z * dt, bz);
-            part[S_VMAX] = (v1x * v1x + v1y * v1y) + v1z * v1z;
+    const PairParam p0 = d.pairs[0];
+    int visits = 0;  // list entries read by this thread (profiling)
+    // A tile's records move between HBM and LDS as whole 16-byte words with
+    // consecutive lanes on consecutive words (fully coalesced); each thread
+    // then reads and writes its own record in LDS.  The next tile's words
+    // load into registers while the current tile is processed.
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    constexpr int RQ = (int)(sizeof(Rec<FT>) / 16);
+    __shared__ u4 s_tile[BS * RQ];
+    u4 pre[RQ];
+    int nn = 0;
+    if (tr.t0 < tr.t1) {
+        const u4* g = (const u4*)(src + G.start + tr.t0 * BS);
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) pre[q] = g[q * BS + lane];
+        nn = tr.t0 * BS + lane < G.count ? d.ncount[base + tr.t0 * BS] : 0;
+    }
+    for (int64_t t = tr.t0; t < tr.t1; ++t) {
+        const int64_t s = base + t * BS;
+        const bool valid = t * BS + lane < G.count;
+#pragma unroll
+        for (int q = 0; q < RQ; ++q) s_tile[q * BS + lane] = pre[q];
+        __syncthreads();
+        const Rec<FT> a = ((const Rec<FT>*)s_tile)[lane];
+        int nnn = 0;
+        if (t + 1 < tr.t1) {
+            const u4* g = (const u4*)(src + G.start + (t + 1) * BS);
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) pre[q] = g[q * BS + lane];
+            nnn = (t + 1) * BS + lane < G.count ? d.ncount[s + BS] : 0;
         }
-        // potential.rs:158-216 over the Verlet list at the drifted positions;
-        // a neighbour's drift is recomputed from its input record with the
-        // same operations its own thread performs, so it is bitwise the same
-        Acc<FT> acc;
-        const int nn = d.ncount[s];
-        const PairParam p0 = d.pairs[0];
-        for (int k = 0; k < nn; ++k) {
-            const int32_t j = d.nbr[(int64_t)k * d.n_slots + s];
-            const Rec<FT> o = src[j];
-            const int tj = MULTI ? (int)o.type : 0;
-            double xj = o.x, yj = o.y, zj = o.z;
-            if (!split) {
-                const double cj = MULTI ? P.cm[tj] : c;
-                xj = wrap1(xj + ((o.vx + (double)o.fx * cj) * lam) * dt, bx);
-                yj = wrap1(yj + ((o.vy + (double)o.fy * cj) * lam) * dt, by);
-                zj = wrap1(zj + ((o.vz + (double)o.fz * cj) * lam) * dt, bz);
+        Rec<FT> res = a;
+        if (valid) {
+            double x1, y1, z1, v1x, v1y, v1z;
+            if (split) {
+                x1 = a.x;
+                y1 = a.y;
+                z1 = a.z;
+                v1x = a.vx;
+                v1y = a.vy;
+                v1z = a.vz;
+            } else {
+                // integrator.rs:28-34 first half kick
+                const double vkx = a.vx + (double)a.fx * c;
+                const double vky = a.vy + (double)a.fy * c;
+                const double vkz = a.vz + (double)a.fz * c;
+                part[S_KVX] += vkx * m;
+                part[S_KVY] += vky * m;
+                part[S_KVZ] += vkz * m;
+                part[S_KV2] += m * ((vkx * vkx + vky * vky) + vkz * vkz);
+                // thermostat.rs:55-57 scale, integrator.rs:40-45 drift + wrap
+                v1x = vkx * lam;
+                v1y = vky * lam;
+                v1z = vkz * lam;
+                x1 = wrap1(a.x + v1x * dt, bx);
+                y1 = wrap1(a.y + v1y * dt, by);
+                z1 = wrap1(a.z + v1z * dt, bz);
+                part[S_VMAX] = fmax(part[S_VMAX], (v1x * v1x + v1y * v1y) + v1z * v1z);
             }
-            const double dx = min_image(xj - x1, bx);
-            const double dy = min_image(yj - y1, by);
-            const double dz = min_image(zj - z1, bz);
-            const double r2 = (dx * dx + dy * dy) + dz * dz;
-            const PairParam& pp = MULTI ? d.pairs[ti * d.n_types + tj] : p0;
-            if (r2 <= pp.r2max) acc.add(dx, dy, dz, r2, pp);
+            // potential.rs:158-216 over the Verlet list at the drifted positions;
+            // a neighbour's drift is recomputed from its input record with the
+            // same operations its own thread performs, so it is bitwise the same
+            Acc<FT> acc;
+            for (int k = 0; k < nn; ++k) {
+                const int32_t j = d.nbr[(int64_t)k * d.n_slots + s];
+                const Rec<FT> o = src[j];
+                const int tj = MULTI ? (int)o.type : 0;
+                double xj = o.x, yj = o.y, zj = o.z;
+                if (!split) {
+                    const double cj = MULTI ? P.cm[tj] : c;
+                    xj = wrap1(xj + ((o.vx + (double)o.fx * cj) * lam) * dt, bx);
+                    yj = wrap1(yj + ((o.vy + (double)o.fy * cj) * lam) * dt, by);
+                    zj = wrap1(zj + ((o.vz + (double)o.fz * cj) * lam) * dt, bz);
+                }
+                const double dx = min_image(xj - x1, bx);
+                const double dy = min_image(yj - y1, by);
+                const double dz = min_image(zj - z1, bz);
+                const double r2 = (dx * dx + dy * dy) + dz * dz;
+                const PairParam& pp = MULTI ? d.pairs[ti * d.n_types + tj] : p0;
+                if (r2 <= pp.r2max) acc.add(dx, dy, dz, r2, pp);
+            }
+            // integrator.rs:47-53 second half kick, barostat.rs:45-48 scale
+            Rec<FT> out;
+            out.vx = v1x + (double)acc.fx * c;
+            out.vy = v1y + (double)acc.fy * c;
+            out.vz = v1z + (double)acc.fz * c;
+            out.x = x1 * mu;
+            out.y = y1 * mu;
+            out.z = z1 * mu;
+            out.fx = acc.fx;
+            out.fy = acc.fy;
+            out.fz = acc.fz;
+            out.type = a.type;
+            res = out;
+            if (flags & F_WRITE_PV) {
+                d.pot[s] = (double)acc.pot;
+                d.vir[s] = (double)acc.vir;
+            }
+            part[S_MVX] += out.vx * m;
+            part[S_MVY] += out.vy * m;
+            part[S_MVZ] += out.vz * m;
+            part[S_MV2] += m * ((out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz);
+            part[S_POT] += (double)acc.pot;
+            part[S_VIR] += (double)acc.vir;
+            part[S_WV] = fmax(part[S_WV], (out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz);
+            part[S_WA] = fmax(part[S_WA],
+                              (((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
+                               (double)acc.fz * (double)acc.fz) * P.inv_m2[ti]);
+            if (flags & F_INSERT) grid_insert(d, C, R, s, out.x, out.y, out.z);
         }
-        // integrator.rs:47-53 second half kick, barostat.rs:45-48 scale
-        Rec<FT> out;
-        out.vx = v1x + (double)acc.fx * c;
-        out.vy = v1y + (double)acc.fy * c;
-        out.vz = v1z + (double)acc.fz * c;
-        out.x = x1 * mu;
-        out.y = y1 * mu;
-        out.z = z1 * mu;
-        out.fx = acc.fx;
-        out.fy = acc.fy;
-        out.fz = acc.fz;
-        out.type = a.type;
-        dst[s] = out;
-        if (flags & F_WRITE_PV) {
-            d.pot[s] = (double)acc.pot;
-            d.vir[s] = (double)acc.vir;
+        visits += valid ? nn : 0;
+        ((Rec<FT>*)s_tile)[lane] = res;
+        __syncthreads();
+        {
+            u4* g = (u4*)(dst + G.start + t * BS);
+#pragma unroll
+            for (int q = 0; q < RQ; ++q) g[q * BS + lane] = s_tile[q * BS + lane];
         }
-        part[S_MVX] = out.vx * m;
-        part[S_MVY] = out.vy * m;
-        part[S_MVZ] = out.vz * m;
-        part[S_MV2] = m * ((out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz);
-        part[S_POT] = (double)acc.pot;
-        part[S_VIR] = (double)acc.vir;
-        part[S_WV] = (out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz;
-        part[S_WA] = (((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
-                      (double)acc.fz * (double)acc.fz) * P.inv_m2[ti];
-        if (flags & F_INSERT) grid_insert(d, C, R, s, out.x, out.y, out.z);
+        __syncthreads();
+        nn = nnn;
+    }
+    __shared__ int s_vis[BS / 64];
+    if (d.prof) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) visits += __shfl_xor(visits, o, 64);
+        if ((lane & 63) == 0) s_vis[lane >> 6] = visits;
     }
     // a split step stored its kick / drift columns in the first half
     block_store<BS>(part, part_out, pstride, split ? kColsState : (kColsState | kick_cols | kColsDrift));
+    if (d.prof && lane == 0) {
+        uint64_t tot = 0;
+        for (int w = 0; w < BS / 64; ++w) tot += (uint64_t)s_vis[w];
+        prof_block(d, P_STEP, t_start, true, tot);
+    }
 }
 
-// List build launch: runs before k_step every step and returns at once unless
-// its replica is flagged F_BUILD (a separate kernel so that the brick build's
-// registers and LDS do not lower the occupancy of k_step).
-template <typename FT, int BS, bool MULTI>
-__global__ __launch_bounds__(BS) void k_build(DevState d) {
+// Stand-alone list build (md_update_force on the list paths): replicas whose
+// control block is flagged F_BUILD build their lists from grid `fill_epoch`.
+template <typename FT, bool MULTI>
+__global__ __launch_bounds__(kFinBS) void k_build(DevState d) {
     const int b = blockIdx.x;
     const int r = d.bld_rep[b];
-    const RepCtrl& C = d.ctrl[r];
+    const RepCtrl& C = ctrl_in(d, r);
     if ((C.flags & (F_BUILD | F_EXIT)) != F_BUILD) return;
     const RepInfo R = d.reps[r];
-    brick_build<FT, BS, MULTI>(d, C, d.params[r], R, b - R.bld_first);
+    brick_build<FT, kFinBS, MULTI>(d, C, d.params[r], R, b - R.bld_first);
 }
 
 // Force evaluation at the current positions through the current lists
@@ -635,9 +744,8 @@ __global__ __launch_bounds__(BS) void k_force_eval(DevState d) {
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
     const int r = G.rep;
-    const RepCtrl& C = d.ctrl[r];
-    const int64_t s = (int64_t)b * BS + threadIdx.x;
-    const bool valid = (s - G.start) < G.count;
+    const RepCtrl& C = ctrl_in(d, r);
+    const TileRange tr = block_tiles(d, G, b, BS);
     Rec<FT>* __restrict__ rc = (Rec<FT>*)d.rec[C.cur];
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double m = G.mass;
@@ -645,7 +753,9 @@ __global__ __launch_bounds__(BS) void k_force_eval(DevState d) {
     double part[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
-    if (valid) {
+    for (int64_t t = tr.t0; t < tr.t1; ++t) {
+        const int64_t s = G.start + t * BS + threadIdx.x;
+        if (t * BS + threadIdx.x >= G.count) continue;
         Rec<FT> a = rc[s];
         Acc<FT> acc;
         const int nn = d.ncount[s];
@@ -666,15 +776,15 @@ __global__ __launch_bounds__(BS) void k_force_eval(DevState d) {
         rc[s] = a;
         d.pot[s] = (double)acc.pot;
         d.vir[s] = (double)acc.vir;
-        part[S_MVX] = a.vx * m;
-        part[S_MVY] = a.vy * m;
-        part[S_MVZ] = a.vz * m;
-        part[S_MV2] = m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
-        part[S_POT] = (double)acc.pot;
-        part[S_VIR] = (double)acc.vir;
-        part[S_WV] = (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz;
-        part[S_WA] = (((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
-                      (double)acc.fz * (double)acc.fz) / (m * m);
+        part[S_MVX] += a.vx * m;
+        part[S_MVY] += a.vy * m;
+        part[S_MVZ] += a.vz * m;
+        part[S_MV2] += m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
+        part[S_POT] += (double)acc.pot;
+        part[S_VIR] += (double)acc.vir;
+        part[S_WV] = fmax(part[S_WV], (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
+        part[S_WA] = fmax(part[S_WA], (((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
+                                       (double)acc.fz * (double)acc.fz) / (m * m));
     }
     block_store<BS>(part, d.partial + b, d.n_blocks, kColsAll);
 }
@@ -683,11 +793,15 @@ template <typename FT, int BS>
 __global__ __launch_bounds__(BS) void k_insert_all(DevState d) {
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
-    const RepCtrl& C = d.ctrl[G.rep];
-    const int64_t s = (int64_t)b * BS + threadIdx.x;
-    if ((s - G.start) >= G.count) return;
-    const Rec<FT>& a = ((const Rec<FT>*)d.rec[C.cur])[s];
-    grid_insert(d, C, d.reps[G.rep], s, a.x, a.y, a.z);
+    const RepCtrl& C = ctrl_in(d, G.rep);
+    const RepInfo R = d.reps[G.rep];
+    const TileRange tr = block_tiles(d, G, b, BS);
+    for (int64_t t = tr.t0; t < tr.t1; ++t) {
+        const int64_t s = G.start + t * BS + threadIdx.x;
+        if (t * BS + threadIdx.x >= G.count) continue;
+        const Rec<FT>& a = ((const Rec<FT>*)d.rec[C.cur])[s];
+        grid_insert(d, C, R, s, a.x, a.y, a.z);
+    }
 }
 
 // Partial sums of the current state (after an upload or a host-side change).
@@ -695,24 +809,25 @@ template <typename FT, int BS>
 __global__ __launch_bounds__(BS) void k_reduce_state(DevState d) {
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
-    const RepCtrl& C = d.ctrl[G.rep];
-    const int64_t s = (int64_t)b * BS + threadIdx.x;
-    const bool valid = (s - G.start) < G.count;
+    const RepCtrl& C = ctrl_in(d, G.rep);
+    const TileRange tr = block_tiles(d, G, b, BS);
     double part[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
-    if (valid) {
+    for (int64_t t = tr.t0; t < tr.t1; ++t) {
+        const int64_t s = G.start + t * BS + threadIdx.x;
+        if (t * BS + threadIdx.x >= G.count) continue;
         const Rec<FT> a = ((const Rec<FT>*)d.rec[C.cur])[s];
         const double m = G.mass;
-        part[S_MVX] = a.vx * m;
-        part[S_MVY] = a.vy * m;
-        part[S_MVZ] = a.vz * m;
-        part[S_MV2] = m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
-        part[S_POT] = d.pot[s];
-        part[S_VIR] = d.vir[s];
+        part[S_MVX] += a.vx * m;
+        part[S_MVY] += a.vy * m;
+        part[S_MVZ] += a.vz * m;
+        part[S_MV2] += m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
+        part[S_POT] += d.pot[s];
+        part[S_VIR] += d.vir[s];
         const double fx = (double)a.fx, fy = (double)a.fy, fz = (double)a.fz;
-        part[S_WV] = (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz;
-        part[S_WA] = ((fx * fx + fy * fy) + fz * fz) / (m * m);
+        part[S_WV] = fmax(part[S_WV], (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
+        part[S_WA] = fmax(part[S_WA], ((fx * fx + fy * fy) + fz * fz) / (m * m));
     }
     block_store<BS>(part, d.partial + b, d.n_blocks, kColsAll);
 }
@@ -724,26 +839,31 @@ template <int BS>
 __global__ __launch_bounds__(BS) void k_drift(DevState d) {
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
-    const RepCtrl& C = d.ctrl[G.rep];
-    if (C.flags & F_EXIT) return;
+    const RepCtrl& C = ctrl_in(d, G.rep);
+    if (C.flags & F_EXIT) {
+        prof_block(d, P_DRIFT, 0, false, 0);
+        return;
+    }
+    const uint64_t t_start = prof_now(d);
     const RepParams& P = d.params[G.rep];
-    const int64_t s = (int64_t)b * BS + threadIdx.x;
-    const bool valid = (s - G.start) < G.count;
+    const TileRange tr = block_tiles(d, G, b, BS);
     const int cur = C.cur, nx = cur ^ 1;
+    const double dt = P.dt, lam = C.lambda, m = G.mass;
+    const double c = P.cm[G.type];
     double part[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
-    if (valid) {
-        const double dt = P.dt, lam = C.lambda, m = G.mass;
-        const double c = P.cm[G.type];
+    for (int64_t t = tr.t0; t < tr.t1; ++t) {
+        const int64_t s = G.start + t * BS + threadIdx.x;
+        if (t * BS + threadIdx.x >= G.count) continue;
         Rec<double> a = ((const Rec<double>*)d.rec[cur])[s];
         const double vkx = a.vx + a.fx * c;
         const double vky = a.vy + a.fy * c;
         const double vkz = a.vz + a.fz * c;
-        part[S_KVX] = vkx * m;
-        part[S_KVY] = vky * m;
-        part[S_KVZ] = vkz * m;
-        part[S_KV2] = m * ((vkx * vkx + vky * vky) + vkz * vkz);
+        part[S_KVX] += vkx * m;
+        part[S_KVY] += vky * m;
+        part[S_KVZ] += vkz * m;
+        part[S_KV2] += m * ((vkx * vkx + vky * vky) + vkz * vkz);
         a.vx = vkx * lam;
         a.vy = vky * lam;
         a.vz = vkz * lam;
@@ -751,10 +871,11 @@ __global__ __launch_bounds__(BS) void k_drift(DevState d) {
         a.y = wrap1(a.y + a.vy * dt, C.box[1]);
         a.z = wrap1(a.z + a.vz * dt, C.box[2]);
         ((Rec<double>*)d.rec[nx])[s] = a;
-        part[S_VMAX] = (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz;
+        part[S_VMAX] = fmax(part[S_VMAX], (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
     }
     block_store<BS>(part, d.partial + b, d.n_blocks,
                     (P.th_kind == 2 ? kColsKick : 0u) | kColsDrift);
+    prof_block(d, P_DRIFT, t_start, true, 0);
 }
 
 // update_force by all pairs: thread per i, j tiles staged through LDS, j
@@ -768,79 +889,87 @@ __global__ __launch_bounds__(BS) void k_allpairs(DevState d) {
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
     const int r = G.rep;
-    const RepCtrl& C = d.ctrl[r];
-    if (!FORCE_ONLY && (C.flags & F_EXIT)) return;
+    const RepCtrl& C = ctrl_in(d, r);
+    if (!FORCE_ONLY && (C.flags & F_EXIT)) {
+        prof_block(d, P_STEP, 0, false, 0);
+        return;
+    }
+    const uint64_t t_start = prof_now(d);
     const RepParams& P = d.params[r];
     const RepInfo R = d.reps[r];
-    const int64_t s = (int64_t)b * BS + threadIdx.x;
-    const bool valid = (s - G.start) < G.count;
+    const TileRange tr = block_tiles(d, G, b, BS);
     const int cur = C.cur;
     const Rec<double>* __restrict__ src = (const Rec<double>*)d.rec[FORCE_ONLY ? cur : (cur ^ 1)];
     Rec<double>* __restrict__ dst = (Rec<double>*)d.rec[cur];
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const int ti = G.type;
     const int T = d.n_types;
+    const double m = G.mass;
+    const double c = P.cm[ti];
     __shared__ double tx[BS], ty[BS], tz[BS];
     __shared__ uint8_t tt[BS];
-    Rec<double> a;
-    if (valid) a = src[s];
-    double fx = 0.0, fy = 0.0, fz = 0.0, pot = 0.0, vir = 0.0;
     const PairParam p0 = d.pairs[ti * T + ti];
-    for (int64_t t0 = G.start; t0 < R.end; t0 += BS) {
-        const int64_t jl = t0 + threadIdx.x;
-        __syncthreads();
-        if (jl < R.end) {
-            const Rec<double>& o = src[jl];
-            tx[threadIdx.x] = o.x;
-            ty[threadIdx.x] = o.y;
-            tz[threadIdx.x] = o.z;
-            if (MULTI) tt[threadIdx.x] = (uint8_t)o.type;
-        }
-        __syncthreads();
-        const int n = (R.end - t0) < BS ? (int)(R.end - t0) : BS;
-        if (valid) {
-            for (int k = 0; k < n; ++k) {
-                if (t0 + k == s) continue;
-                const double dx = min_image(tx[k] - a.x, bx);
-                const double dy = min_image(ty[k] - a.y, by);
-                const double dz = min_image(tz[k] - a.z, bz);
-                const double r2 = (dx * dx + dy * dy) + dz * dz;
-                const PairParam& pp = MULTI ? d.pairs[ti * T + tt[k]] : p0;
-                if (r2 <= pp.r2max) pair_exact(dx, dy, dz, r2, pp, fx, fy, fz, pot, vir);
-            }
-        }
-    }
     double part[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
-    if (valid) {
-        const double m = G.mass;
-        const double c = P.cm[ti];
-        if (!FORCE_ONLY) {
-            a.vx = a.vx + fx * c;
-            a.vy = a.vy + fy * c;
-            a.vz = a.vz + fz * c;
-            const double mu = C.myu;
-            a.x = a.x * mu;
-            a.y = a.y * mu;
-            a.z = a.z * mu;
+    for (int64_t it = tr.t0; it < tr.t1; ++it) {
+        const int64_t s = G.start + it * BS + threadIdx.x;
+        const bool valid = it * BS + threadIdx.x < G.count;
+        Rec<double> a;
+        if (valid) a = src[s];
+        double fx = 0.0, fy = 0.0, fz = 0.0, pot = 0.0, vir = 0.0;
+        for (int64_t t0 = G.start; t0 < R.end; t0 += BS) {
+            const int64_t jl = t0 + threadIdx.x;
+            __syncthreads();
+            if (jl < R.end) {
+                const Rec<double>& o = src[jl];
+                tx[threadIdx.x] = o.x;
+                ty[threadIdx.x] = o.y;
+                tz[threadIdx.x] = o.z;
+                if (MULTI) tt[threadIdx.x] = (uint8_t)o.type;
+            }
+            __syncthreads();
+            const int n = (R.end - t0) < BS ? (int)(R.end - t0) : BS;
+            if (valid) {
+                for (int k = 0; k < n; ++k) {
+                    if (t0 + k == s) continue;
+                    const double dx = min_image(tx[k] - a.x, bx);
+                    const double dy = min_image(ty[k] - a.y, by);
+                    const double dz = min_image(tz[k] - a.z, bz);
+                    const double r2 = (dx * dx + dy * dy) + dz * dz;
+                    const PairParam& pp = MULTI ? d.pairs[ti * T + tt[k]] : p0;
+                    if (r2 <= pp.r2max) pair_exact(dx, dy, dz, r2, pp, fx, fy, fz, pot, vir);
+                }
+            }
         }
-        a.fx = fx;
-        a.fy = fy;
-        a.fz = fz;
-        dst[s] = a;
-        d.pot[s] = pot;
-        d.vir[s] = vir;
-        part[S_MVX] = a.vx * m;
-        part[S_MVY] = a.vy * m;
-        part[S_MVZ] = a.vz * m;
-        part[S_MV2] = m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
-        part[S_POT] = pot;
-        part[S_VIR] = vir;
-        part[S_WV] = (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz;
-        part[S_WA] = ((fx * fx + fy * fy) + fz * fz) / (m * m);
+        if (valid) {
+            if (!FORCE_ONLY) {
+                a.vx = a.vx + fx * c;
+                a.vy = a.vy + fy * c;
+                a.vz = a.vz + fz * c;
+                const double mu = C.myu;
+                a.x = a.x * mu;
+                a.y = a.y * mu;
+                a.z = a.z * mu;
+            }
+            a.fx = fx;
+            a.fy = fy;
+            a.fz = fz;
+            dst[s] = a;
+            d.pot[s] = pot;
+            d.vir[s] = vir;
+            part[S_MVX] += a.vx * m;
+            part[S_MVY] += a.vy * m;
+            part[S_MVZ] += a.vz * m;
+            part[S_MV2] += m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
+            part[S_POT] += pot;
+            part[S_VIR] += vir;
+            part[S_WV] = fmax(part[S_WV], (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
+            part[S_WA] = fmax(part[S_WA], ((fx * fx + fy * fy) + fz * fz) / (m * m));
+        }
     }
     block_store<BS>(part, d.partial + b, d.n_blocks, FORCE_ONLY ? kColsAll : kColsState);
+    if (!FORCE_ONLY) prof_block(d, P_STEP, t_start, true, 0);
 }
 
 // ---------------------------------------------------------------- finalize
@@ -861,12 +990,23 @@ __device__ __forceinline__ void thermo(const double* S, const GroupInfo& G, cons
     pressure = (th2 + (-S[S_VIR]) * 0.5) / volume / 3.0;
 }
 
+__device__ __forceinline__ void fill_geometry(double (&gbox)[3], int32_t (&gdim)[3], int32_t (&bedge)[3],
+                                              int32_t& bwhole, const RepParams& P, const RepInfo& R,
+                                              const double* gb) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gbox[k] = gb[k];
+    grid_dims(gbox, P.rlist, R.cell_budget, gdim);
+    brick_dims(gdim, R.n_atoms, bedge, &bwhole);
+}
+
+// Geometry of the next grid fill (buffer ins_epoch & 1), chosen with fixed
+// indices so that the control block can live in registers.
 __device__ void set_fill(RepCtrl& C, const RepParams& P, const RepInfo& R, const double* gb) {
     C.ins_epoch = C.fill_epoch + 1u;
-    const int g = C.ins_epoch & 1;
-    for (int k = 0; k < 3; ++k) C.gbox[g][k] = gb[k];
-    grid_dims(C.gbox[g], P.rlist, R.cell_budget, C.gdim[g]);
-    brick_dims(C.gdim[g], R.n_atoms, C.bedge[g], &C.bwhole[g]);
+    if (C.ins_epoch & 1)
+        fill_geometry(C.gbox[1], C.gdim[1], C.bedge[1], C.bwhole[1], P, R, gb);
+    else
+        fill_geometry(C.gbox[0], C.gdim[0], C.bedge[0], C.bwhole[0], P, R, gb);
 }
 
 // Advance a replica's controls after a launch (M_STEP) or prepare the first
@@ -876,6 +1016,8 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
     const int T = R.n_types;
     const double dt = P.dt;
     const bool baro = P.ba_kind == 1;
+    // lists built in this launch describe the current positions exactly
+    const bool built = mode == M_STEP && (C.flags & F_BUILD);
     bool controls = true, decide = true;
     if (mode == M_STEP) {
         C.launches++;
@@ -987,8 +1129,8 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
                                   (1.0 + 1e-12));
         const double w = C.lambda * wmax * dt / C.s;
         const double thr = P.rlist - P.rc_max / C.s - P.margin;
-        const bool need =
-            !(C.flags & F_HAVE_LIST) || P.rebuild_every_step || !(2.0 * (C.D + w) < thr);
+        const bool need = !(C.flags & F_HAVE_LIST) || (P.rebuild_every_step && !built) ||
+                          !(2.0 * (C.D + w) < thr);
         if (need) {
             const double w0 = C.lambda * wmax * dt;
             if (!(2.0 * w0 < P.rlist - P.rc_max - P.margin)) {
@@ -1055,80 +1197,149 @@ __device__ void reduce_rows(const DevState& d, const GroupInfo& G, int lo, int h
     block_store<BS>(acc, out, 1, kColsAll);
 }
 
-// First level of a sliced reduction: grid (slices, replicas); slice sums go
-// to part2 and the finalize adds them in slice order (the kernel boundary
-// orders the two, so no fences are needed).
+// Word-wise block copy (LDS staging of the control structures).
 template <int BS>
-__global__ __launch_bounds__(BS) void k_reduce_slices(DevState d, int mode, int S) {
-    const int sl = blockIdx.x;
-    const int r = blockIdx.y;
-    const RepInfo R = d.reps[r];
-    if (!finalize_reduces(mode, d.ctrl[r].flags)) return;
-    for (int t = 0; t < R.n_types; ++t) {
-        const int g = R.first_group + t;
-        const GroupInfo G = d.groups[g];
-        const int lo = (int)((int64_t)G.nblocks * sl / S);
-        const int hi = (int)((int64_t)G.nblocks * (sl + 1) / S);
-        reduce_rows<BS>(d, G, lo, hi, d.part2 + ((int64_t)g * S + sl) * NS);
-    }
+__device__ __forceinline__ void copy_words(void* dst, const void* src, int bytes) {
+    for (int i = threadIdx.x; i < bytes / 4; i += BS) ((uint32_t*)dst)[i] = ((const uint32_t*)src)[i];
 }
 
-// grid (replicas).  Group sums (directly from the block partials, or from the
-// slice sums of k_reduce_slices), then one thread advances the replica's
-// controls from LDS-staged inputs.
-template <int BS>
-__global__ __launch_bounds__(BS) void k_finalize(DevState d, int mode, int S) {
-    const int r = blockIdx.x;
-    const RepInfo R = d.reps[r];
-    RepCtrl* Cp = d.ctrl + r;
-    const uint32_t f0 = Cp->flags;
-    if (mode == M_STEP && (f0 & F_EXIT)) return;
-    if (finalize_reduces(mode, f0)) {
-        for (int t = 0; t < R.n_types; ++t) {
-            const int g = R.first_group + t;
-            if (S == 1) {
-                const GroupInfo G = d.groups[g];
-                reduce_rows<BS>(d, G, 0, G.nblocks, d.sums + (int64_t)g * NS);
-            } else {
-                const int k = threadIdx.x;
-                if (k < NS) {
-                    const double* row = d.part2 + (int64_t)g * S * NS;
-                    double a = row[k];
-                    for (int q = 1; q < S; ++q)
-                        a = k >= kFirstMax ? fmax(a, row[q * NS + k]) : a + row[q * NS + k];
-                    d.sums[(int64_t)g * NS + k] = a;
-                }
-            }
-        }
-    }
-    if (mode == M_REDUCE) return;
-    // Stage everything the control update reads into LDS with one round of
-    // parallel loads: the update itself is a serial chain on one lane and
-    // would otherwise wait on each global load in turn.
+// Finalize of replica r by one block: group sums (from the block partials of
+// this launch, or the stored ones when the launch did not step), then one
+// thread advances the controls on register copies of the LDS-staged inputs,
+// and the block writes the control block of the next launch.  Returns the
+// launch index when the replica did work in this launch, else -1.
+__device__ int64_t finalize_replica(const DevState& d, int r, int mode) {
+    constexpr int BS = kFinBS;
+    const int T = d.n_types;
+    const int64_t g0 = (int64_t)r * T;  // groups are replica-major
     __shared__ RepCtrl s_ctrl;
     __shared__ RepParams s_par;
     __shared__ GroupInfo s_grp[kMaxTypes];
     __shared__ double s_sums[kMaxTypes * NS];
+    const RepInfo R = d.reps[r];
+    // one system of one type: its partial rows are [0, n_blocks), so their
+    // loads go out together with the staging loads (one memory round trip)
+    const bool single = d.n_reps == 1 && T == 1;
+    double acc[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+    if (single && mode != M_PRIME) {
+        const int64_t nb = d.n_blocks;
+        for (int b0 = threadIdx.x; b0 < nb; b0 += 2 * BS) {
+            double v[2][NS];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int64_t bb = b0 + u * BS;
+#pragma unroll
+                for (int k = 0; k < NS; ++k) v[u][k] = bb < nb ? d.partial[k * nb + bb] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int k = 0; k < NS; ++k) acc[k] = k >= kFirstMax ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
+        }
+    }
+    copy_words<BS>(&s_ctrl, &ctrl_in(d, r), (int)sizeof(RepCtrl));
+    copy_words<BS>(&s_par, d.params + r, (int)sizeof(RepParams));
+    copy_words<BS>(s_grp, d.groups + g0, (int)sizeof(GroupInfo) * T);
     __syncthreads();
-    {
-        const int T = R.n_types;
-        const int nc = (int)(sizeof(RepCtrl) / 4), np = (int)(sizeof(RepParams) / 4);
-        const int ng = (int)(sizeof(GroupInfo) / 4) * T;
-        const int tid = threadIdx.x;
-        if (tid < nc) ((uint32_t*)&s_ctrl)[tid] = ((const uint32_t*)Cp)[tid];
-        if (tid >= 128 && tid - 128 < np)
-            ((uint32_t*)&s_par)[tid - 128] = ((const uint32_t*)(d.params + r))[tid - 128];
-        if (tid >= 256 && tid - 256 < ng)
-            ((uint32_t*)s_grp)[tid - 256] = ((const uint32_t*)(d.groups + R.first_group))[tid - 256];
-        if (tid >= 384 && tid - 384 < T * NS)
-            s_sums[tid - 384] = d.sums[(int64_t)R.first_group * NS + (tid - 384)];
+    const uint32_t f0 = s_ctrl.flags;
+    const int64_t launch = s_ctrl.launches;
+    if (mode == M_STEP && (f0 & F_EXIT)) {
+        // target reached: carry the block over to the buffer the next launch reads
+        if (threadIdx.x == 0) s_ctrl.launches++;
+        __syncthreads();
+        copy_words<BS>(ctrl_out(d, r), &s_ctrl, (int)sizeof(RepCtrl));
+        return -1;
+    }
+    if (finalize_reduces(mode, f0)) {
+        if (single)
+            block_store<BS>(acc, s_sums, 1, kColsAll);
+        else
+            for (int t = 0; t < T; ++t) reduce_rows<BS>(d, s_grp[t], 0, s_grp[t].nblocks, s_sums + t * NS);
+        for (int i = threadIdx.x; i < T * NS; i += BS) d.sums[g0 * NS + i] = s_sums[i];
+    } else {
+        for (int i = threadIdx.x; i < T * NS; i += BS) s_sums[i] = d.sums[g0 * NS + i];
     }
     __syncthreads();
-    if (threadIdx.x == 0) advance(s_ctrl, s_par, R, s_grp, s_sums, mode);
+    if (mode == M_REDUCE) return -1;
+    if (threadIdx.x == 0) {
+        RepCtrl C = s_ctrl;
+        const RepParams P = s_par;
+        advance(C, P, R, s_grp, s_sums, mode);
+        s_ctrl = C;
+    }
     __syncthreads();
-    {
-        const int nc = (int)(sizeof(RepCtrl) / 4);
-        if ((int)threadIdx.x < nc) ((uint32_t*)Cp)[threadIdx.x] = ((const uint32_t*)&s_ctrl)[threadIdx.x];
+    copy_words<BS>(ctrl_out(d, r), &s_ctrl, (int)sizeof(RepCtrl));
+    return mode == M_STEP ? launch : -1;
+}
+
+// Fold the per-block records of this launch's per-slot kernels into the
+// ring entry of the launch (one block).
+__device__ void prof_fold(const DevState& d, int64_t launch, bool drift) {
+    constexpr int BS = kFinBS;
+    __shared__ unsigned long long s_mn[BS / 64], s_mx[BS / 64], s_en[BS / 64];
+    for (int slot = 0; slot < (drift ? 2 : 1); ++slot) {
+        const int k = slot == 0 ? P_STEP : P_DRIFT;
+        const unsigned long long* e = d.prof_blk + (int64_t)k * d.n_blocks * 3;
+        unsigned long long mn = ~0ull, mx = 0ull, en = 0ull;
+        for (int i = threadIdx.x; i < d.n_blocks; i += BS) {
+            mn = e[3 * i] < mn ? e[3 * i] : mn;
+            mx = e[3 * i + 1] > mx ? e[3 * i + 1] : mx;
+            en += e[3 * i + 2];
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const unsigned long long a = __shfl_xor(mn, o, 64), b = __shfl_xor(mx, o, 64);
+            mn = a < mn ? a : mn;
+            mx = b > mx ? b : mx;
+            en += __shfl_xor(en, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            s_mn[threadIdx.x >> 6] = mn;
+            s_mx[threadIdx.x >> 6] = mx;
+            s_en[threadIdx.x >> 6] = en;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < BS / 64; ++w) {
+                mn = s_mn[w] < mn ? s_mn[w] : mn;
+                mx = s_mx[w] > mx ? s_mx[w] : mx;
+                en += s_en[w];
+            }
+            if (mn != ~0ull) {
+                unsigned long long* r = d.prof + ((launch % kProfRing) * P_NUM + k) * 2;
+                r[0] = mn;
+                r[1] = mx;
+            }
+            d.prof[kProfCount] += en;
+        }
+        __syncthreads();
+    }
+}
+
+template <typename FT, bool MULTI, bool BUILD>
+__global__ __launch_bounds__(kFinBS) void k_fin(DevState d, int mode) {
+    const int b = blockIdx.x;
+    const uint64_t t_start = prof_now(d);
+    // the last block folds the per-block records of k_step (or k_drift +
+    // k_allpairs) while the first blocks finalize
+    if (d.prof && mode == M_STEP && b == (int)gridDim.x - 1)
+        prof_fold(d, ctrl_in(d, 0).launches, !BUILD);
+    int64_t launch = -1;
+    if (b < d.n_reps) launch = finalize_replica(d, b, mode);
+    if (BUILD && b < d.n_bld) {
+        const int r = d.bld_rep[b];
+        const RepCtrl& C = ctrl_in(d, r);
+        if ((C.flags & (F_BUILD | F_EXIT)) == F_BUILD) {
+            const RepInfo R = d.reps[r];
+            brick_build<FT, kFinBS, MULTI>(d, C, d.params[r], R, b - R.bld_first);
+            launch = C.launches;
+        }
+    }
+    if (launch >= 0) {
+        __syncthreads();
+        prof_mark(d, launch, P_FIN, t_start);
     }
 }
 
@@ -1139,7 +1350,7 @@ __global__ __launch_bounds__(256) void k_scale(DevState d, int64_t start, int64_
                                                double fac, int which) {
     const int64_t i = start + (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= start + count) return;
-    Rec<FT>& a = ((Rec<FT>*)d.rec[d.ctrl[0].cur])[i];
+    Rec<FT>& a = ((Rec<FT>*)d.rec[ctrl_in(d, 0).cur])[i];
     if (which == 0) {
         a.vx *= fac;
         a.vy *= fac;
@@ -1155,13 +1366,24 @@ template <typename FT, int BS>
 __global__ __launch_bounds__(BS) void k_wrap(DevState d) {
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
-    const RepCtrl& C = d.ctrl[G.rep];
-    const int64_t s = (int64_t)b * BS + threadIdx.x;
-    if ((s - G.start) >= G.count) return;
-    Rec<FT>& a = ((Rec<FT>*)d.rec[C.cur])[s];
-    a.x = wrap1(a.x, C.box[0]);
-    a.y = wrap1(a.y, C.box[1]);
-    a.z = wrap1(a.z, C.box[2]);
+    const RepCtrl& C = ctrl_in(d, G.rep);
+    const TileRange tr = block_tiles(d, G, b, BS);
+    for (int64_t t = tr.t0; t < tr.t1; ++t) {
+        const int64_t s = G.start + t * BS + threadIdx.x;
+        if (t * BS + threadIdx.x >= G.count) continue;
+        Rec<FT>& a = ((Rec<FT>*)d.rec[C.cur])[s];
+        a.x = wrap1(a.x, C.box[0]);
+        a.y = wrap1(a.y, C.box[1]);
+        a.z = wrap1(a.z, C.box[2]);
+    }
+}
+
+// Timestamp ring reset: starts at the largest tick, ends at zero; the list
+// entry counter after the ring at zero.
+__global__ __launch_bounds__(256) void k_prof_init(DevState d) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < kProfCount) d.prof[i] = (i & 1) ? 0ull : ~0ull;
+    if (i == kProfCount) d.prof[i] = 0ull;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1173,15 +1395,6 @@ __global__ __launch_bounds__(BS) void k_wrap(DevState d) {
         case 256: F(256); break;              \
         case 512: F(512); break;              \
         default: return hipErrorInvalidValue; \
-    }
-
-#define MD_LAUNCH_FT_MULTI_G(K, B, G)                                                     \
-    if (fp32) {                                                                            \
-        if (multi) hipLaunchKernelGGL((K<float, B, true>), dim3(G), dim3(B), 0, st, d);      \
-        else hipLaunchKernelGGL((K<float, B, false>), dim3(G), dim3(B), 0, st, d);           \
-    } else {                                                                               \
-        if (multi) hipLaunchKernelGGL((K<double, B, true>), dim3(G), dim3(B), 0, st, d);     \
-        else hipLaunchKernelGGL((K<double, B, false>), dim3(G), dim3(B), 0, st, d);          \
     }
 
 #define MD_LAUNCH_FT_MULTI(K, B)                                                               \
@@ -1201,11 +1414,22 @@ hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_build(const DevState& d, bool fp32, int bs, hipStream_t st) {
+hipError_t launch_build(const DevState& d, bool fp32, hipStream_t st) {
     const bool multi = d.n_types > 1;
-#define L(B) MD_LAUNCH_FT_MULTI_G(k_build, B, d.n_bld)
-    MD_BS_SWITCH(bs, L)
-#undef L
+    if (d.n_bld <= 0) return hipSuccess;
+    if (fp32) {
+        if (multi) hipLaunchKernelGGL((k_build<float, true>), dim3(d.n_bld), dim3(kFinBS), 0, st, d);
+        else hipLaunchKernelGGL((k_build<float, false>), dim3(d.n_bld), dim3(kFinBS), 0, st, d);
+    } else {
+        if (multi) hipLaunchKernelGGL((k_build<double, true>), dim3(d.n_bld), dim3(kFinBS), 0, st, d);
+        else hipLaunchKernelGGL((k_build<double, false>), dim3(d.n_bld), dim3(kFinBS), 0, st, d);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_prof_init(const DevState& d, hipStream_t st) {
+    const int n = (int)kProfCount + 1;
+    hipLaunchKernelGGL(k_prof_init, dim3((n + 255) / 256), dim3(256), 0, st, d);
     return hipGetLastError();
 }
 
@@ -1217,11 +1441,20 @@ hipError_t launch_force_eval(const DevState& d, bool fp32, int bs, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const DevState& d, int mode, int n_reps, int slices, hipStream_t st) {
-    if (mode == M_PRIME) slices = 1;
-    if (slices > 1)
-        hipLaunchKernelGGL((k_reduce_slices<512>), dim3(slices, n_reps), dim3(512), 0, st, d, mode, slices);
-    hipLaunchKernelGGL((k_finalize<512>), dim3(n_reps), dim3(512), 0, st, d, mode, slices);
+hipError_t launch_finalize(const DevState& d, int mode, bool build, bool fp32, hipStream_t st) {
+    const bool multi = d.n_types > 1;
+    if (build && mode == M_STEP && d.n_bld > 0) {
+        const int grid = d.n_bld > d.n_reps ? d.n_bld : d.n_reps;
+        if (fp32) {
+            if (multi) hipLaunchKernelGGL((k_fin<float, true, true>), dim3(grid), dim3(kFinBS), 0, st, d, mode);
+            else hipLaunchKernelGGL((k_fin<float, false, true>), dim3(grid), dim3(kFinBS), 0, st, d, mode);
+        } else {
+            if (multi) hipLaunchKernelGGL((k_fin<double, true, true>), dim3(grid), dim3(kFinBS), 0, st, d, mode);
+            else hipLaunchKernelGGL((k_fin<double, false, true>), dim3(grid), dim3(kFinBS), 0, st, d, mode);
+        }
+    } else {
+        hipLaunchKernelGGL((k_fin<double, false, false>), dim3(d.n_reps), dim3(kFinBS), 0, st, d, mode);
+    }
     return hipGetLastError();
 }
 

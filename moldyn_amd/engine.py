@@ -35,7 +35,8 @@ class Engine:
     """
 
     def __init__(self, counts, masses, box, force_mode="fp32_list", skin=None, block_size=256,
-                 cells_per_atom=2.0, graph_chunk=32, list_capacity=0, cell_capacity=0):
+                 cells_per_atom=2.0, graph_chunk=32, list_capacity=0, cell_capacity=0,
+                 tiles_per_block=0):
         self._lib = L.load()
         counts = np.ascontiguousarray(counts, dtype=np.int64)
         if counts.ndim == 1:
@@ -63,6 +64,7 @@ class Engine:
         opt.graph_chunk = int(graph_chunk)
         opt.list_capacity = int(list_capacity)
         opt.cell_capacity = int(cell_capacity)
+        opt.tiles_per_block = int(tiles_per_block)
         self._chk(self._lib.md_set_options(self._h, C.byref(opt)))
         self.force_mode = force_mode
 

@@ -436,3 +436,87 @@ def test_reupload_into_used_engine(mode):
     f.update_force()
     f.step(0.002, 200)
     assert np.array_equal(e.download()["x"], f.download()["x"])
+
+
+@pytest.mark.parametrize("mode", FP64_MODES)
+def test_multitype_npt(mode):
+    """Two types under the Berendsen thermostat and barostat: lambda is
+    overwritten per type and the box is scaled once per type
+    (integrator.rs:16-27,54-58, barostat.rs:45-48), so positions and box
+    scale inconsistently and the list paths rebuild before every step."""
+    import moldyn_amd as M
+    rng = np.random.default_rng(11)
+    n0, n1 = 90, 70
+    box = 3.2
+    pos = rng.uniform(0, box, size=(n0 + n1, 3))
+    vel = rng.normal(0, 0.3, size=(n0 + n1, 3))
+    masses = [AR_MASS, 39.948]
+    s = O.State([n0, n1], masses, [box] * 3, pos=pos, vel=vel)
+    e = M.Engine([n0, n1], masses, [box] * 3, force_mode=mode)
+    e.upload(pos, vel, [box] * 3)
+    e.update_force()
+    th = M.thermostat("berendsen", tau=2.0, target_t=200.0)
+    ba = M.barostat("berendsen", beta=1.0, tau=5.0, target_p=20.0)
+    e.step(0.002, 60, th, ba)
+    d = e.download()
+    oth = O.Thermostat.berendsen(2.0)
+    oba = O.Barostat.berendsen(1.0, 5.0)
+    run_oracle(s, 60, 0.002, oth, 200.0, oba, 20.0)
+    assert np.allclose(d["box"], s.box, rtol=1e-10, atol=0)
+    dx = d["x"] - s.pos
+    dx -= s.box * np.round(dx / s.box)
+    assert np.abs(dx).max() < 1e-9
+    assert np.allclose(d["v"], s.vel, rtol=1e-9, atol=1e-10)
+    assert abs(th.lambda_ - oth.lambda_) < 1e-12 and abs(ba.myu - oba.myu) < 1e-12
+    if mode == "fp64_list":
+        assert e.stats()["rebuilds"] >= 60
+
+
+@pytest.mark.parametrize("mode", ["fp64_list", "fp32_list"])
+def test_tiles_per_block(mode):
+    """Blocks that own several particle tiles (software-pipelined loads):
+    forces and NVE trajectories are bitwise those of one tile per block;
+    under a thermostat only the reduction grouping differs."""
+    import moldyn_amd as M
+    s = liquid_state(6, temperature=150.0)  # 864 atoms, 14 tiles of 64
+    runs = []
+    for tpb in (1, 3, 16):
+        e = engine_for(s, mode, block_size=64, tiles_per_block=tpb)
+        e.update_force()
+        f0 = e.download(("force",))["force"]
+        e.step(0.002, 200)
+        d = e.download()
+        th = M.thermostat("berendsen", tau=1.0, target_t=100.0)
+        e.step(0.002, 100, th)
+        runs.append((f0, d, e.download(), th.lambda_))
+        e.close()
+    f0, d0, t0, l0 = runs[0]
+    for f, d, t, lam in runs[1:]:
+        assert np.array_equal(f, f0)
+        assert np.array_equal(d["x"], d0["x"]) and np.array_equal(d["v"], d0["v"])
+        assert np.allclose(t["v"], t0["v"], rtol=1e-9, atol=1e-12)
+        assert abs(lam - l0) < 1e-12
+
+
+def test_device_kernel_timing():
+    """Per-launch device timestamps cover every launch of a graph-replayed
+    run and leave its results unchanged."""
+    import moldyn_amd as M
+    s = sc_state(12, seed=4)
+    outs = []
+    for prof in (False, True):
+        e = engine_for(s, "fp32_list")
+        e.update_force()
+        e.set_profiling(prof)
+        th = M.thermostat("berendsen", tau=10.0, target_t=300.0)
+        e.step(0.002, 700, th)
+        outs.append(e.download(("x", "v")))
+        if prof:
+            ms, n = e.kernel_time("step")
+            fms, fn = e.kernel_time("finalize")
+            st = e.stats()
+            assert n >= 700 - st["rebuilds"] - 1 and n <= st["launches"]
+            assert fn == st["launches"]
+            assert 0.0 < ms / n < 5.0 and 0.0 < fms / fn < 5.0
+        e.close()
+    assert np.array_equal(outs[0]["x"], outs[1]["x"]) and np.array_equal(outs[0]["v"], outs[1]["v"])
