@@ -47,6 +47,166 @@ def controls(M):
     return th, ba
 
 
+def nve_64k(M, steps=10000):
+    """64,000-atom argon, SC 40^3 at a = 3.338339 with uniform jitter of
+    +-0.02 a (seed 2), MB 120 K (seed 2), NVE, skin 0.3 sigma, FP32 forces:
+    energy drift per ns and the force RMSE against the FP64 path."""
+    n_side = 40
+    n = n_side ** 3
+    rng = np.random.default_rng(2)
+    pos = M.lattice("u", [n_side] * 3, A_LAT) + rng.uniform(-0.02, 0.02, (n, 3)) * A_LAT
+    box = np.full(3, n_side * A_LAT)
+    pos = np.mod(pos, box)
+    vel = M.mb_velocities(n, 120.0, MASS, 2)
+    e = M.Engine([n], [MASS], box, force_mode="fp32_list", skin=0.3 * 0.3418)
+    e.upload(pos, vel, box)
+    e.update_force()
+    m0 = e.macro()
+    t0 = time.perf_counter()
+    e.step(DT, steps)
+    wall = time.perf_counter() - t0
+    m1 = e.macro()
+    d = e.download(("x", "v", "force", "box"))
+    e.close()
+    ref = M.Engine([n], [MASS], d["box"], force_mode="fp64_list")
+    ref.upload(d["x"], d["v"], d["box"])
+    ref.update_force()
+    f64 = ref.download(("force",))["force"]
+    ref.close()
+    e0 = m0["kinetic"] + m0["potential"]
+    e1 = m1["kinetic"] + m1["potential"]
+    ns = steps * DT * 1e-3
+    err = d["force"] - f64
+    rms_f = float(np.sqrt(np.mean(f64 ** 2)))
+    return {"workload": "argon SC 40^3 (64000 atoms), +-0.02a jitter, 120 K, NVE, skin 0.3 sigma, FP32 forces",
+            "steps": steps, "atom_steps_per_s": n * steps / wall,
+            "energy_drift_per_ns": (e1 - e0) / ns, "energy_drift_rel_per_ns": (e1 - e0) / abs(e0) / ns,
+            "total_energy_start": e0,
+            "force_rmse_rel_final": float(np.sqrt(np.mean(err ** 2))) / rms_f if rms_f > 0 else 0.0,
+            "force_max_abs_err_final": float(np.abs(err).max())}
+
+
+def replicas_256(M, steps=2000, seq_sample=48):
+    """256 argon replicas in one engine: N_r = round(exp(U[ln 500, ln 32768]))
+    (seed 4) on ceil(N^(1/3))^3 SC grids filled in index order, a_r in
+    U[3.0, 4.2], Berendsen T0 in U[80, 400] K and P0 in U[0.5, 5.0]; the
+    aggregate rate against the same replicas run one engine at a time."""
+    rng = np.random.default_rng(4)
+    R = 256
+    counts = np.rint(np.exp(rng.uniform(np.log(500), np.log(32768), R))).astype(np.int64)
+    a = rng.uniform(3.0, 4.2, R)
+    t0 = rng.uniform(80.0, 400.0, R)
+    p0 = rng.uniform(0.5, 5.0, R)
+    pos, vel, boxes = [], [], []
+    for r in range(R):
+        side = int(np.ceil(counts[r] ** (1.0 / 3.0) - 1e-9))
+        x = M.lattice("u", [side] * 3, a[r])[:counts[r]]
+        pos.append(x)
+        vel.append(M.mb_velocities(int(counts[r]), t0[r], MASS, 1000 + r))
+        boxes.append([side * a[r]] * 3)
+    boxes = np.array(boxes)
+    L = M.engine.L
+    e = M.Engine(counts[:, None], [MASS], boxes, force_mode="fp32_list")
+    e.upload(np.concatenate(pos), np.concatenate(vel), boxes)
+    e.update_force()
+    th = (L.MdThermostat * R)(*[M.thermostat("berendsen", 10.0, float(t0[r])) for r in range(R)])
+    ba = (L.MdBarostat * R)(*[M.barostat("berendsen", 1.0, 5.0, float(p0[r])) for r in range(R)])
+    e.step(DT, 20, th, ba)
+    tb = time.perf_counter()
+    e.step(DT, steps, th, ba)
+    wall_b = time.perf_counter() - tb
+    e.close()
+    total = int(counts.sum())
+    # one engine per replica, a sample of them, scaled to all atoms
+    idx = rng.choice(R, seq_sample, replace=False)
+    wall_s, atoms_s = 0.0, 0
+    for r in idx:
+        one = M.Engine([int(counts[r])], [MASS], boxes[r], force_mode="fp32_list")
+        one.upload(pos[r], vel[r], boxes[r])
+        one.update_force()
+        th1 = M.thermostat("berendsen", 10.0, float(t0[r]))
+        ba1 = M.barostat("berendsen", 1.0, 5.0, float(p0[r]))
+        one.step(DT, 20, th1, ba1)
+        ts = time.perf_counter()
+        one.step(DT, steps, th1, ba1)
+        wall_s += time.perf_counter() - ts
+        atoms_s += int(counts[r])
+        one.close()
+    seq_rate = atoms_s * steps / wall_s
+    return {"workload": "256 argon replicas, N log-uniform in [500, 32768] (seed 4), a in [3.0, 4.2], "
+                        "Berendsen T0 in [80, 400] K, P0 in [0.5, 5.0], FP32 forces",
+            "atoms": total, "steps": steps, "batched_atom_steps_per_s": total * steps / wall_b,
+            "batched_ms_per_step": 1e3 * wall_b / steps,
+            "sequential_atom_steps_per_s": seq_rate,
+            "sequential_sample": f"{seq_sample} replicas ({atoms_s} atoms) one engine each, timed steps only",
+            "speedup": total * steps / wall_b / seq_rate}
+
+
+def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log):
+    """Surrogate: 4,096-atom LJ argon frames from thermostatted SC runs at
+    lattice edges 0.38-0.7 nm and 80-400 K, labelled by the FP64 LJ path,
+    trained on the GPU; inference on a 64^3 (262,144-atom) SC grid at
+    a = 0.47 nm, Berendsen T 150 K and P at the state's own pressure, against
+    the FP32 LJ path on the same run."""
+    from moldyn_amd import mlp
+    t0 = time.perf_counter()
+    fr = mlp.make_frames(n_side=16, n_frames=n_frames, runs=20, equil=400, every=20, seed=5)
+    t_gen = time.perf_counter() - t0
+    held = np.arange(n_frames - 50, n_frames, 5)  # every 5th of the last run's frames, held out
+    train_idx = np.setdiff1d(np.arange(n_frames), np.arange(n_frames - 50, n_frames))
+    t0 = time.perf_counter()
+    sur = mlp.train(fr.subset(train_idx), epochs=epochs, batch_frames=16, lr=2e-3, lr_final=1e-4, log=log)
+    t_train = time.perf_counter() - t0
+    # held-out force error of the deployed (BF16 MFMA) surrogate
+    rel = []
+    for k in held:
+        x, box = fr.pos[k], fr.box[k]
+        e = M.Engine([len(x)], [MASS], box, force_mode="mlp")
+        sur.apply(e, judge_every=0, range_check=False)
+        e.upload(x, np.zeros_like(x), box)
+        e.update_force()
+        f = e.download(("force",))["force"]
+        e.close()
+        rel.append(float(np.sqrt(np.mean((f - fr.f_ref[k]) ** 2)) / np.sqrt(np.mean(fr.f_ref[k] ** 2))))
+    # inference run: 64^3 SC at a = 0.47, 150 K
+    n_side, a = 64, 0.47
+    n = n_side ** 3
+    box = np.full(3, n_side * a)
+    rng = np.random.default_rng(6)
+    pos = np.mod(M.lattice("u", [n_side] * 3, a) + rng.uniform(-0.02, 0.02, (n, 3)) * a, box)
+    vel = M.mb_velocities(n, 150.0, MASS, 6)
+    probe = M.Engine([n], [MASS], box, force_mode="fp32_list")
+    probe.upload(pos, vel, box)
+    probe.update_force()
+    p_state = probe.macro()["pressure"]
+    probe.close()
+    res = {}
+    for mode in ("mlp", "fp32_list"):
+        e = M.Engine([n], [MASS], box, force_mode=mode)
+        if mode == "mlp":
+            sur.apply(e, judge_every=100, range_check=True)
+        e.upload(pos, vel, box)
+        e.update_force()
+        th = M.thermostat("berendsen", tau=1.0, target_t=150.0)
+        ba = M.barostat("berendsen", beta=1.0, tau=5.0, target_p=p_state)
+        e.step(DT, 20, th, ba)
+        ts = time.perf_counter()
+        e.step(DT, steps, th, ba)
+        wall = time.perf_counter() - ts
+        res[mode] = {"atom_steps_per_s": n * steps / wall, "ms_per_step": 1e3 * wall / steps}
+        if mode == "mlp":
+            res[mode].update(e.mlp_stats())
+        e.close()
+    return {"workload": f"64^3 SC argon ({n} atoms) at a = {a} nm, 150 K, Berendsen T and P "
+                        f"(P0 = initial pressure {p_state:.4g}), dt {DT}, {steps} steps",
+            "training": {"frames": int(len(train_idx)), "atoms_per_frame": 4096, "epochs": epochs,
+                         "frame_generation_s": t_gen, "training_s": t_train,
+                         "final_loss": sur.history[-1], "first_loss": sur.history[0]},
+            "heldout_force_rmse_rel": float(np.mean(rel)),
+            "mlp": res["mlp"], "lj_fp32": res["fp32_list"],
+            "speed_vs_lj": res["mlp"]["atom_steps_per_s"] / res["fp32_list"]["atom_steps_per_s"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,15 +217,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--skin", type=float, default=None)
+    ap.add_argument("--skin", type=float, default=0.5, help="nm (the autotuner's choice for this workload)")
     ap.add_argument("--block-size", type=int, default=256)
-    ap.add_argument("--cells-per-atom", type=float, default=2.0)
+    ap.add_argument("--cells-per-atom", type=float, default=4.0)
     ap.add_argument("--tiles-per-block", type=int, default=0)
     ap.add_argument("--graph-chunk", type=int, default=32, help="launches per captured graph (1: none)")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-launch device timestamps in the timed region (no roofline)")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-steps", type=int, default=300)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the 64k NVE, 256-replica and surrogate measurements")
+    ap.add_argument("--mlp-frames", type=int, default=2000)
+    ap.add_argument("--mlp-epochs", type=int, default=40)
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_step_1M_npt_traffic.txt"),
                     help="rocprof_summary traffic file holding bytes_per_launch of the step kernel")
     args = ap.parse_args()
@@ -78,7 +242,7 @@ def main():
 
     n, pos, vel, box = initial_state(M, args.n_side)
     log(f"[bench] state ready: {n} atoms, box {box[0]:.4f}")
-    opts = {"skin": args.skin if args.skin is not None else 0.3 * 0.3418,
+    opts = {"skin": args.skin,
             "block_size": args.block_size, "cells_per_atom": args.cells_per_atom,
             "tiles_per_block": args.tiles_per_block}
     if not args.no_tune:
@@ -99,29 +263,38 @@ def main():
                            steps=args.tune_steps, warmup=20, log=log)
         log(f"[bench] autotuned options: {opts}")
     e = M.Engine([n], [MASS], box, force_mode="fp32_list", graph_chunk=args.graph_chunk, **opts)
-    e.upload(pos, vel, box)
-    e.update_force()
-    th, ba = controls(M)
-    e.step(DT, args.warmup, th, ba)
-    log(f"[bench] warmup {args.warmup} steps done")
 
-    # per-launch device timestamps of every kernel in the timed region (the
-    # graphs keep running; each launch is timed from its first block's start
-    # to its last block's end on the GPU clock)
-    e.set_profiling(not args.no_profile)
-    t0 = time.perf_counter()
-    e.step(DT, args.steps, th, ba)  # returns after the device finished
-    t1 = time.perf_counter()
-    st = e.stats()
-    value = n * args.steps / (t1 - t0)
-    ms_per_step = 1e3 * (t1 - t0) / args.steps
+    def run(profile):
+        # warmup then the timed steps, from the initial state
+        e.upload(pos, vel, box)
+        e.update_force()
+        th, ba = controls(M)
+        e.step(DT, args.warmup, th, ba)
+        e.set_profiling(profile)
+        t0 = time.perf_counter()
+        e.step(DT, args.steps, th, ba)  # returns after the device finished
+        t1 = time.perf_counter()
+        return t1 - t0, e.stats()
+
+    wall, st = run(False)
+    value = n * args.steps / wall
+    ms_per_step = 1e3 * wall / args.steps
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value:.4e} atom-steps/s; stats {st}")
-    step_ms, step_n = e.kernel_time("step")
-    fin_ms, fin_n = e.kernel_time("finalize")
-    e.set_profiling(False)
+    # The same trajectory again with per-launch device timestamps on every
+    # kernel (the graphs keep running; each launch is timed from its first
+    # block's start to its last block's end on the GPU wall clock).
+    step_ms = fin_ms = span_ms = 0.0
+    step_n = fin_n = span_n = 0
+    if not args.no_profile:
+        wall_p, st = run(True)
+        step_ms, step_n = e.kernel_time("step")
+        fin_ms, fin_n = e.kernel_time("finalize")
+        span_ms, span_n = e.kernel_time("step", span=True)
+        e.set_profiling(False)
+        log(f"[bench] profiled pass: {1e3 * wall_p / args.steps:.4f} ms/step")
     avg_step_s = step_ms / max(step_n, 1) * 1e-3
     # mean list length over the timed launches (entries read / atoms / launches)
-    n_l = st["list_visits"] / (n * max(step_n, 1)) if step_n else st["mean_list_len"]
+    n_l = st["list_visits"] / (n * max(span_n, 1)) if span_n else st["mean_list_len"]
     bytes_per_launch = n * (124.0 + 28.0 * n_l)
     achieved = bytes_per_launch / avg_step_s / 1e9 if step_n else 0.0
     log(f"[bench] k_step avg {avg_step_s * 1e6:.2f} us over {step_n} launches, "
@@ -131,6 +304,9 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": f"k_step<float,{opts['block_size']}>", "avg_us": round(avg_step_s * 1e6, 3),
+                "avg_us_basis": "stream time: previous kernel end to kernel end on the GPU clock, "
+                                "back-to-back launches (the interval rocprofv3 reports)",
+                "span_avg_us": round(span_ms / max(span_n, 1) * 1e3, 3),
                 "launches_timed": step_n, "mean_list_len_timed": round(n_l, 5),
                 "k_fin_avg_us": round(fin_ms / max(fin_n, 1) * 1e3, 3),
                 "algorithmic_bytes_per_launch": round(bytes_per_launch)}
@@ -174,6 +350,18 @@ def main():
                  "speedup": value / pv}
         log(f"[bench] fp64 plain all-pairs: {pv:.4e} atom-steps/s ({plain['ms_per_step']:.1f} ms/step)")
 
+    extras = {}
+    if not args.no_extras:
+        log("[bench] 64k NVE")
+        extras["nve_64k"] = nve_64k(M)
+        log(f"[bench] nve_64k: {extras['nve_64k']}")
+        log("[bench] 256 replicas")
+        extras["replicas_256"] = replicas_256(M)
+        log(f"[bench] replicas_256: {extras['replicas_256']}")
+        log("[bench] surrogate")
+        extras["mlp_surrogate"] = mlp_surrogate(M, n_frames=args.mlp_frames, epochs=args.mlp_epochs)
+        log(f"[bench] mlp_surrogate: {extras['mlp_surrogate']}")
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -210,6 +398,7 @@ def main():
         "cpu_baseline": cpu,
         "fp64_plain": plain,
         "force_error_vs_fp64": force_err,
+        "other_workloads": extras or None,
     }
     print(json.dumps(out), flush=True)
 

@@ -46,7 +46,8 @@ enum {
 enum {
     MD_FORCE_FP64_ALLPAIRS = 0, /* O(N^2), reference criterion and sum order: bitwise update_force */
     MD_FORCE_FP64_LIST = 1,     /* cell grid + Verlet list, FP64: same pair set and sum order */
-    MD_FORCE_FP32_LIST = 2      /* cell grid + Verlet list, FP32 pair forces, FP64 reductions */
+    MD_FORCE_FP32_LIST = 2,     /* cell grid + Verlet list, FP32 pair forces, FP64 reductions */
+    MD_FORCE_MLP = 3            /* learned per-atom energy surrogate (one particle type), see md_mlp_set_params */
 };
 
 /* Thermostat, initializer/thermostat.rs:4-20 (enum Thermostat) plus the
@@ -177,6 +178,59 @@ int md_init_lattice(int kind, const double start[3], const int64_t grid[3], doub
  * uses SplitMix64 + Box-Muller so inputs are reproducible). */
 int md_init_mb_velocities(double* vel, int64_t n, double temperature, double mass, uint64_t seed);
 
+/* Learned per-atom energy surrogate (no reference counterpart: the reference
+ * has Lennard-Jones only).  With force_mode MD_FORCE_MLP, md_update_force and
+ * md_step use E_i = e_scale * MLP(x_i) + e_shift with
+ *   x_i = (G_i - g_mean) * g_inv_std,
+ *   G_ik = sum_{j, r_ij < 2.5 sigma} exp(-(r_ij - mu_k)^2 / (2 w^2)) (cos(pi r_ij / 2.5 sigma) + 1) / 2,
+ *   mu_k = sigma (0.8 + 1.7 k / 31), w = 0.1 sigma, k = 0..31 (sigma of type pair (0, 0)),
+ *   MLP = 32-128-128-1 with SiLU, BF16 weights and activations, FP32 accumulation,
+ * and forces F_i = -dE/dr_i.  theta holds MD_MLP_NUM_PARAMS floats:
+ *   W1 [128][32] | b1 [128] | W2 [128][128] | b2 [128] | w3 [128] | b3.
+ * g_lo / g_hi (nullable): per-feature range seen in training; a launch in
+ * which any feature leaves it uses the LJ forces instead (counted as a
+ * fallback).  judge_every > 0: every that many steps the LJ forces of the
+ * same positions are computed beside and the force RMSE recorded. */
+#define MD_MLP_NUM_PARAMS 20865
+int md_mlp_set_params(md_ctx* ctx, const float* theta, const float* g_mean, const float* g_inv_std,
+                      const float* g_lo, const float* g_hi, double e_scale, double e_shift,
+                      int judge_every);
+/* out[0] relative force RMSE (vs LJ) at the last check, out[1] largest over
+ * the checks, out[2] checks, out[3] fallback steps, out[4] RMS LJ force
+ * component at the last check. */
+int md_mlp_get_stats(md_ctx* ctx, int replica, double out[5]);
+
+/* Surrogate training on the device (no reference counterpart).  A data set
+ * of n_frames periodic frames of n_atoms atoms each: positions (frame-major,
+ * N x 3), boxes (3 per frame), per-atom energy labels e_ref (the LJ
+ * Particle::potential / 2) and forces f_ref.  The features of every frame are
+ * computed once at creation (all pairs, basis of md_mlp_set_params with the
+ * given sigma).  Training runs FP32 kernels: forward, analytic force
+ * backward, loss L = w_e mean (E - e_ref)^2 + w_f mean (F - f_ref)^2 and its
+ * gradient through the forces (double backward), Adam. */
+typedef struct md_trainer md_trainer;
+int md_trainer_create(md_trainer** out, int n_frames, int n_atoms, const double* pos, const double* boxes,
+                      const double* e_ref, const double* f_ref, double sigma);
+void md_trainer_destroy(md_trainer* tr);
+const char* md_trainer_last_error(const md_trainer* tr);
+/* Per-feature mean, standard deviation, minimum and maximum over the data set
+ * (out: 4 x 32). */
+int md_trainer_feature_stats(md_trainer* tr, double out[128]);
+/* Parameters (layout of md_mlp_set_params) and normalisation; resets Adam. */
+int md_trainer_set_params(md_trainer* tr, const float* theta, const float* g_mean, const float* g_inv_std,
+                          double e_scale, double e_shift);
+int md_trainer_get_params(md_trainer* tr, float* theta);
+/* Loss over the whole data set: out = {L, energy MSE, force MSE per component};
+ * grad (nullable) receives dL/dtheta.  batch_frames bounds the work arrays. */
+int md_trainer_loss(md_trainer* tr, double w_e, double w_f, int batch_frames, double out[3], float* grad);
+/* One epoch: minibatches of batch_frames frames (in the given order, or
+ * sequential when order is NULL), one Adam step (beta 0.9, 0.999) per
+ * minibatch on its mean loss; out = mean over the minibatches. */
+int md_trainer_epoch(md_trainer* tr, int batch_frames, double lr, double w_e, double w_f, const int32_t* order,
+                     double out[3]);
+/* Per-atom energies and forces of one frame under the current parameters. */
+int md_trainer_predict(md_trainer* tr, int frame, double* e_out, double* f_out);
+
 /* Observability (no reference counterpart). */
 typedef struct {
     int64_t steps;          /* real steps taken in the last md_step call */
@@ -193,15 +247,20 @@ typedef struct {
     int64_t list_visits;    /* neighbour-list entries read by the step kernel while profiling */
 } md_stats;
 int md_get_stats(md_ctx* ctx, md_stats* out);
-/* Per-kernel device time of every launch of md_step: the step kernel
- * (kernel 0), the finalize / list-build kernel (1), the all-pairs force kernel
- * (2) and the drift kernel (3) (kernel 4, the former separate list build,
- * reports nothing).  Each launch of a kernel is timed on the device from the
- * start of its first block to the end of its last block with the GPU wall
- * clock (s_memrealtime), so the captured graphs keep running unchanged.
- * md_set_profiling(ctx, 1) also clears the totals. */
+/* Per-kernel device time of the launches of md_step: the step kernel
+ * (kernel 0; the pair-force kernel on the surrogate path), the finalize /
+ * list-build kernel (1), the all-pairs force kernel (2) and the drift kernel
+ * (3; the feature kernel on the surrogate path); kernel 4 reports nothing.
+ * Every block records its start and end on the GPU wall clock
+ * (s_memrealtime), so the captured graphs keep running unchanged.
+ *   total_ms / launches: stream time, from the end of the previous kernel of
+ *     the launch sequence to the end of this one, over launches that ran back
+ *     to back (the interval a dispatch trace such as rocprofv3 reports);
+ *   span_ms / span_launches: first block start to last block end.
+ * Any output may be NULL.  md_set_profiling(ctx, 1) also clears the totals. */
 int md_set_profiling(md_ctx* ctx, int on);
-int md_kernel_time(md_ctx* ctx, int kernel, double* total_ms, int64_t* launches);
+int md_kernel_time(md_ctx* ctx, int kernel, double* total_ms, int64_t* launches, double* span_ms,
+                   int64_t* span_launches);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,8 @@ MD_ERR_NO_DEVICE = -7
 MD_FORCE_FP64_ALLPAIRS = 0
 MD_FORCE_FP64_LIST = 1
 MD_FORCE_FP32_LIST = 2
+MD_FORCE_MLP = 3
+MD_MLP_NUM_PARAMS = 20865
 
 # every entry point of include/moldyn_hip.h
 EXPORTS = (
@@ -32,7 +34,10 @@ EXPORTS = (
     "md_update_force", "md_apply_boundary_conditions", "md_step", "md_step_replicas",
     "md_thermostat_calculate_lambda", "md_thermostat_update", "md_barostat_calculate_myu",
     "md_barostat_update", "md_macro", "md_init_lattice", "md_init_mb_velocities", "md_get_stats",
-    "md_set_profiling", "md_kernel_time",
+    "md_set_profiling", "md_kernel_time", "md_mlp_set_params", "md_mlp_get_stats",
+    "md_trainer_create", "md_trainer_destroy", "md_trainer_last_error", "md_trainer_feature_stats",
+    "md_trainer_set_params", "md_trainer_get_params", "md_trainer_loss", "md_trainer_epoch",
+    "md_trainer_predict",
 )
 
 
@@ -115,6 +120,20 @@ def load():
     L.md_init_mb_velocities.argtypes = [dp, C.c_int64, C.c_double, C.c_double, C.c_uint64]
     L.md_get_stats.argtypes = [vp, C.POINTER(MdStats)]
     L.md_set_profiling.argtypes = [vp, C.c_int]
-    L.md_kernel_time.argtypes = [vp, C.c_int, dp, i64p]
+    L.md_kernel_time.argtypes = [vp, C.c_int, dp, i64p, dp, i64p]
+    fp = C.POINTER(C.c_float)
+    L.md_mlp_set_params.argtypes = [vp, fp, fp, fp, fp, fp, C.c_double, C.c_double, C.c_int]
+    L.md_mlp_get_stats.argtypes = [vp, C.c_int, dp]
+    L.md_trainer_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, dp, dp, dp, dp, C.c_double]
+    L.md_trainer_destroy.argtypes = [vp]
+    L.md_trainer_destroy.restype = None
+    L.md_trainer_last_error.argtypes = [vp]
+    L.md_trainer_last_error.restype = C.c_char_p
+    L.md_trainer_feature_stats.argtypes = [vp, dp]
+    L.md_trainer_set_params.argtypes = [vp, fp, fp, fp, C.c_double, C.c_double]
+    L.md_trainer_get_params.argtypes = [vp, fp]
+    L.md_trainer_loss.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp, fp]
+    L.md_trainer_epoch.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int32), dp]
+    L.md_trainer_predict.argtypes = [vp, C.c_int, dp, dp]
     _lib = L
     return L

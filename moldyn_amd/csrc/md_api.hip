@@ -66,6 +66,10 @@ struct md_ctx {
     int par = 0;       // control buffer the next launch reads
     int snap_par = 0;  // ... at the snapshot
     int n_cu = 256;
+    // surrogate
+    bool mlp = false, mlp_ready = false;
+    std::vector<float> mlp_theta = std::vector<float>(kMlpNP, 0.0f);
+    MlpConst mlp_const{};
     int tpb = 1;
     // graphs, by the parity they start at
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
@@ -74,9 +78,12 @@ struct md_ctx {
     bool prof = false;
     unsigned long long* prof_ring = nullptr;
     unsigned long long* prof_blk = nullptr;
+    unsigned long long* prof_host = nullptr;  // pinned staging of the ring
     double wall_hz = 1e8;
-    double ktime[P_NUM] = {0, 0, 0};
+    double ktime[P_NUM] = {0, 0, 0};      // stream time (previous kernel end to end)
     int64_t kcount[P_NUM] = {0, 0, 0};
+    double kspan_time[P_NUM] = {0, 0, 0};  // first block start to last block end
+    int64_t kspan_count[P_NUM] = {0, 0, 0};
     int64_t list_visits = 0;
     md_stats stats{};
     int64_t last_rebuilds = 0, last_refresh = 0;
@@ -209,6 +216,8 @@ static void free_all(md_ctx* c) {
     c->prof_ring = nullptr;
     if (c->prof_blk) (void)hipFree(c->prof_blk);
     c->prof_blk = nullptr;
+    if (c->prof_host) (void)hipHostFree(c->prof_host);
+    c->prof_host = nullptr;
     c->d.prof = nullptr;
     c->d.prof_blk = nullptr;
     c->prof = false;  // a new layout starts unprofiled
@@ -345,16 +354,57 @@ static int upload_pairs(md_ctx* c) {
     return MD_OK;
 }
 
+static bool fp32_records(int mode) { return mode == MD_FORCE_FP32_LIST || mode == MD_FORCE_MLP; }
+
 static size_t rec_size(const md_ctx* c) {
-    return c->opt.force_mode == MD_FORCE_FP32_LIST ? sizeof(Rec<float>) : sizeof(Rec<double>);
+    return fp32_records(c->opt.force_mode) ? sizeof(Rec<float>) : sizeof(Rec<double>);
+}
+
+// Surrogate buffers (features, dE/dG, energies, statistics, check sums,
+// parameters) and the upload of the host copy of the parameters.
+static int mlp_upload(md_ctx* c) {
+    if (!c->mlp || !c->mlp_ready) return MD_OK;
+    HIPCHK(c, hipMemcpyAsync((void*)c->d.mlp_theta, c->mlp_theta.data(), sizeof(float) * kMlpNP,
+                             hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync((void*)c->d.mlp_c, &c->mlp_const, sizeof(MlpConst), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, launch_mlp_convert(c->d, c->st));
+    return MD_OK;
+}
+
+static int alloc_mlp(md_ctx* c) {
+    if (!c->mlp) return MD_OK;
+    DevState& d = c->d;
+    const size_t N = (size_t)std::max<int64_t>(c->n_slots, 1);
+    void* p;
+    int rc;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(__bf16) * N * kMlpK))) return rc;
+    d.mlp_x = (__bf16*)p;
+    HIPCHK(c, hipMemset(p, 0, sizeof(__bf16) * N * kMlpK));
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(float) * N * kMlpK))) return rc;
+    d.mlp_gam = (float*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(float) * N))) return rc;
+    d.mlp_e = (float*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(MlpStat) * c->R))) return rc;
+    d.mlp_stat = (MlpStat*)p;
+    HIPCHK(c, hipMemset(p, 0, sizeof(MlpStat) * c->R));
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * 2 * (size_t)std::max(c->n_blocks, 1)))) return rc;
+    d.mlp_judge = (double*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(MlpConst)))) return rc;
+    d.mlp_c = (const MlpConst*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(float) * kMlpNP))) return rc;
+    d.mlp_theta = (const float*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(__bf16) * kMlpBfN))) return rc;
+    d.mlp_w = (const __bf16*)p;
+    return mlp_upload(c);
 }
 
 // Allocate everything for the current layout; state buffers are NaN / zero.
 static int alloc_base(md_ctx* c) {
     free_all(c);
     make_layout(c);
-    c->fp32 = c->opt.force_mode == MD_FORCE_FP32_LIST;
+    c->fp32 = fp32_records(c->opt.force_mode);
     c->list_mode = c->opt.force_mode != MD_FORCE_FP64_ALLPAIRS;
+    c->mlp = c->opt.force_mode == MD_FORCE_MLP;
     DevState& d = c->d;
     d = DevState{};
     const int64_t N = c->n_slots;
@@ -415,6 +465,7 @@ static int alloc_base(md_ctx* c) {
     if ((rc = upload_pairs(c))) return rc;
     auto_capacities(c);
     if ((rc = alloc_lists(c))) return rc;
+    if ((rc = alloc_mlp(c))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->st));
     return MD_OK;
 }
@@ -455,6 +506,7 @@ static RepParams base_params(md_ctx* c) {
     RepParams P{};
     set_kick(c, P, c->last_dt);
     P.list_mode = c->list_mode ? 1 : 0;
+    P.in_place = c->mlp ? 1 : 0;
     P.rlist = c->rlist;
     P.rc_max = c->rc_max;
     P.margin = 1e-9 * c->rlist;
@@ -494,16 +546,48 @@ static int read_sums(md_ctx* c, std::vector<double>& S) {
 // Fold the device timestamp ring into the per-kernel totals and reset it.
 static int prof_collect(md_ctx* c) {
     if (!c->prof || !c->prof_ring) return MD_OK;
-    std::vector<unsigned long long> h((size_t)kProfCount + 1);
-    HIPCHK(c, hipMemcpyAsync(h.data(), c->prof_ring, sizeof(unsigned long long) * h.size(),
+    const size_t n = (size_t)kProfCount + 1;
+    if (!c->prof_host) {
+        void* p = nullptr;
+        HIPCHK(c, hipHostMalloc(&p, sizeof(unsigned long long) * n, hipHostMallocDefault));
+        c->prof_host = (unsigned long long*)p;
+    }
+    const unsigned long long* h = c->prof_host;  // pinned: the copy is a DMA transfer
+    HIPCHK(c, hipMemcpyAsync(c->prof_host, c->prof_ring, sizeof(unsigned long long) * n,
                              hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    // span: first block start to last block end of each kernel launch
     for (size_t e = 0; e < (size_t)kProfRing * P_NUM; ++e) {
         const unsigned long long a = h[2 * e], b = h[2 * e + 1];
         if (a == ~0ull || b < a) continue;
         const int k = (int)(e % P_NUM);
-        c->ktime[k] += (double)(b - a) / c->wall_hz * 1e3;
-        c->kcount[k] += 1;
+        c->kspan_time[k] += (double)(b - a) / c->wall_hz * 1e3;
+        c->kspan_count[k] += 1;
+    }
+    // stream time: from the end of the previous recorded kernel on the stream
+    // (in launch order drift, step, finalize) to this kernel's end, counted
+    // where the two ran back to back (the interval a dispatch trace reports)
+    const unsigned long long gap = (unsigned long long)(20e-6 * c->wall_hz);
+    auto rec = [&](int l, int k, unsigned long long* a, unsigned long long* b) {
+        const size_t e = ((size_t)l * P_NUM + k) * 2;
+        *a = h[e];
+        *b = h[e + 1];
+        return *a != ~0ull && *b >= *a;
+    };
+    for (int l = 0; l < kProfRing; ++l) {
+        unsigned long long pa, pb, a, b;
+        const int lp = (l + kProfRing - 1) % kProfRing;
+        bool have_prev = rec(lp, P_FIN, &pa, &pb);
+        unsigned long long prev_end = pb;
+        for (int k : {(int)P_DRIFT, (int)P_STEP, (int)P_FIN}) {
+            if (!rec(l, k, &a, &b)) continue;
+            if (have_prev && a >= prev_end && a - prev_end < gap) {
+                c->ktime[k] += (double)(b - prev_end) / c->wall_hz * 1e3;
+                c->kcount[k] += 1;
+            }
+            have_prev = true;
+            prev_end = b;
+        }
     }
     c->list_visits += (int64_t)h[(size_t)kProfCount];
     HIPCHK(c, launch_prof_init(c->d, c->st));
@@ -515,7 +599,11 @@ static int prof_collect(md_ctx* c) {
 static hipError_t launch_once(md_ctx* c) {
     hipError_t e;
     c->d.par = c->par;
-    if (c->list_mode) {
+    if (c->mlp) {
+        if ((e = launch_mlp_desc(c->d, c->bs, false, c->st))) return e;
+        if ((e = launch_mlp_net(c->d, c->st))) return e;
+        if ((e = launch_mlp_force(c->d, c->bs, false, c->st))) return e;
+    } else if (c->list_mode) {
         if ((e = launch_step(c->d, c->fp32, c->bs, c->st))) return e;
     } else {
         if ((e = launch_drift(c->d, c->bs, c->st))) return e;
@@ -552,7 +640,10 @@ static int ensure_graph(md_ctx* c, int len) {
 // ------------------------------------------------------------------ snapshot
 
 static size_t state_bytes(md_ctx* c) {
-    return (rec_size(c) + 2 * sizeof(double)) * (size_t)c->n_slots;  // record, pot, vir
+    // records, pot, vir, the group sums the first step's controls derive from,
+    // and the surrogate's counters
+    return (rec_size(c) + 2 * sizeof(double)) * (size_t)c->n_slots + sizeof(double) * NS * c->groups.size() +
+           (c->d.mlp_stat ? sizeof(MlpStat) * c->R : 0);
 }
 
 static int snapshot(md_ctx* c, bool save, int buf) {
@@ -563,10 +654,12 @@ static int snapshot(md_ctx* c, bool save, int buf) {
         HIPCHK(c, hipMalloc((void**)&c->snap_ctrl, sizeof(RepCtrl) * c->R));
     }
     char* s = (char*)c->snap;
-    void* arrs[3] = {c->d.rec[buf], c->d.pot, c->d.vir};
-    const size_t sz[3] = {rec_size(c) * (size_t)c->n_slots, n, n};
+    void* arrs[5] = {c->d.rec[buf], c->d.pot, c->d.vir, c->d.sums, c->d.mlp_stat};
+    const size_t sz[5] = {rec_size(c) * (size_t)c->n_slots, n, n, sizeof(double) * NS * c->groups.size(),
+                          c->d.mlp_stat ? sizeof(MlpStat) * c->R : 0};
     size_t off = 0;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 5; ++k) {
+        if (sz[k] == 0) continue;
         if (save)
             HIPCHK(c, hipMemcpyAsync(s + off, arrs[k], sz[k], hipMemcpyDeviceToDevice, c->st));
         else
@@ -634,7 +727,13 @@ static int list_force_eval(md_ctx* c) {
         if (c->n_blocks > 0) {
             HIPCHK(c, launch_insert_all(c->d, c->fp32, c->bs, c->st));
             HIPCHK(c, launch_build(c->d, c->fp32, c->st));
-            HIPCHK(c, launch_force_eval(c->d, c->fp32, c->bs, c->st));
+            if (c->mlp) {
+                HIPCHK(c, launch_mlp_desc(c->d, c->bs, true, c->st));
+                HIPCHK(c, launch_mlp_net(c->d, c->st));
+                HIPCHK(c, launch_mlp_force(c->d, c->bs, true, c->st));
+            } else {
+                HIPCHK(c, launch_force_eval(c->d, c->fp32, c->bs, c->st));
+            }
         }
         HIPCHK(c, finalize(c, M_REDUCE));
         RepStats s;
@@ -663,6 +762,7 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
                      bool per_rep) {
     if (n < 0) return fail(c, MD_ERR_ARG, "n_steps < 0");
     if (n == 0) return MD_OK;
+    if (c->mlp && !c->mlp_ready) return fail(c, MD_ERR_ARG, "surrogate parameters not set (md_mlp_set_params)");
     for (int r = 0; r < c->R; ++r) {
         const md_thermostat* t = th ? &th[per_rep ? r : 0] : nullptr;
         const md_barostat* b = ba ? &ba[per_rep ? r : 0] : nullptr;
@@ -1146,7 +1246,9 @@ static int relayout(md_ctx* c, const md_options& nopt) {
 int md_set_options(md_ctx* c, const md_options* o) {
     if (!c || !o) return fail(c, MD_ERR_ARG, "NULL argument");
     md_options n = *o;
-    if (n.force_mode < 0 || n.force_mode > 2) return fail(c, MD_ERR_ARG, "bad force_mode");
+    if (n.force_mode < 0 || n.force_mode > 3) return fail(c, MD_ERR_ARG, "bad force_mode");
+    if (n.force_mode == MD_FORCE_MLP && c->T != 1)
+        return fail(c, MD_ERR_UNIMPLEMENTED, "the surrogate is defined for one particle type");
     if (n.block_size == 0) n.block_size = 256;
     if (n.block_size != 64 && n.block_size != 128 && n.block_size != 256 && n.block_size != 512)
         return fail(c, MD_ERR_ARG, "block_size must be 64, 128, 256 or 512");
@@ -1213,6 +1315,7 @@ int md_set_pair_potential_lj(md_ctx* c, uint16_t t0, uint16_t t1, double sigma, 
 
 int md_update_force(md_ctx* c) {
     if (!c) return MD_ERR_ARG;
+    if (c->mlp && !c->mlp_ready) return fail(c, MD_ERR_ARG, "surrogate parameters not set (md_mlp_set_params)");
     if (c->list_mode) return list_force_eval(c);
     std::vector<RepParams> P((size_t)c->R, base_params(c));
     int rc = write_params(c, P);
@@ -1452,11 +1555,61 @@ int md_get_stats(md_ctx* c, md_stats* out) {
     return MD_OK;
 }
 
+int md_mlp_set_params(md_ctx* c, const float* theta, const float* g_mean, const float* g_inv_std,
+                      const float* g_lo, const float* g_hi, double e_scale, double e_shift,
+                      int judge_every) {
+    if (!c || !theta || !g_mean || !g_inv_std) return fail(c, MD_ERR_ARG, "NULL argument");
+    if (judge_every < 0) return fail(c, MD_ERR_ARG, "judge_every < 0");
+    c->mlp_theta.assign(theta, theta + kMlpNP);
+    MlpConst& k = c->mlp_const;
+    k = MlpConst{};
+    // radial basis of the Lennard-Jones sigma of type pair (0, 0)
+    const double sigma = c->pairs[0].sigma;
+    const double w = 0.1 * sigma;
+    for (int q = 0; q < kMlpK; ++q) {
+        k.g_mean[q] = g_mean[q];
+        k.g_inv_std[q] = g_inv_std[q];
+        k.g_lo[q] = g_lo ? g_lo[q] : -INFINITY;
+        k.g_hi[q] = g_hi ? g_hi[q] : INFINITY;
+        k.mu[q] = (float)(sigma * (0.8 + 1.7 * q / (kMlpK - 1)));
+    }
+    k.inv_2w2 = (float)(1.0 / (2.0 * w * w));
+    k.inv_w2 = (float)(1.0 / (w * w));
+    k.rc = (float)(2.5 * sigma);
+    k.e_scale = (float)e_scale;
+    k.e_shift = (float)e_shift;
+    k.b3 = theta[kMlpOffB3];
+    k.judge_every = judge_every;
+    k.check_range = (g_lo && g_hi) ? 1 : 0;
+    if ((double)k.rc > c->rc_max)
+        return fail(c, MD_ERR_ARG, "surrogate cutoff exceeds the neighbour-list cutoff");
+    c->mlp_ready = true;
+    c->sums_valid = false;
+    int rc = mlp_upload(c);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return MD_OK;
+}
+
+int md_mlp_get_stats(md_ctx* c, int replica, double out[5]) {
+    if (!c || !out || replica < 0 || replica >= c->R) return fail(c, MD_ERR_ARG, "bad argument");
+    if (!c->mlp || !c->d.mlp_stat) return fail(c, MD_ERR_ARG, "force mode is not MD_FORCE_MLP");
+    MlpStat m;
+    HIPCHK(c, hipMemcpy(&m, c->d.mlp_stat + replica, sizeof(MlpStat), hipMemcpyDeviceToHost));
+    const double n = (double)std::max<int64_t>(c->reps[replica].n_atoms, 1);
+    out[0] = m.last_ref2 > 0.0 ? sqrt(m.last_err2 / m.last_ref2) : 0.0;
+    out[1] = m.max_rel;
+    out[2] = (double)m.checks;
+    out[3] = (double)m.fallbacks;
+    out[4] = sqrt(m.last_ref2 / (3.0 * n));
+    return MD_OK;
+}
+
 int md_set_profiling(md_ctx* c, int on) {
     if (!c) return MD_ERR_ARG;
     for (int k = 0; k < P_NUM; ++k) {
-        c->ktime[k] = 0.0;
-        c->kcount[k] = 0;
+        c->ktime[k] = c->kspan_time[k] = 0.0;
+        c->kcount[k] = c->kspan_count[k] = 0;
     }
     c->list_visits = 0;
     if (on && !c->prof_ring) {
@@ -1477,7 +1630,8 @@ int md_set_profiling(md_ctx* c, int on) {
     return MD_OK;
 }
 
-int md_kernel_time(md_ctx* c, int kernel, double* total_ms, int64_t* launches) {
+int md_kernel_time(md_ctx* c, int kernel, double* total_ms, int64_t* launches, double* span_ms,
+                   int64_t* span_launches) {
     if (!c || kernel < 0 || kernel >= K_NUM) return MD_ERR_ARG;
     // the step slot holds k_step on the list paths and k_allpairs on the plain path
     int slot = -1;
@@ -1487,6 +1641,8 @@ int md_kernel_time(md_ctx* c, int kernel, double* total_ms, int64_t* launches) {
     if (kernel == K_DRIFT && !c->list_mode) slot = P_DRIFT;
     if (total_ms) *total_ms = slot >= 0 ? c->ktime[slot] : 0.0;
     if (launches) *launches = slot >= 0 ? c->kcount[slot] : 0;
+    if (span_ms) *span_ms = slot >= 0 ? c->kspan_time[slot] : 0.0;
+    if (span_launches) *span_launches = slot >= 0 ? c->kspan_count[slot] : 0;
     return MD_OK;
 }
 

@@ -115,7 +115,7 @@ struct RepParams {
     double rlist, rc_max, margin, insert_growth;
     double psi_in;        // Nose-Hoover psi handed in by the caller
     int32_t use_psi_in;   // M_PRIME starts from psi_in instead of psi_used
-    int32_t pad0;
+    int32_t in_place;     // a step ends in the buffer it started from (surrogate path)
     double cm[kMaxTypes];     // dt / (2 m_t), the half-kick factor (integrator.rs:29)
     double inv_m2[kMaxTypes]; // 1 / m_t^2 (drift-bound bookkeeping only)
 };
@@ -145,6 +145,46 @@ struct RepStats {
     unsigned int brick_fallback, pad;
 };
 
+// Learned per-atom energy surrogate: 32 radial features, 128-128 SiLU MLP.
+constexpr int kMlpK = 32;
+constexpr int kMlpH = 128;
+// flat parameter vector: W1 [H][K] | b1 [H] | W2 [H][H] | b2 [H] | w3 [H] | b3
+constexpr int kMlpOffW1 = 0;
+constexpr int kMlpOffB1 = kMlpOffW1 + kMlpH * kMlpK;
+constexpr int kMlpOffW2 = kMlpOffB1 + kMlpH;
+constexpr int kMlpOffB2 = kMlpOffW2 + kMlpH * kMlpH;
+constexpr int kMlpOffW3 = kMlpOffB2 + kMlpH;
+constexpr int kMlpOffB3 = kMlpOffW3 + kMlpH;
+constexpr int kMlpNP = kMlpOffB3 + 1;
+// BF16 inference copies: W1 [H][K] | W1t [K][H] | W2 [H][H] | W2t [H][H]
+constexpr int kMlpBfW1 = 0;
+constexpr int kMlpBfW1t = kMlpBfW1 + kMlpH * kMlpK;
+constexpr int kMlpBfW2 = kMlpBfW1t + kMlpH * kMlpK;
+constexpr int kMlpBfW2t = kMlpBfW2 + kMlpH * kMlpH;
+constexpr int kMlpBfN = kMlpBfW2t + kMlpH * kMlpH;
+
+struct MlpConst {
+    float g_mean[kMlpK], g_inv_std[kMlpK];  // input normalisation
+    float g_lo[kMlpK], g_hi[kMlpK];         // feature range seen in training
+    float mu[kMlpK];                        // basis centres (nm)
+    float inv_2w2;                          // 1 / (2 w^2)
+    float inv_w2;                           // 1 / w^2
+    float rc;                               // basis cutoff (nm)
+    float e_scale, e_shift;
+    float b3;
+    int32_t judge_every;                    // LJ force check interval (steps), 0 = never
+    int32_t check_range;                    // fall back to LJ outside [g_lo, g_hi]
+};
+
+// per-replica surrogate statistics (device)
+struct MlpStat {
+    double last_err2, last_ref2;  // sum |F - F_LJ|^2 and sum |F_LJ|^2 at the last check
+    double max_rel;               // largest RMSE / RMS over the checks
+    int64_t checks, fallbacks;
+    uint32_t oor;                 // a feature left the training range in this launch
+    uint32_t pad;
+};
+
 struct DevState {
     void* rec[2];            // Rec<float> or Rec<double>, n_slots each
     double* pot;
@@ -167,6 +207,15 @@ struct DevState {
     double* sums;            // n_groups x NS
     unsigned long long* prof;  // per-launch kernel timestamps (null: off)
     unsigned long long* prof_blk;  // per-block {start, end, list entries} of the per-slot kernels
+    // surrogate path (null unless force mode MD_FORCE_MLP)
+    const MlpConst* mlp_c;
+    const float* mlp_theta;       // kMlpNP
+    const __bf16* mlp_w;          // kMlpBfN
+    __bf16* mlp_x;                // n_slots x kMlpK normalised features
+    float* mlp_gam;               // n_slots x kMlpK  dE_i / dG_i
+    float* mlp_e;                 // n_slots          E_i
+    MlpStat* mlp_stat;            // n_reps
+    double* mlp_judge;            // 2 x n_blocks per-block check sums
     int32_t n_types;
     int32_t n_blocks;
     int32_t n_reps;
@@ -302,5 +351,12 @@ hipError_t launch_reduce_state(const DevState& d, bool fp32, int bs, hipStream_t
 hipError_t launch_scale(const DevState& d, bool fp32, int64_t start, int64_t count, double fac,
                         int which /*0 velocity, 1 position*/, hipStream_t st);
 hipError_t launch_wrap(const DevState& d, bool fp32, int bs, hipStream_t st);
+
+// surrogate step kernels (md_mlp.hip); force_only evaluates at the current
+// positions without advancing time
+hipError_t launch_mlp_desc(const DevState& d, int bs, bool force_only, hipStream_t st);
+hipError_t launch_mlp_net(const DevState& d, hipStream_t st);
+hipError_t launch_mlp_force(const DevState& d, int bs, bool force_only, hipStream_t st);
+hipError_t launch_mlp_convert(const DevState& d, hipStream_t st);
 
 }  // namespace md

@@ -24,199 +24,9 @@
 // the reference (Rust) never forms.  The FP32 pair kernel uses explicit fmaf.
 #include <math.h>
 
-#include "md_device.h"
+#include "md_common.h"
 
 namespace md {
-
-__device__ __forceinline__ double min_image(double r, double b) {
-    // potential.rs:186-200: one correction against b / 2
-    if (r < -b / 2.0) {
-        r += b;
-    } else if (r > b / 2.0) {
-        r -= b;
-    }
-    return r;
-}
-
-__device__ __forceinline__ double wrap1(double x, double b) {
-    // core/src/particle.rs:124-138
-    if (x < 0.0) {
-        x += b;
-    } else if (x >= b) {
-        x -= b;
-    }
-    return x;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// Block reduction of NS columns in a fixed order (wave butterfly, then waves
-// in index order); the columns set in mask are stored to out[k * stride].
-template <int BS>
-__device__ __forceinline__ void block_store(const double (&v)[NS], double* out, int64_t stride,
-                                            uint32_t mask) {
-    constexpr int NW = BS / 64;
-    __shared__ double red[NW * NS];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-        if (!((mask >> k) & 1u)) continue;
-        const double t = k >= kFirstMax ? wave_max(v[k]) : wave_sum(v[k]);
-        if (lane == 0) red[w * NS + k] = t;
-    }
-    __syncthreads();
-    const int k = threadIdx.x;
-    if (k < NS && ((mask >> k) & 1u)) {
-        double acc = red[k];
-        for (int i = 1; i < NW; ++i)
-            acc = k >= kFirstMax ? fmax(acc, red[i * NS + k]) : acc + red[i * NS + k];
-        out[k * stride] = acc;
-    }
-    __syncthreads();
-}
-
-// In-place exclusive scan of a[0..n) (n <= kBrickCells); a[n] receives the total.
-template <int BS>
-__device__ void block_exclusive_scan(int* a, int n) {
-    constexpr int NW = BS / 64;
-    __shared__ int wsum[NW];
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int chunk = (n + BS - 1) / BS;
-    const int lo = t * chunk < n ? t * chunk : n;
-    const int hi = lo + chunk < n ? lo + chunk : n;
-    int local = 0;
-    for (int i = lo; i < hi; ++i) local += a[i];
-    int incl = local;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
-    if (lane == 63) wsum[w] = incl;
-    __syncthreads();
-    int before = 0;
-    for (int i = 0; i < w; ++i) before += wsum[i];
-    int run = before + incl - local;
-    for (int i = lo; i < hi; ++i) {
-        const int c = a[i];
-        a[i] = run;
-        run += c;
-    }
-    if (t == BS - 1) a[n] = before + incl;
-    __syncthreads();
-}
-
-// ---------------------------------------------------------------- pair terms
-
-// potential.rs:201-211 with LJ of potential.rs:57-69, FP64, reference order.
-__device__ __forceinline__ void pair_exact(double dx, double dy, double dz, double r2,
-                                           const PairParam& p, double& fx, double& fy,
-                                           double& fz, double& pot, double& vir) {
-    const double r = sqrt(r2);
-    const double sr = p.sigma / r;
-    const double s2 = sr * sr;
-    const double s4 = s2 * s2;
-    const double s6 = s2 * s4;
-    const double s12 = s6 * s6;
-    const double u = 4.0 * p.eps * (s12 - s6) - p.u_cut;
-    const double du = (24.0 * p.eps / r) * (s6 - 2.0 * s12);
-    const double gx = (dx / r) * du;
-    const double gy = (dy / r) * du;
-    const double gz = (dz / r) * du;
-    const double t = (gx * dx + gy * dy) + gz * dz;
-    fx += gx;
-    fy += gy;
-    fz += gz;
-    pot += u;
-    vir += t;
-}
-
-// Same physics in FP32 from the FP64 separation: F/r = 24 eps (s6 - 2 s12) / r^2,
-// virial F.r = (F/r) r^2, U = 4 eps (s12 - s6) - u_cut.
-__device__ __forceinline__ void pair_fp32(double dx, double dy, double dz, const PairParam& p,
-                                          float& fx, float& fy, float& fz, float& pot,
-                                          float& vir) {
-    const float ax = (float)dx, ay = (float)dy, az = (float)dz;
-    const float r2 = fmaf(ax, ax, fmaf(ay, ay, az * az));
-    const float ir2 = 1.0f / r2;
-    const float s2 = p.sig2f * ir2;
-    const float s6 = s2 * s2 * s2;
-    const float s12 = s6 * s6;
-    const float fr = p.eps24f * ir2 * fmaf(-2.0f, s12, s6);
-    fx = fmaf(ax, fr, fx);
-    fy = fmaf(ay, fr, fy);
-    fz = fmaf(az, fr, fz);
-    pot += fmaf(p.eps4f, s12 - s6, -p.ucutf);
-    vir = fmaf(fr, r2, vir);
-}
-
-template <typename FT>
-struct Acc;
-template <>
-struct Acc<double> {
-    double fx = 0.0, fy = 0.0, fz = 0.0, pot = 0.0, vir = 0.0;
-    __device__ __forceinline__ void add(double dx, double dy, double dz, double r2,
-                                        const PairParam& p) {
-        pair_exact(dx, dy, dz, r2, p, fx, fy, fz, pot, vir);
-    }
-};
-template <>
-struct Acc<float> {
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f, pot = 0.0f, vir = 0.0f;
-    __device__ __forceinline__ void add(double dx, double dy, double dz, double,
-                                        const PairParam& p) {
-        pair_fp32(dx, dy, dz, p, fx, fy, fz, pot, vir);
-    }
-};
-
-// ---------------------------------------------------------------- cell grid
-
-__device__ __forceinline__ int cell_coord(double x, double b, int m) {
-    double u = x / b;
-    u = u - floor(u);
-    int q = (int)(u * (double)m);
-    return q < 0 ? 0 : (q >= m ? m - 1 : q);
-}
-
-__device__ __forceinline__ void grid_insert(const DevState& d, const RepCtrl& C,
-                                            const RepInfo& R, int64_t slot, double x, double y,
-                                            double z) {
-    const uint32_t ep = C.ins_epoch;
-    const int buf = ep & 1;
-    const int* gd = C.gdim[buf];
-    const double* gb = C.gbox[buf];
-    const int64_t cell = R.cell_base + ((int64_t)cell_coord(x, gb[0], gd[0]) * gd[1] +
-                                        cell_coord(y, gb[1], gd[1])) * gd[2] +
-                         cell_coord(z, gb[2], gd[2]);
-    unsigned long long* h = d.cell_hdr[buf] + cell;
-    unsigned long long old = __hip_atomic_load(h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long assumed;
-    uint32_t k;
-    do {
-        assumed = old;
-        k = ((uint32_t)(assumed >> 32) == ep) ? (uint32_t)assumed : 0u;
-        const unsigned long long nw = ((unsigned long long)ep << 32) | (unsigned long long)(k + 1u);
-        old = atomicCAS(h, assumed, nw);
-    } while (old != assumed);
-    if ((int)k < d.cell_cap) {
-        d.cell_slots[buf][cell * d.cell_cap + k] = (int32_t)slot;
-        // within a quarter of the capacity: report, the host grows it between chunks
-        if (4 * ((int)k + 1) > 3 * d.cell_cap) atomicMax(&d.stats->max_cell, (int)k + 1);
-    } else {
-        atomicOr(&d.stats->cell_overflow, 1u);
-        atomicMax(&d.stats->max_cell, (int)k + 1);
-    }
-}
 
 // Sorted insertion of j into the list column of slot i (column-major).
 __device__ __forceinline__ void list_push(int32_t* __restrict__ col, int64_t N, int n, int cap,
@@ -487,33 +297,6 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
 }
 
 // ---------------------------------------------------------------- fused step
-
-// Per-launch kernel timestamps: first block start to last block end, in
-// wall-clock ticks, accumulated per launch with atomics (only when profiling).
-__device__ __forceinline__ uint64_t prof_now(const DevState& d) {
-    return d.prof ? (uint64_t)wall_clock64() : 0ull;
-}
-
-__device__ __forceinline__ void prof_mark(const DevState& d, int64_t launch, int kernel, uint64_t t0) {
-    if (d.prof && threadIdx.x == 0) {
-        const uint64_t t1 = (uint64_t)wall_clock64();
-        unsigned long long* e = d.prof + ((launch % kProfRing) * P_NUM + kernel) * 2;
-        atomicMin(e, (unsigned long long)t0);
-        atomicMax(e + 1, (unsigned long long)t1);
-    }
-}
-
-// Per-block record of a per-slot kernel (thread 0, after the block's last
-// barrier); blocks that did no work store an empty interval.
-__device__ __forceinline__ void prof_block(const DevState& d, int slot, uint64_t t0, bool worked,
-                                           uint64_t entries) {
-    if (d.prof && threadIdx.x == 0) {
-        unsigned long long* e = d.prof_blk + ((int64_t)slot * d.n_blocks + blockIdx.x) * 3;
-        e[0] = worked ? (unsigned long long)t0 : ~0ull;
-        e[1] = worked ? (unsigned long long)wall_clock64() : 0ull;
-        e[2] = entries;
-    }
-}
 
 // One launch of the step kernel over the block's tiles.  The record and list
 // length of the next tile are loaded before the current tile is processed.
@@ -1085,8 +868,9 @@ __device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const 
                 C.fill_epoch = C.ins_epoch;
                 nf |= F_GRID_FRESH;
             }
-            // k_step wrote buffer cur ^ 1 (a split step ends in cur)
-            if (P.list_mode && !(f & F_SPLIT_B)) C.cur ^= 1;
+            // k_step wrote buffer cur ^ 1 (a split step, and every surrogate
+            // step, ends in cur)
+            if (P.list_mode && !P.in_place && !(f & F_SPLIT_B)) C.cur ^= 1;
             C.flags = nf;
         }
     } else {
@@ -1262,6 +1046,33 @@ __device__ int64_t finalize_replica(const DevState& d, int r, int mode) {
         for (int i = threadIdx.x; i < T * NS; i += BS) s_sums[i] = d.sums[g0 * NS + i];
     }
     __syncthreads();
+    if (d.mlp_stat) {
+        // surrogate: the LJ force check of this step, and fallbacks
+        MlpStat* ms = d.mlp_stat + r;
+        const bool stepped = mode == M_STEP && finalize_reduces(mode, f0);
+        const int je = d.mlp_c->judge_every;
+        if (stepped && je > 0 && (s_ctrl.step + 1) % je == 0) {
+            __shared__ double s_j[NS];
+            double jv[NS];
+#pragma unroll
+            for (int k = 0; k < NS; ++k) jv[k] = 0.0;
+            for (int q = threadIdx.x; q < R.nblocks; q += BS) {
+                jv[0] += d.mlp_judge[R.first_block + q];
+                jv[1] += d.mlp_judge[d.n_blocks + R.first_block + q];
+            }
+            block_store<BS>(jv, s_j, 1, 3u);
+            if (threadIdx.x == 0) {
+                ms->last_err2 = s_j[0];
+                ms->last_ref2 = s_j[1];
+                if (s_j[1] > 0.0) ms->max_rel = fmax(ms->max_rel, sqrt(s_j[0] / s_j[1]));
+                ms->checks++;
+            }
+        }
+        if (threadIdx.x == 0 && ms->oor) {
+            if (stepped) ms->fallbacks++;
+            ms->oor = 0u;
+        }
+    }
     if (mode == M_REDUCE) return -1;
     if (threadIdx.x == 0) {
         RepCtrl C = s_ctrl;

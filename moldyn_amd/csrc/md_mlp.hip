@@ -1,0 +1,503 @@
+// md_mlp.hip -- learned per-atom energy surrogate on gfx950 (inference).
+//
+// E_i = e_scale * MLP(x_i) + e_shift with x_i = (G_i - mean) * inv_std, where
+// G_ik = sum_j exp(-(r_ij - mu_k)^2 / (2 w^2)) f_c(r_ij) (32 radial features,
+// cosine cutoff f_c) and MLP = 32-128-128-1 with SiLU.  Forces are the exact
+// negative gradient of sum_i E_i:
+//   F_i = sum_j (c_i(r_ij) + c_j(r_ij)) d_ij / r_ij,  c_i(r) = sum_k dE_i/dG_ik dg_k/dr.
+// A surrogate step launch is four kernels:
+//   k_mlp_desc   half kick, thermostat scale, drift, wrap (integrator.rs:28-45)
+//                into buffer cur ^ 1 and the features of the drifted positions
+//                (neighbour drift recomputed from the input records);
+//   k_mlp_net    per 16-atom row block of a wave: the forward pass and the
+//                backward pass to dE/dG on BF16 MFMA (FP32 accumulation),
+//                weights staged in LDS;
+//   k_mlp_force  pair forces from both atoms' dE/dG, the LJ fallback when a
+//                feature left the training range, the periodic LJ check,
+//                second half kick and barostat scale into buffer cur;
+//   k_fin        the usual finalize.
+#include "md_common.h"
+
+namespace md {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr float kPi = 3.14159265358979323846f;
+
+__device__ __forceinline__ float silu_f(float a, float& ds) {
+    const float s = 1.0f / (1.0f + __expf(-a));
+    ds = s * (1.0f + a * (1.0f - s));
+    return a * s;
+}
+
+// Features of one particle from its neighbour list.  Positions come from
+// `pos` records; with DRIFT the neighbours' drifted positions are recomputed
+// from their input records with the operations their own threads perform.
+template <bool DRIFT>
+__device__ __forceinline__ void mlp_features(const DevState& d, const MlpConst& mc, const Rec<float>* __restrict__ pos,
+                                             int64_t s, double x1, double y1, double z1, double bx, double by,
+                                             double bz, double c, double lam, double dt, float (&g)[kMlpK]) {
+#pragma unroll
+    for (int k = 0; k < kMlpK; ++k) g[k] = 0.0f;
+    const int nn = d.ncount[s];
+    const double rc2 = (double)mc.rc * (double)mc.rc;
+    for (int q = 0; q < nn; ++q) {
+        const int32_t j = d.nbr[(int64_t)q * d.n_slots + s];
+        const Rec<float> o = pos[j];
+        double xj = o.x, yj = o.y, zj = o.z;
+        if (DRIFT) {
+            xj = wrap1(xj + ((o.vx + (double)o.fx * c) * lam) * dt, bx);
+            yj = wrap1(yj + ((o.vy + (double)o.fy * c) * lam) * dt, by);
+            zj = wrap1(zj + ((o.vz + (double)o.fz * c) * lam) * dt, bz);
+        }
+        const double dx = min_image(xj - x1, bx);
+        const double dy = min_image(yj - y1, by);
+        const double dz = min_image(zj - z1, bz);
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        if (!(r2 < rc2)) continue;
+        const float r = (float)sqrt(r2);
+        const float fc = 0.5f * (__cosf(kPi * r / mc.rc) + 1.0f);
+#pragma unroll
+        for (int k = 0; k < kMlpK; ++k) {
+            const float t = r - mc.mu[k];
+            g[k] = fmaf(__expf(-t * t * mc.inv_2w2), fc, g[k]);
+        }
+    }
+}
+
+__device__ __forceinline__ bool mlp_store_features(const DevState& d, const MlpConst& mc, int64_t s,
+                                                   const float (&g)[kMlpK], bool check) {
+    bool oor = false;
+    bf16x8 v[kMlpK / 8];
+#pragma unroll
+    for (int k = 0; k < kMlpK; ++k) {
+        if (check && mc.check_range && (g[k] < mc.g_lo[k] || g[k] > mc.g_hi[k])) oor = true;
+        v[k / 8][k % 8] = (__bf16)((g[k] - mc.g_mean[k]) * mc.g_inv_std[k]);
+    }
+    bf16x8* out = (bf16x8*)(d.mlp_x + s * kMlpK);
+#pragma unroll
+    for (int q = 0; q < kMlpK / 8; ++q) out[q] = v[q];
+    return oor;
+}
+
+// Features of every particle (FORCE_ONLY: at the current positions), and in
+// a stepping launch the first half of the velocity-Verlet step into cur ^ 1.
+template <int BS, bool FORCE_ONLY>
+__global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
+    const int b = blockIdx.x;
+    const GroupInfo G = d.groups[d.blk_group[b]];
+    const int r = G.rep;
+    const RepCtrl& C = ctrl_in(d, r);
+    const uint32_t flags = FORCE_ONLY ? 0u : C.flags;
+    if (flags & (F_EXIT | F_BUILD)) {
+        prof_block(d, P_DRIFT, 0, false, 0);
+        return;
+    }
+    const uint64_t t_start = prof_now(d);
+    const MlpConst& mc = *d.mlp_c;
+    const RepParams& P = d.params[r];
+    const RepInfo R = d.reps[r];
+    const TileRange tr = block_tiles(d, G, b, BS);
+    const int lane = threadIdx.x;
+    const int cur = C.cur, nx = cur ^ 1;
+    const Rec<float>* __restrict__ rin = (const Rec<float>*)d.rec[cur];
+    Rec<float>* __restrict__ rnx = (Rec<float>*)d.rec[nx];
+    if (flags & F_NULL) {
+        for (int64_t t = tr.t0; t < tr.t1; ++t) {
+            const int64_t s = G.start + t * BS + lane;
+            if (t * BS + lane < G.count) grid_insert(d, C, R, s, rin[s].x, rin[s].y, rin[s].z);
+        }
+        __syncthreads();
+        prof_block(d, P_DRIFT, t_start, true, 0);
+        return;
+    }
+    const bool split_a = (flags & F_SPLIT_A) != 0;
+    const bool split_b = (flags & F_SPLIT_B) != 0;
+    const bool drift = !FORCE_ONLY && !split_b;  // this launch moves the particles
+    const double bx = C.box[0], by = C.box[1], bz = C.box[2];
+    const double dt = P.dt, lam = C.lambda;
+    const double m = G.mass;
+    const double c = P.cm[G.type];
+    double part[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) part[k] = 0.0;
+    bool oor = false;
+    for (int64_t t = tr.t0; t < tr.t1; ++t) {
+        const int64_t s = G.start + t * BS + lane;
+        if (t * BS + lane >= G.count) {
+            if (!split_a) {
+                const float z[kMlpK] = {};
+                mlp_store_features(d, mc, s, z, false);  // padding slot
+            }
+            continue;
+        }
+        const Rec<float> a = (split_b ? rnx : rin)[s];
+        double x1 = a.x, y1 = a.y, z1 = a.z;
+        if (drift) {
+            // integrator.rs:28-45
+            const double vkx = a.vx + (double)a.fx * c;
+            const double vky = a.vy + (double)a.fy * c;
+            const double vkz = a.vz + (double)a.fz * c;
+            part[S_KVX] += vkx * m;
+            part[S_KVY] += vky * m;
+            part[S_KVZ] += vkz * m;
+            part[S_KV2] += m * ((vkx * vkx + vky * vky) + vkz * vkz);
+            const double v1x = vkx * lam, v1y = vky * lam, v1z = vkz * lam;
+            x1 = wrap1(a.x + v1x * dt, bx);
+            y1 = wrap1(a.y + v1y * dt, by);
+            z1 = wrap1(a.z + v1z * dt, bz);
+            part[S_VMAX] = fmax(part[S_VMAX], (v1x * v1x + v1y * v1y) + v1z * v1z);
+            Rec<float> o = a;
+            o.x = x1;
+            o.y = y1;
+            o.z = z1;
+            o.vx = v1x;
+            o.vy = v1y;
+            o.vz = v1z;
+            rnx[s] = o;
+            if (split_a) {
+                grid_insert(d, C, R, s, x1, y1, z1);
+                continue;
+            }
+        }
+        float g[kMlpK];
+        if (drift)
+            mlp_features<true>(d, mc, rin, s, x1, y1, z1, bx, by, bz, c, lam, dt, g);
+        else
+            mlp_features<false>(d, mc, split_b ? rnx : rin, s, x1, y1, z1, bx, by, bz, c, lam, dt, g);
+        oor |= mlp_store_features(d, mc, s, g, true);
+    }
+    if (oor) atomicOr(&d.mlp_stat[r].oor, 1u);
+    if (drift) block_store<BS>(part, d.partial + b, d.n_blocks, (P.th_kind == 2 ? kColsKick : 0u) | kColsDrift);
+    prof_block(d, P_DRIFT, t_start, true, 0);
+}
+
+// ---------------------------------------------------------------- network
+
+// LDS row strides (BF16 elements), padded so that the 16 rows an MFMA
+// fragment load touches fall on distinct banks.
+constexpr int kStrK = kMlpK + 8;  // rows of 32
+constexpr int kStrH = kMlpH + 8;  // rows of 128
+constexpr int kNetWaves = 8;
+constexpr int kNetBS = 64 * kNetWaves;
+
+// D (16 x 16, FP32) of v_mfma_f32_16x16x32_bf16 in lane l: rows 4 (l / 16) + i,
+// column l % 16.  A (16 x 32): lane l holds row l % 16, k = 8 (l / 16) .. + 7;
+// B (32 x 16): lane l holds column l % 16, k = 8 (l / 16) .. + 7.
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Forward and backward pass of 16 particles per wave and iteration.
+__global__ __launch_bounds__(kNetBS) void k_mlp_net(DevState d) {
+    __shared__ __attribute__((aligned(16))) __bf16 sW1[kMlpH * kStrK];   // W1 [n][k]: B of a1 = x W1^T
+    __shared__ __attribute__((aligned(16))) __bf16 sW1t[kMlpK * kStrH];  // W1^T [n][k]: B of gx = d1 W1
+    __shared__ __attribute__((aligned(16))) __bf16 sW2[kMlpH * kStrH];   // W2 [n][k]: B of a2 = h1 W2^T
+    __shared__ __attribute__((aligned(16))) __bf16 sW2t[kMlpH * kStrH];  // W2^T [n][k]: B of t = d2 W2
+    __shared__ __attribute__((aligned(16))) __bf16 sH[kNetWaves][16 * kStrH];
+    __shared__ float sb1[kMlpH], sb2[kMlpH], sw3[kMlpH];
+    const MlpConst& mc = *d.mlp_c;
+    const __bf16* w = d.mlp_w;
+    for (int i = threadIdx.x; i < kMlpH * kMlpK; i += kNetBS) {
+        sW1[(i / kMlpK) * kStrK + i % kMlpK] = w[kMlpBfW1 + i];
+        sW1t[(i / kMlpH) * kStrH + i % kMlpH] = w[kMlpBfW1t + i];
+    }
+    for (int i = threadIdx.x; i < kMlpH * kMlpH; i += kNetBS) {
+        sW2[(i / kMlpH) * kStrH + i % kMlpH] = w[kMlpBfW2 + i];
+        sW2t[(i / kMlpH) * kStrH + i % kMlpH] = w[kMlpBfW2t + i];
+    }
+    for (int i = threadIdx.x; i < kMlpH; i += kNetBS) {
+        sb1[i] = d.mlp_theta[kMlpOffB1 + i];
+        sb2[i] = d.mlp_theta[kMlpOffB2 + i];
+        sw3[i] = d.mlp_theta[kMlpOffW3 + i];
+    }
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int lr = l & 15, lq = l >> 4;  // fragment row / column index, k group
+    __bf16* H = sH[wv];
+    const int64_t nrow = d.n_slots / 16;  // slots come in whole tiles of >= 64
+    for (int64_t rb = (int64_t)blockIdx.x * kNetWaves + wv; rb < nrow; rb += (int64_t)gridDim.x * kNetWaves) {
+        const int64_t r0 = rb * 16;
+        // a1 = x W1^T + b1 (K = 32: one MFMA per 16 outputs)
+        const bf16x8 ax = *(const bf16x8*)(d.mlp_x + (r0 + lr) * kMlpK + lq * 8);
+        float s1[8][4];  // silu'(a1) in D layout
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            const bf16x8 bw = *(const bf16x8*)(sW1 + (nt * 16 + lr) * kStrK + lq * 8);
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma(ax, bw, acc);
+            const int col = nt * 16 + lr;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float ds;
+                const float h = silu_f(acc[i] + sb1[col], ds);
+                s1[nt][i] = ds;
+                H[(lq * 4 + i) * kStrH + col] = (__bf16)h;
+            }
+        }
+        // a2 = h1 W2^T + b2, then y = w3 . h2 and d2 = w3 * silu'(a2)
+        bf16x8 ah[4];
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) ah[kc] = *(const bf16x8*)(H + lr * kStrH + kc * 32 + lq * 8);
+        float y[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc)
+                acc = mfma(ah[kc], *(const bf16x8*)(sW2 + (nt * 16 + lr) * kStrH + kc * 32 + lq * 8), acc);
+            const int col = nt * 16 + lr;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float ds;
+                const float h = silu_f(acc[i] + sb2[col], ds);
+                y[i] = fmaf(sw3[col], h, y[i]);
+                H[(lq * 4 + i) * kStrH + col] = (__bf16)(sw3[col] * ds);
+            }
+        }
+        // t = d2 W2, d1 = t * silu'(a1)
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) ah[kc] = *(const bf16x8*)(H + lr * kStrH + kc * 32 + lq * 8);
+#pragma unroll
+        for (int nt = 0; nt < 8; ++nt) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc)
+                acc = mfma(ah[kc], *(const bf16x8*)(sW2t + (nt * 16 + lr) * kStrH + kc * 32 + lq * 8), acc);
+            const int col = nt * 16 + lr;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) H[(lq * 4 + i) * kStrH + col] = (__bf16)(acc[i] * s1[nt][i]);
+        }
+        // gx = d1 W1; dE/dG = e_scale * inv_std * gx
+#pragma unroll
+        for (int kc = 0; kc < 4; ++kc) ah[kc] = *(const bf16x8*)(H + lr * kStrH + kc * 32 + lq * 8);
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kc = 0; kc < 4; ++kc)
+                acc = mfma(ah[kc], *(const bf16x8*)(sW1t + (nt * 16 + lr) * kStrH + kc * 32 + lq * 8), acc);
+            const int col = nt * 16 + lr;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                d.mlp_gam[(r0 + lq * 4 + i) * kMlpK + col] = mc.e_scale * mc.g_inv_std[col] * acc[i];
+        }
+        // per-row energy: sum the 16 column lanes of each row group
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float v = y[i];
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+            if (lr == 0) d.mlp_e[r0 + lq * 4 + i] = mc.e_scale * (v + mc.b3) + mc.e_shift;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- forces
+
+// Pair forces from dE/dG of both particles; LJ forces where the surrogate
+// falls back and at the periodic check; then (stepping launches) the second
+// half kick and barostat scale of integrator.rs:47-58 into buffer cur.
+template <int BS, bool FORCE_ONLY>
+__global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
+    const int b = blockIdx.x;
+    const GroupInfo G = d.groups[d.blk_group[b]];
+    const int r = G.rep;
+    const RepCtrl& C = ctrl_in(d, r);
+    const uint32_t flags = FORCE_ONLY ? 0u : C.flags;
+    if (flags & (F_EXIT | F_BUILD | F_NULL | F_SPLIT_A)) {
+        prof_block(d, P_STEP, 0, false, 0);
+        return;
+    }
+    const uint64_t t_start = prof_now(d);
+    const MlpConst& mc = *d.mlp_c;
+    const RepParams& P = d.params[r];
+    const RepInfo R = d.reps[r];
+    const TileRange tr = block_tiles(d, G, b, BS);
+    const int lane = threadIdx.x;
+    const int cur = C.cur;
+    // stepping launches read the drifted state of cur ^ 1 and end in cur
+    const Rec<float>* __restrict__ src = (const Rec<float>*)d.rec[FORCE_ONLY ? cur : (cur ^ 1)];
+    Rec<float>* __restrict__ dst = (Rec<float>*)d.rec[cur];
+    const bool fallback = d.mlp_stat[r].oor != 0u;
+    const bool check = !FORCE_ONLY && mc.judge_every > 0 && (C.step + 1) % mc.judge_every == 0;
+    const bool lj = fallback || check;
+    const double bx = C.box[0], by = C.box[1], bz = C.box[2];
+    const double mu = FORCE_ONLY ? 1.0 : C.myu;
+    const double m = G.mass;
+    const double c = P.cm[G.type];
+    const PairParam p0 = d.pairs[0];
+    const double rc2 = (double)mc.rc * (double)mc.rc;
+    double part[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) part[k] = 0.0;
+    double jerr = 0.0, jref = 0.0;
+    for (int64_t t = tr.t0; t < tr.t1; ++t) {
+        const int64_t s = G.start + t * BS + lane;
+        if (t * BS + lane >= G.count) continue;
+        const Rec<float> a = src[s];
+        float gi[kMlpK];
+        {
+            const f32x4* gp = (const f32x4*)(d.mlp_gam + s * kMlpK);
+#pragma unroll
+            for (int q = 0; q < kMlpK / 4; ++q) {
+                const f32x4 v = gp[q];
+                gi[4 * q] = v[0];
+                gi[4 * q + 1] = v[1];
+                gi[4 * q + 2] = v[2];
+                gi[4 * q + 3] = v[3];
+            }
+        }
+        float fx = 0.f, fy = 0.f, fz = 0.f, vir = 0.f;
+        Acc<float> acc;  // LJ (fallback and check)
+        const int nn = d.ncount[s];
+        for (int q = 0; q < nn; ++q) {
+            const int32_t j = d.nbr[(int64_t)q * d.n_slots + s];
+            const Rec<float> o = src[j];
+            const double dx = min_image(o.x - a.x, bx);
+            const double dy = min_image(o.y - a.y, by);
+            const double dz = min_image(o.z - a.z, bz);
+            const double r2 = (dx * dx + dy * dy) + dz * dz;
+            if (lj && r2 <= p0.r2max) acc.add(dx, dy, dz, r2, p0);
+            if (!(r2 < rc2)) continue;
+            const float rr = (float)sqrt(r2);
+            const float arg = kPi * rr / mc.rc;
+            const float fc = 0.5f * (__cosf(arg) + 1.0f);
+            const float dfc = -0.5f * kPi / mc.rc * __sinf(arg);
+            const f32x4* gp = (const f32x4*)(d.mlp_gam + (int64_t)j * kMlpK);
+            float cc = 0.f;
+#pragma unroll
+            for (int q4 = 0; q4 < kMlpK / 4; ++q4) {
+                const f32x4 gj = gp[q4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int k = 4 * q4 + u;
+                    const float tt = rr - mc.mu[k];
+                    const float e = __expf(-tt * tt * mc.inv_2w2);
+                    const float dg = e * fmaf(-tt * mc.inv_w2, fc, dfc);
+                    cc = fmaf(gi[k] + gj[u], dg, cc);
+                }
+            }
+            const float ir = 1.0f / rr;
+            fx = fmaf(cc * ir, (float)dx, fx);
+            fy = fmaf(cc * ir, (float)dy, fy);
+            fz = fmaf(cc * ir, (float)dz, fz);
+            vir = fmaf(cc, rr, vir);
+        }
+        float pot = 2.0f * d.mlp_e[s];  // Particle::potential counts each energy twice
+        if (check) {
+            const double ex = (double)fx - (double)acc.fx, ey = (double)fy - (double)acc.fy,
+                         ez = (double)fz - (double)acc.fz;
+            jerr += (ex * ex + ey * ey) + ez * ez;
+            jref += ((double)acc.fx * acc.fx + (double)acc.fy * acc.fy) + (double)acc.fz * acc.fz;
+        }
+        if (fallback) {
+            fx = acc.fx;
+            fy = acc.fy;
+            fz = acc.fz;
+            pot = acc.pot;
+            vir = acc.vir;
+        }
+        Rec<float> out = a;
+        if (!FORCE_ONLY) {
+            out.vx = a.vx + (double)fx * c;
+            out.vy = a.vy + (double)fy * c;
+            out.vz = a.vz + (double)fz * c;
+            out.x = a.x * mu;
+            out.y = a.y * mu;
+            out.z = a.z * mu;
+        }
+        out.fx = fx;
+        out.fy = fy;
+        out.fz = fz;
+        dst[s] = out;
+        if (FORCE_ONLY || (flags & F_WRITE_PV)) {
+            d.pot[s] = (double)pot;
+            d.vir[s] = (double)vir;
+        }
+        part[S_MVX] += out.vx * m;
+        part[S_MVY] += out.vy * m;
+        part[S_MVZ] += out.vz * m;
+        part[S_MV2] += m * ((out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz);
+        part[S_POT] += (double)pot;
+        part[S_VIR] += (double)vir;
+        part[S_WV] = fmax(part[S_WV], (out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz);
+        part[S_WA] = fmax(part[S_WA], (((double)fx * fx + (double)fy * fy) + (double)fz * fz) * P.inv_m2[G.type]);
+        if (flags & F_INSERT) grid_insert(d, C, R, s, out.x, out.y, out.z);
+    }
+    block_store<BS>(part, d.partial + b, d.n_blocks, FORCE_ONLY ? kColsAll : kColsState);
+    if (check) {
+        double jv[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) jv[k] = 0.0;
+        jv[0] = jerr;
+        jv[1] = jref;
+        block_store<BS>(jv, d.mlp_judge + b, d.n_blocks, 3u);
+    }
+    prof_block(d, P_STEP, t_start, true, 0);
+}
+
+// FP32 parameters -> BF16 inference copies (W1, W1^T, W2, W2^T).
+__global__ __launch_bounds__(256) void k_mlp_convert(DevState d) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    __bf16* w = (__bf16*)d.mlp_w;
+    const float* th = d.mlp_theta;
+    if (i < kMlpH * kMlpK) {
+        const int n = i / kMlpK, k = i % kMlpK;
+        w[kMlpBfW1 + i] = (__bf16)th[kMlpOffW1 + i];
+        w[kMlpBfW1t + k * kMlpH + n] = (__bf16)th[kMlpOffW1 + i];
+    }
+    if (i < kMlpH * kMlpH) {
+        const int n = i / kMlpH, k = i % kMlpH;
+        w[kMlpBfW2 + i] = (__bf16)th[kMlpOffW2 + i];
+        w[kMlpBfW2t + k * kMlpH + n] = (__bf16)th[kMlpOffW2 + i];
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+hipError_t launch_mlp_desc(const DevState& d, int bs, bool force_only, hipStream_t st) {
+#define L(B)                                                                                        \
+    if (force_only) hipLaunchKernelGGL((k_mlp_desc<B, true>), dim3(d.n_blocks), dim3(B), 0, st, d); \
+    else hipLaunchKernelGGL((k_mlp_desc<B, false>), dim3(d.n_blocks), dim3(B), 0, st, d);
+    switch (bs) {
+        case 64: L(64) break;
+        case 128: L(128) break;
+        case 256: L(256) break;
+        case 512: L(512) break;
+        default: return hipErrorInvalidValue;
+    }
+#undef L
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_net(const DevState& d, hipStream_t st) {
+    const int64_t rows = d.n_slots / 16;
+    int grid = (int)std::min<int64_t>((rows + kNetWaves - 1) / kNetWaves, 1024);
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_mlp_net, dim3(grid), dim3(kNetBS), 0, st, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_force(const DevState& d, int bs, bool force_only, hipStream_t st) {
+#define L(B)                                                                                         \
+    if (force_only) hipLaunchKernelGGL((k_mlp_force<B, true>), dim3(d.n_blocks), dim3(B), 0, st, d); \
+    else hipLaunchKernelGGL((k_mlp_force<B, false>), dim3(d.n_blocks), dim3(B), 0, st, d);
+    switch (bs) {
+        case 64: L(64) break;
+        case 128: L(128) break;
+        case 256: L(256) break;
+        case 512: L(512) break;
+        default: return hipErrorInvalidValue;
+    }
+#undef L
+    return hipGetLastError();
+}
+
+hipError_t launch_mlp_convert(const DevState& d, hipStream_t st) {
+    hipLaunchKernelGGL(k_mlp_convert, dim3((kMlpH * kMlpH + 255) / 256), dim3(256), 0, st, d);
+    return hipGetLastError();
+}
+
+}  // namespace md

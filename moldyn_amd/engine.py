@@ -12,7 +12,7 @@ import numpy as np
 from . import _lib as L
 
 FORCE_MODES = {"fp64_allpairs": L.MD_FORCE_FP64_ALLPAIRS, "fp64_list": L.MD_FORCE_FP64_LIST,
-               "fp32_list": L.MD_FORCE_FP32_LIST}
+               "fp32_list": L.MD_FORCE_FP32_LIST, "mlp": L.MD_FORCE_MLP}
 KERNELS = {"step": 0, "finalize": 1, "allpairs": 2, "drift": 3}
 
 
@@ -160,6 +160,33 @@ class Engine:
         return {"kinetic": out[0], "potential": out[1], "thermal": out[2], "temperature": out[3],
                 "pressure": out[4], "momentum": out[5:8].copy(), "com_velocity": out[8:11].copy()}
 
+    # -- learned surrogate (force_mode "mlp")
+    def set_mlp(self, theta, g_mean, g_inv_std, g_lo=None, g_hi=None, e_scale=1.0, e_shift=0.0,
+                judge_every=100):
+        """Network parameters (flat, W1|b1|W2|b2|w3|b3), input normalisation,
+        training range of the features (None: no fallback) and the LJ check interval."""
+        def f32(a, n):
+            a = np.ascontiguousarray(a, dtype=np.float32).ravel()
+            if a.size != n:
+                raise ValueError(f"expected {n} values, got {a.size}")
+            return a
+        fp = C.POINTER(C.c_float)
+        th = f32(theta, L.MD_MLP_NUM_PARAMS)
+        gm, gs = f32(g_mean, 32), f32(g_inv_std, 32)
+        lo = f32(g_lo, 32) if g_lo is not None else None
+        hi = f32(g_hi, 32) if g_hi is not None else None
+        self._chk(self._lib.md_mlp_set_params(
+            self._h, th.ctypes.data_as(fp), gm.ctypes.data_as(fp), gs.ctypes.data_as(fp),
+            lo.ctypes.data_as(fp) if lo is not None else None,
+            hi.ctypes.data_as(fp) if hi is not None else None,
+            float(e_scale), float(e_shift), int(judge_every)))
+
+    def mlp_stats(self, replica=0):
+        out = np.zeros(5)
+        self._chk(self._lib.md_mlp_get_stats(self._h, replica, _dp(out)))
+        return {"last_rmse_rel": out[0], "max_rmse_rel": out[1], "checks": int(out[2]),
+                "fallbacks": int(out[3]), "last_rms_force": out[4]}
+
     # -- observability
     def stats(self):
         s = L.MdStats()
@@ -169,11 +196,14 @@ class Engine:
     def set_profiling(self, on):
         self._chk(self._lib.md_set_profiling(self._h, 1 if on else 0))
 
-    def kernel_time(self, kernel):
-        ms = C.c_double()
-        cnt = C.c_int64()
-        self._chk(self._lib.md_kernel_time(self._h, KERNELS[kernel], C.byref(ms), C.byref(cnt)))
-        return ms.value, cnt.value
+    def kernel_time(self, kernel, span=False):
+        """(total ms, launches) of a kernel while profiling: stream time (previous
+        kernel end to end), or with span=True first block start to last block end."""
+        ms, cnt = C.c_double(), C.c_int64()
+        sms, scnt = C.c_double(), C.c_int64()
+        self._chk(self._lib.md_kernel_time(self._h, KERNELS[kernel], C.byref(ms), C.byref(cnt),
+                                           C.byref(sms), C.byref(scnt)))
+        return (sms.value, scnt.value) if span else (ms.value, cnt.value)
 
 
 def thermostat(kind, tau=0.0, target_t=0.0, lambda_=0.0, psi=0.0):

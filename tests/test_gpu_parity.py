@@ -252,6 +252,28 @@ def test_list_schedule_invariance(mode):
         assert np.array_equal(o["force"], outs[0]["force"])
 
 
+@pytest.mark.parametrize("mode", ["fp64_list", "fp32_list"])
+def test_overflow_replay_under_thermostat_and_barostat(mode):
+    """A capacity overflow replays the chunk from its snapshot; the replay's
+    first controls come from the sums at the chunk start, so an NPT
+    trajectory with replays equals one without."""
+    import moldyn_amd as M
+    s = sc_state(8, a=0.55, temperature=120.0, seed=7, jitter=0.04)
+    outs = []
+    for kw in [dict(), dict(list_capacity=2, cell_capacity=1)]:
+        e = engine_for(s, mode, **kw)
+        e.update_force()
+        th = M.thermostat("berendsen", tau=1.0, target_t=150.0)
+        ba = M.barostat("berendsen", beta=1.0, tau=5.0, target_p=5.0)
+        e.step(0.002, 300, th, ba)
+        outs.append((e.download(), th.lambda_, ba.myu, e.stats()))
+        e.close()
+    (a, la, ma, sa), (b, lb, mb, sb) = outs
+    assert sb["retries"] >= 1
+    assert np.array_equal(a["x"], b["x"]) and np.array_equal(a["v"], b["v"])
+    assert np.array_equal(a["box"], b["box"]) and la == lb and ma == mb
+
+
 def test_fp64_list_equals_allpairs_trajectory():
     s = liquid_state(4, temperature=150.0)
     a = engine_for(s, "fp64_allpairs")
@@ -512,11 +534,14 @@ def test_device_kernel_timing():
         e.step(0.002, 700, th)
         outs.append(e.download(("x", "v")))
         if prof:
-            ms, n = e.kernel_time("step")
-            fms, fn = e.kernel_time("finalize")
+            ms, n = e.kernel_time("step", span=True)
+            fms, fn = e.kernel_time("finalize", span=True)
             st = e.stats()
             assert n >= 700 - st["rebuilds"] - 1 and n <= st["launches"]
             assert fn == st["launches"]
             assert 0.0 < ms / n < 5.0 and 0.0 < fms / fn < 5.0
+            # stream time: back-to-back launches only, never shorter than the span
+            sms, sn = e.kernel_time("step")
+            assert 0.8 * n <= sn <= n and sms / sn >= 0.99 * ms / n
         e.close()
     assert np.array_equal(outs[0]["x"], outs[1]["x"]) and np.array_equal(outs[0]["v"], outs[1]["v"])
