@@ -146,8 +146,8 @@ def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log):
     """Surrogate: 4,096-atom LJ argon frames from thermostatted SC runs at
     lattice edges 0.38-0.7 nm and 80-400 K, labelled by the FP64 LJ path,
     trained on the GPU; inference on a 64^3 (262,144-atom) SC grid at
-    a = 0.47 nm, Berendsen T 150 K and P at the state's own pressure, against
-    the FP32 LJ path on the same run."""
+    a = 0.47 nm equilibrated at 150 K, then Berendsen T and P at the
+    equilibrated pressure, against the FP32 LJ path on the same run."""
     from moldyn_amd import mlp
     t0 = time.perf_counter()
     fr = mlp.make_frames(n_side=16, n_frames=n_frames, runs=20, equil=400, every=20, seed=5)
@@ -168,7 +168,10 @@ def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log):
         f = e.download(("force",))["force"]
         e.close()
         rel.append(float(np.sqrt(np.mean((f - fr.f_ref[k]) ** 2)) / np.sqrt(np.mean(fr.f_ref[k] ** 2))))
-    # inference run: 64^3 SC at a = 0.47, 150 K
+    # inference run: 64^3 SC at a = 0.47, equilibrated 500 steps at 150 K by
+    # the FP32 LJ path (Berendsen T, tau 0.1 ps); both paths then continue from
+    # that state under Berendsen T (tau 1 ps) and P (beta 1, tau 50 ps, P0 =
+    # the equilibrated state's pressure)
     n_side, a = 64, 0.47
     n = n_side ** 3
     box = np.full(3, n_side * a)
@@ -178,27 +181,38 @@ def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log):
     probe = M.Engine([n], [MASS], box, force_mode="fp32_list")
     probe.upload(pos, vel, box)
     probe.update_force()
-    p_state = probe.macro()["pressure"]
+    th_eq = M.thermostat("berendsen", tau=0.1, target_t=150.0)
+    probe.step(DT, 400, th_eq)
+    p_hist = []
+    for _ in range(5):
+        probe.step(DT, 20, th_eq)
+        p_hist.append(probe.macro()["pressure"])
+    p_state = float(np.mean(p_hist))
+    st = probe.download(("x", "v", "box"))
     probe.close()
     res = {}
     for mode in ("mlp", "fp32_list"):
-        e = M.Engine([n], [MASS], box, force_mode=mode)
+        e = M.Engine([n], [MASS], st["box"], force_mode=mode)
         if mode == "mlp":
             sur.apply(e, judge_every=100, range_check=True)
-        e.upload(pos, vel, box)
+        e.upload(st["x"], st["v"], st["box"])
         e.update_force()
         th = M.thermostat("berendsen", tau=1.0, target_t=150.0)
-        ba = M.barostat("berendsen", beta=1.0, tau=5.0, target_p=p_state)
+        ba = M.barostat("berendsen", beta=1.0, tau=50.0, target_p=p_state)
         e.step(DT, 20, th, ba)
         ts = time.perf_counter()
         e.step(DT, steps, th, ba)
         wall = time.perf_counter() - ts
-        res[mode] = {"atom_steps_per_s": n * steps / wall, "ms_per_step": 1e3 * wall / steps}
+        mac = e.macro()
+        res[mode] = {"atom_steps_per_s": n * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                     "temperature_final": mac["temperature"], "pressure_final": mac["pressure"],
+                     "box_final": float(e.download(("box",))["box"][0])}
         if mode == "mlp":
             res[mode].update(e.mlp_stats())
         e.close()
-    return {"workload": f"64^3 SC argon ({n} atoms) at a = {a} nm, 150 K, Berendsen T and P "
-                        f"(P0 = initial pressure {p_state:.4g}), dt {DT}, {steps} steps",
+    return {"workload": f"64^3 SC argon ({n} atoms) at a = {a} nm equilibrated 500 steps at 150 K, then Berendsen T "
+                        f"(150 K, tau 1) and P (beta 1, tau 50, P0 = equilibrated pressure {p_state:.4g}), "
+                        f"dt {DT}, {steps} steps; box at start {float(st['box'][0]):.4f}",
             "training": {"frames": int(len(train_idx)), "atoms_per_frame": 4096, "epochs": epochs,
                          "frame_generation_s": t_gen, "training_s": t_train,
                          "final_loss": sur.history[-1], "first_loss": sur.history[0]},

@@ -1,12 +1,14 @@
 // md_mlp_train.hip -- training kernels of the learned per-atom energy
 // surrogate (FP32), on frames of one periodic system size.
 //
-// A minibatch of M atoms (whole frames) goes through
-//   features   G (all pairs of each frame, computed once per data set),
+// Pair lists (all pairs within the cutoff, ascending j) and features are
+// built once per data set.  A minibatch of M atoms (whole frames, taken in any
+// order through a frame list) goes through
+//   features   G,
 //   forward    A1 = X W1^T, H1 = silu(A1 + b1), A2 = H1 W2^T, H2 = silu(A2 + b2),
 //              y = H2 w3 + b3, E = e_scale y + e_shift,
 //   backward   D2 = w3 * silu'(A2), T = D2 W2, D1 = T * silu'(A1), dE/dG = e_scale inv_std (D1 W1),
-//   forces     F_i = sum_j (c_i + c_j)(r_ij) d_ij / r_ij (all pairs of the frame),
+//   forces     F_i = sum_j (c_i + c_j)(r_ij) d_ij / r_ij (pair lists),
 //   loss       L = w_e mean (E - E_ref)^2 + w_f mean (F - F_ref)^2,
 //   gradient   the energy term by back-propagation and the force term by the
 //              double backward through dE/dG:
@@ -111,15 +113,15 @@ __global__ __launch_bounds__(256) void k_split_sum(const float* __restrict__ par
     out[i] = accumulate ? out[i] + s : s;
 }
 
-// ---------------------------------------------------------------- features
+// ---------------------------------------------------------------- pairs, features
 
-// G of every atom of frames [f0, f0 + nf): all pairs of the frame, positions
-// staged in LDS 256 at a time (the reference minimum image).
-__global__ __launch_bounds__(256) void k_tr_features(TrainState t, int f0, int nf) {
+// Positions of frame f staged in LDS 256 at a time; atom i visits every
+// j != i of its frame in ascending order and calls visit(j) for the pairs with
+// r^2 < rc^2 (the reference minimum image).
+template <class Visit>
+__device__ __forceinline__ void tr_all_pairs(const TrainState& t, int f, int i, Visit visit) {
     __shared__ double sx[256], sy[256], sz[256];
-    const int f = f0 + blockIdx.y;
     const int n = t.n_atoms;
-    const int i = blockIdx.x * 256 + threadIdx.x;
     const double* P = t.pos + (int64_t)f * n * 3;
     const double bx = t.box[3 * f], by = t.box[3 * f + 1], bz = t.box[3 * f + 2];
     double xi = 0, yi = 0, zi = 0;
@@ -128,9 +130,6 @@ __global__ __launch_bounds__(256) void k_tr_features(TrainState t, int f0, int n
         yi = P[3 * i + 1];
         zi = P[3 * i + 2];
     }
-    float g[kMlpK];
-#pragma unroll
-    for (int k = 0; k < kMlpK; ++k) g[k] = 0.0f;
     const double rc2 = (double)t.rc * t.rc;
     for (int j0 = 0; j0 < n; j0 += 256) {
         __syncthreads();
@@ -142,28 +141,83 @@ __global__ __launch_bounds__(256) void k_tr_features(TrainState t, int f0, int n
         }
         __syncthreads();
         const int cnt = min(256, n - j0);
-        if (i < n) {
+        if (i < n)
             for (int q = 0; q < cnt; ++q) {
                 if (j0 + q == i) continue;
                 const double dx = min_image(sx[q] - xi, bx), dy = min_image(sy[q] - yi, by),
                              dz = min_image(sz[q] - zi, bz);
                 const double r2 = (dx * dx + dy * dy) + dz * dz;
-                if (!(r2 < rc2)) continue;
-                const float r = (float)sqrt(r2);
-                const float fc = 0.5f * (cosf(kPiT * r / t.rc) + 1.0f);
-#pragma unroll
-                for (int k = 0; k < kMlpK; ++k) {
-                    const float u = r - t.mu[k];
-                    g[k] = fmaf(expf(-u * u * t.inv_2w2), fc, g[k]);
-                }
+                if (r2 < rc2) visit(j0 + q);
             }
+    }
+}
+
+// neighbour counts of every atom of frames [f0, f0 + nf), and their maximum
+__global__ __launch_bounds__(256) void k_tr_count(TrainState t, int f0, int nf, int32_t* max_cnt) {
+    const int f = f0 + blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int c = 0;
+    tr_all_pairs(t, f, i, [&](int) { ++c; });
+    if (i < t.n_atoms) {
+        t.ncnt[(int64_t)f * t.n_atoms + i] = c;
+        atomicMax(max_cnt, c);
+    }
+}
+
+// neighbour indices (ascending) of every atom of frames [f0, f0 + nf)
+__global__ __launch_bounds__(256) void k_tr_fill(TrainState t, int f0, int nf) {
+    const int f = f0 + blockIdx.y;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    int32_t* out = t.nbr + ((int64_t)f * t.n_atoms + (i < t.n_atoms ? i : 0)) * t.cap;
+    int c = 0;
+    tr_all_pairs(t, f, i, [&](int j) {
+        if (c < t.cap) out[c] = j;
+        ++c;
+    });
+}
+
+// displacement x_j - x_i (minimum image) and distance of a listed pair
+struct TrPair {
+    double dx, dy, dz;
+    float r;
+};
+__device__ __forceinline__ TrPair tr_pair(const double* P, double xi, double yi, double zi, double bx, double by,
+                                          double bz, int j) {
+    TrPair p;
+    p.dx = min_image(P[3 * j] - xi, bx);
+    p.dy = min_image(P[3 * j + 1] - yi, by);
+    p.dz = min_image(P[3 * j + 2] - zi, bz);
+    p.r = (float)sqrt((p.dx * p.dx + p.dy * p.dy) + p.dz * p.dz);
+    return p;
+}
+
+// G of every atom of frames [f0, f0 + nf) from the pair lists
+__global__ __launch_bounds__(256) void k_tr_features(TrainState t, int f0, int nf) {
+    const int f = f0 + blockIdx.y;
+    const int n = t.n_atoms;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int64_t row = (int64_t)f * n + i;
+    const double* P = t.pos + (int64_t)f * n * 3;
+    const double bx = t.box[3 * f], by = t.box[3 * f + 1], bz = t.box[3 * f + 2];
+    const double xi = P[3 * i], yi = P[3 * i + 1], zi = P[3 * i + 2];
+    float g[kMlpK];
+#pragma unroll
+    for (int k = 0; k < kMlpK; ++k) g[k] = 0.0f;
+    const int cnt = t.ncnt[row];
+    const int32_t* nb = t.nbr + row * t.cap;
+    for (int q = 0; q < cnt; ++q) {
+        const TrPair p = tr_pair(P, xi, yi, zi, bx, by, bz, nb[q]);
+        const float fc = 0.5f * (cosf(kPiT * p.r / t.rc) + 1.0f);
+#pragma unroll
+        for (int k = 0; k < kMlpK; ++k) {
+            const float u = p.r - t.mu[k];
+            g[k] = fmaf(expf(-u * u * t.inv_2w2), fc, g[k]);
         }
     }
-    if (i < n) {
-        float* out = t.G + ((int64_t)f * n + i) * kMlpK;
+    float* out = t.G + row * kMlpK;
 #pragma unroll
-        for (int k = 0; k < kMlpK; ++k) out[k] = g[k];
-    }
+    for (int k = 0; k < kMlpK; ++k) out[k] = g[k];
 }
 
 // per-feature sum, sum of squares, min, max over all atoms (block partials)
@@ -201,12 +255,15 @@ __global__ __launch_bounds__(256) void k_tr_feature_stats(TrainState t, int64_t 
 
 // ---------------------------------------------------------------- elementwise
 
-// X = (G - mean) inv_std for rows [r0, r0 + M)
-__global__ __launch_bounds__(256) void k_tr_normalize(TrainState t, int64_t r0, int64_t M) {
+// X = (G - mean) inv_std for the M rows of the minibatch (slot b = frame bf[b])
+__global__ __launch_bounds__(256) void k_tr_normalize(TrainState t, int64_t M) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= M * kMlpK) return;
     const int k = (int)(e % kMlpK);
-    t.X[e] = (t.G[r0 * kMlpK + e] - t.g_mean[k]) * t.g_inv_std[k];
+    const int64_t r = e / kMlpK;
+    const int64_t b = r / t.n_atoms, i = r % t.n_atoms;
+    const int64_t src = ((int64_t)t.bf[b] * t.n_atoms + i) * kMlpK + k;
+    t.X[e] = (t.G[src] - t.g_mean[k]) * t.g_inv_std[k];
 }
 
 // layer activations: H = silu(A + b), S = silu', Sp = silu'' (in place over A -> H)
@@ -269,70 +326,45 @@ __global__ __launch_bounds__(256) void k_tr_g1(TrainState t, int64_t M) {
 
 // ---------------------------------------------------------------- pair kernels
 
-// Forces of frames [f0, f0 + nf) from Gam (atom-major rows r = f n + i), the
-// loss terms and their derivatives phi = dL/dF, dy = dL/dy.
-__global__ __launch_bounds__(256) void k_tr_forces(TrainState t, int f0, int nf, double* loss_part) {
-    __shared__ double sx[256], sy[256], sz[256];
-    __shared__ float sg[256][kMlpK + 1];
-    const int fl = blockIdx.y, f = f0 + fl;
+// Forces of the minibatch frames from Gam (rows r = b n + i), the loss terms
+// and their derivatives phi = dL/dF, dy = dL/dy.
+__global__ __launch_bounds__(256) void k_tr_forces(TrainState t, double* loss_part) {
+    const int b = blockIdx.y, f = t.bf[b];
     const int n = t.n_atoms;
     const int i = blockIdx.x * 256 + threadIdx.x;
-    const double* P = t.pos + (int64_t)f * n * 3;
-    const double bx = t.box[3 * f], by = t.box[3 * f + 1], bz = t.box[3 * f + 2];
-    const int64_t ri = (int64_t)fl * n + i;  // row in the minibatch
-    double xi = 0, yi = 0, zi = 0;
-    float gi[kMlpK];
-#pragma unroll
-    for (int k = 0; k < kMlpK; ++k) gi[k] = 0.0f;
-    if (i < n) {
-        xi = P[3 * i];
-        yi = P[3 * i + 1];
-        zi = P[3 * i + 2];
-#pragma unroll
-        for (int k = 0; k < kMlpK; ++k) gi[k] = t.Gam[ri * kMlpK + k];
-    }
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-    const double rc2 = (double)t.rc * t.rc;
-    for (int j0 = 0; j0 < n; j0 += 256) {
-        __syncthreads();
-        const int jl = j0 + threadIdx.x;
-        if (jl < n) {
-            sx[threadIdx.x] = P[3 * jl];
-            sy[threadIdx.x] = P[3 * jl + 1];
-            sz[threadIdx.x] = P[3 * jl + 2];
-            for (int k = 0; k < kMlpK; ++k) sg[threadIdx.x][k] = t.Gam[((int64_t)fl * n + jl) * kMlpK + k];
-        }
-        __syncthreads();
-        const int cnt = min(256, n - j0);
-        if (i < n) {
-            for (int q = 0; q < cnt; ++q) {
-                if (j0 + q == i) continue;
-                const double dx = min_image(sx[q] - xi, bx), dy = min_image(sy[q] - yi, by),
-                             dz = min_image(sz[q] - zi, bz);
-                const double r2 = (dx * dx + dy * dy) + dz * dz;
-                if (!(r2 < rc2)) continue;
-                const float r = (float)sqrt(r2);
-                const float a = kPiT * r / t.rc;
-                const float fc = 0.5f * (cosf(a) + 1.0f), dfc = -0.5f * kPiT / t.rc * sinf(a);
-                float c = 0.f;
-#pragma unroll
-                for (int k = 0; k < kMlpK; ++k) {
-                    const float u = r - t.mu[k];
-                    const float e = expf(-u * u * t.inv_2w2);
-                    c = fmaf(gi[k] + sg[q][k], e * fmaf(-u * t.inv_w2, fc, dfc), c);
-                }
-                const float s = c / r;
-                fx = fmaf(s, (float)dx, fx);
-                fy = fmaf(s, (float)dy, fy);
-                fz = fmaf(s, (float)dz, fz);
-            }
-        }
-    }
     double le = 0.0, lf = 0.0;
     if (i < n) {
-        const int64_t gi_row = (int64_t)f * n + i;  // row in the data set
-        const float ex = fx - t.Fref[3 * gi_row], ey = fy - t.Fref[3 * gi_row + 1], ez = fz - t.Fref[3 * gi_row + 2];
-        const float ee = t.E[ri] - t.Eref[gi_row];
+        const int64_t ri = (int64_t)b * n + i;  // row in the minibatch
+        const int64_t di = (int64_t)f * n + i;  // row in the data set
+        const double* P = t.pos + (int64_t)f * n * 3;
+        const double bx = t.box[3 * f], by = t.box[3 * f + 1], bz = t.box[3 * f + 2];
+        const double xi = P[3 * i], yi = P[3 * i + 1], zi = P[3 * i + 2];
+        float gi[kMlpK];
+#pragma unroll
+        for (int k = 0; k < kMlpK; ++k) gi[k] = t.Gam[ri * kMlpK + k];
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        const int cnt = t.ncnt[di];
+        const int32_t* nb = t.nbr + di * t.cap;
+        for (int q = 0; q < cnt; ++q) {
+            const int j = nb[q];
+            const TrPair p = tr_pair(P, xi, yi, zi, bx, by, bz, j);
+            const float a = kPiT * p.r / t.rc;
+            const float fc = 0.5f * (cosf(a) + 1.0f), dfc = -0.5f * kPiT / t.rc * sinf(a);
+            const float* gj = t.Gam + ((int64_t)b * n + j) * kMlpK;
+            float c = 0.f;
+#pragma unroll
+            for (int k = 0; k < kMlpK; ++k) {
+                const float u = p.r - t.mu[k];
+                const float e = expf(-u * u * t.inv_2w2);
+                c = fmaf(gi[k] + gj[k], e * fmaf(-u * t.inv_w2, fc, dfc), c);
+            }
+            const float s = c / p.r;
+            fx = fmaf(s, (float)p.dx, fx);
+            fy = fmaf(s, (float)p.dy, fy);
+            fz = fmaf(s, (float)p.dz, fz);
+        }
+        const float ex = fx - t.Fref[3 * di], ey = fy - t.Fref[3 * di + 1], ez = fz - t.Fref[3 * di + 2];
+        const float ee = t.E[ri] - t.Eref[di];
         t.F[3 * ri] = fx;
         t.F[3 * ri + 1] = fy;
         t.F[3 * ri + 2] = fz;
@@ -350,80 +382,59 @@ __global__ __launch_bounds__(256) void k_tr_forces(TrainState t, int f0, int nf,
     sl[1][threadIdx.x] = lf;
     __syncthreads();
     if (threadIdx.x == 0) {
-        double a = 0, b = 0;
+        double sa = 0, sb = 0;
         for (int q = 0; q < 256; ++q) {
-            a += sl[0][q];
-            b += sl[1][q];
+            sa += sl[0][q];
+            sb += sl[1][q];
         }
         const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
-        loss_part[2 * blk] = a;
-        loss_part[2 * blk + 1] = b;
+        loss_part[2 * blk] = sa;
+        loss_part[2 * blk + 1] = sb;
     }
 }
 
+// out[0..1] = block partials of the squared errors summed in block order
+__global__ __launch_bounds__(64) void k_tr_loss_sum(const double* __restrict__ part, int64_t blocks,
+                                                   double* __restrict__ out) {
+    if (threadIdx.x >= 2) return;
+    double s = 0.0;
+    for (int64_t q = 0; q < blocks; ++q) s += part[2 * q + threadIdx.x];
+    out[threadIdx.x] = s;
+}
+
 // U_i = sum_j ((phi_i - phi_j) . d_ij / r_ij) dg(r_ij)
-__global__ __launch_bounds__(256) void k_tr_u(TrainState t, int f0, int nf) {
-    __shared__ double sx[256], sy[256], sz[256];
-    __shared__ float sp[256][3];
-    const int fl = blockIdx.y, f = f0 + fl;
+__global__ __launch_bounds__(256) void k_tr_u(TrainState t) {
+    const int b = blockIdx.y, f = t.bf[b];
     const int n = t.n_atoms;
     const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int64_t ri = (int64_t)b * n + i;
+    const int64_t di = (int64_t)f * n + i;
     const double* P = t.pos + (int64_t)f * n * 3;
     const double bx = t.box[3 * f], by = t.box[3 * f + 1], bz = t.box[3 * f + 2];
-    const int64_t ri = (int64_t)fl * n + i;
-    double xi = 0, yi = 0, zi = 0;
-    float px = 0, py = 0, pz = 0;
-    if (i < n) {
-        xi = P[3 * i];
-        yi = P[3 * i + 1];
-        zi = P[3 * i + 2];
-        px = t.phi[3 * ri];
-        py = t.phi[3 * ri + 1];
-        pz = t.phi[3 * ri + 2];
-    }
+    const double xi = P[3 * i], yi = P[3 * i + 1], zi = P[3 * i + 2];
+    const float px = t.phi[3 * ri], py = t.phi[3 * ri + 1], pz = t.phi[3 * ri + 2];
     float u[kMlpK];
 #pragma unroll
     for (int k = 0; k < kMlpK; ++k) u[k] = 0.0f;
-    const double rc2 = (double)t.rc * t.rc;
-    for (int j0 = 0; j0 < n; j0 += 256) {
-        __syncthreads();
-        const int jl = j0 + threadIdx.x;
-        if (jl < n) {
-            sx[threadIdx.x] = P[3 * jl];
-            sy[threadIdx.x] = P[3 * jl + 1];
-            sz[threadIdx.x] = P[3 * jl + 2];
-            const int64_t rj = (int64_t)fl * n + jl;
-            sp[threadIdx.x][0] = t.phi[3 * rj];
-            sp[threadIdx.x][1] = t.phi[3 * rj + 1];
-            sp[threadIdx.x][2] = t.phi[3 * rj + 2];
-        }
-        __syncthreads();
-        const int cnt = min(256, n - j0);
-        if (i < n) {
-            for (int q = 0; q < cnt; ++q) {
-                if (j0 + q == i) continue;
-                const double dx = min_image(sx[q] - xi, bx), dy = min_image(sy[q] - yi, by),
-                             dz = min_image(sz[q] - zi, bz);
-                const double r2 = (dx * dx + dy * dy) + dz * dz;
-                if (!(r2 < rc2)) continue;
-                const float r = (float)sqrt(r2);
-                const float proj = ((px - sp[q][0]) * (float)dx + (py - sp[q][1]) * (float)dy +
-                                    (pz - sp[q][2]) * (float)dz) / r;
-                const float a = kPiT * r / t.rc;
-                const float fc = 0.5f * (cosf(a) + 1.0f), dfc = -0.5f * kPiT / t.rc * sinf(a);
+    const int cnt = t.ncnt[di];
+    const int32_t* nb = t.nbr + di * t.cap;
+    for (int q = 0; q < cnt; ++q) {
+        const int j = nb[q];
+        const TrPair p = tr_pair(P, xi, yi, zi, bx, by, bz, j);
+        const float* pj = t.phi + 3 * ((int64_t)b * n + j);
+        const float proj = ((px - pj[0]) * (float)p.dx + (py - pj[1]) * (float)p.dy + (pz - pj[2]) * (float)p.dz) / p.r;
+        const float a = kPiT * p.r / t.rc;
+        const float fc = 0.5f * (cosf(a) + 1.0f), dfc = -0.5f * kPiT / t.rc * sinf(a);
 #pragma unroll
-                for (int k = 0; k < kMlpK; ++k) {
-                    const float w = r - t.mu[k];
-                    const float e = expf(-w * w * t.inv_2w2);
-                    u[k] = fmaf(proj, e * fmaf(-w * t.inv_w2, fc, dfc), u[k]);
-                }
-            }
+        for (int k = 0; k < kMlpK; ++k) {
+            const float w = p.r - t.mu[k];
+            const float e = expf(-w * w * t.inv_2w2);
+            u[k] = fmaf(proj, e * fmaf(-w * t.inv_w2, fc, dfc), u[k]);
         }
     }
-    if (i < n) {
 #pragma unroll
-        for (int k = 0; k < kMlpK; ++k) t.U[ri * kMlpK + k] = u[k];
-    }
+    for (int k = 0; k < kMlpK; ++k) t.U[ri * kMlpK + k] = u[k];
 }
 
 // column sums of an M x ncol matrix into out[0..ncol) (block partials, fixed order)
@@ -483,6 +494,16 @@ hipError_t tr_split_sum(const float* part, int splits, int64_t n, float* out, bo
     return hipGetLastError();
 }
 
+hipError_t tr_count_pairs(const TrainState& t, int f0, int nf, int32_t* max_cnt, hipStream_t st) {
+    hipLaunchKernelGGL(k_tr_count, dim3(nblk(t.n_atoms), nf), dim3(256), 0, st, t, f0, nf, max_cnt);
+    return hipGetLastError();
+}
+
+hipError_t tr_fill_pairs(const TrainState& t, int f0, int nf, hipStream_t st) {
+    hipLaunchKernelGGL(k_tr_fill, dim3(nblk(t.n_atoms), nf), dim3(256), 0, st, t, f0, nf);
+    return hipGetLastError();
+}
+
 hipError_t tr_features(const TrainState& t, int f0, int nf, hipStream_t st) {
     hipLaunchKernelGGL(k_tr_features, dim3(nblk(t.n_atoms), nf), dim3(256), 0, st, t, f0, nf);
     return hipGetLastError();
@@ -493,10 +514,11 @@ hipError_t tr_feature_stats(const TrainState& t, int64_t rows, double* part, int
     return hipGetLastError();
 }
 
-hipError_t tr_forward_backward(const TrainState& t, int64_t r0, int64_t M, hipStream_t st) {
+hipError_t tr_forward_backward(const TrainState& t, int nf, hipStream_t st) {
     hipError_t e;
     const float* th = t.theta;
-    hipLaunchKernelGGL(k_tr_normalize, dim3(nblk(M * kMlpK)), dim3(256), 0, st, t, r0, M);
+    const int64_t M = (int64_t)nf * t.n_atoms;
+    hipLaunchKernelGGL(k_tr_normalize, dim3(nblk(M * kMlpK)), dim3(256), 0, st, t, M);
     // A1 = X W1^T -> H1
     if ((e = tr_gemm(false, true, (int)M, kMlpH, kMlpK, 1.f, t.X, kMlpK, th + kMlpOffW1, kMlpK, 0.f, t.H1, kMlpH, 1, st))) return e;
     hipLaunchKernelGGL(k_tr_act, dim3(nblk(M * kMlpH)), dim3(256), 0, st, M, th + kMlpOffB1, t.H1, t.S1, t.S1p);
@@ -512,16 +534,22 @@ hipError_t tr_forward_backward(const TrainState& t, int64_t r0, int64_t M, hipSt
     return hipGetLastError();
 }
 
-hipError_t tr_forces(const TrainState& t, int f0, int nf, double* loss_part, hipStream_t st) {
-    hipLaunchKernelGGL(k_tr_forces, dim3(nblk(t.n_atoms), nf), dim3(256), 0, st, t, f0, nf, loss_part);
+hipError_t tr_forces(const TrainState& t, int nf, double* loss_part, hipStream_t st) {
+    hipLaunchKernelGGL(k_tr_forces, dim3(nblk(t.n_atoms), nf), dim3(256), 0, st, t, loss_part);
+    return hipGetLastError();
+}
+
+hipError_t tr_loss_sum(const TrainState& t, int nf, const double* loss_part, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_tr_loss_sum, dim3(1), dim3(64), 0, st, loss_part, (int64_t)nblk(t.n_atoms) * nf, out);
     return hipGetLastError();
 }
 
 // Gradient of the minibatch loss added into t.grad (which the caller clears).
-hipError_t tr_gradient(const TrainState& t, int f0, int nf, int64_t M, hipStream_t st) {
+hipError_t tr_gradient(const TrainState& t, int nf, hipStream_t st) {
     hipError_t e;
     const float* th = t.theta;
-    hipLaunchKernelGGL(k_tr_u, dim3(nblk(t.n_atoms), nf), dim3(256), 0, st, t, f0, nf);
+    const int64_t M = (int64_t)nf * t.n_atoms;
+    hipLaunchKernelGGL(k_tr_u, dim3(nblk(t.n_atoms), nf), dim3(256), 0, st, t);
     hipLaunchKernelGGL(k_tr_scale_in, dim3(nblk(M * kMlpK)), dim3(256), 0, st, t, M, t.U);
     // V1 = Ut W1^T, PP = V1 S1, V2 = PP W2^T
     if ((e = tr_gemm(false, true, (int)M, kMlpH, kMlpK, 1.f, t.U, kMlpK, th + kMlpOffW1, kMlpK, 0.f, t.V1, kMlpH, 1, st))) return e;

@@ -21,6 +21,9 @@ struct md_trainer {
     int64_t batch_rows = 0;  // capacity of the minibatch arrays (atoms)
     int step = 0;            // Adam steps taken
     double* loss_part = nullptr;
+    int32_t* frames_seq = nullptr;  // 0, 1, ..., n_frames - 1
+    int32_t* frames_ord = nullptr;  // the epoch's frame order
+    double* batch_loss = nullptr;   // per minibatch: summed squared energy and force errors
     std::string err;
 };
 
@@ -82,27 +85,27 @@ static int ensure_batch(md_trainer* c, int64_t rows) {
     return MD_OK;
 }
 
-// Forward, forces and (optionally) gradient of frames [f0, f0 + nf); returns
-// the summed squared energy and force errors.
-static int run_batch(md_trainer* c, int f0, int nf, bool grad, double* se, double* sf) {
+// Forward, forces and (optionally) gradient of the nf frames frames[0..nf)
+// (device array); the summed squared energy and force errors go to
+// out[0..1] (device).  Nothing waits for the device.
+static int run_batch(md_trainer* c, const int32_t* frames, int nf, bool grad, double* out) {
     TrainState& t = c->t;
     const int64_t M = (int64_t)nf * t.n_atoms;
     int rc;
     if ((rc = ensure_batch(c, M))) return rc;
-    TCHK(c, tr_forward_backward(t, (int64_t)f0 * t.n_atoms, M, c->st));
-    TCHK(c, tr_forces(t, f0, nf, c->loss_part, c->st));
-    if (grad) TCHK(c, tr_gradient(t, f0, nf, M, c->st));
-    const int64_t blocks = (int64_t)((t.n_atoms + 255) / 256) * nf;
-    std::vector<double> lp((size_t)(2 * blocks));
-    TCHK(c, hipMemcpyAsync(lp.data(), c->loss_part, sizeof(double) * lp.size(), hipMemcpyDeviceToHost, c->st));
+    t.bf = frames;
+    TCHK(c, tr_forward_backward(t, nf, c->st));
+    TCHK(c, tr_forces(t, nf, c->loss_part, c->st));
+    TCHK(c, tr_loss_sum(t, nf, c->loss_part, out, c->st));
+    if (grad) TCHK(c, tr_gradient(t, nf, c->st));
+    return MD_OK;
+}
+
+// per-minibatch loss sums of the last run, copied to the host
+static int read_batch_loss(md_trainer* c, int nb, std::vector<double>& h) {
+    h.assign((size_t)2 * nb, 0.0);
+    TCHK(c, hipMemcpyAsync(h.data(), c->batch_loss, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->st));
     TCHK(c, hipStreamSynchronize(c->st));
-    double a = 0.0, b = 0.0;
-    for (int64_t q = 0; q < blocks; ++q) {
-        a += lp[(size_t)(2 * q)];
-        b += lp[(size_t)(2 * q + 1)];
-    }
-    *se = a;
-    *sf = b;
     return MD_OK;
 }
 
@@ -156,7 +159,10 @@ int md_trainer_create(md_trainer** out, int n_frames, int n_atoms, const double*
         (rc = talloc_t(c, &de, na)) || (rc = talloc_t(c, &df, na * 3)) || (rc = talloc_t(c, &t.G, na * kMlpK)) ||
         (rc = talloc_t(c, &t.theta, kMlpNP)) || (rc = talloc_t(c, &t.grad, kMlpNP)) ||
         (rc = talloc_t(c, &t.adam_m, kMlpNP)) || (rc = talloc_t(c, &t.adam_v, kMlpNP)) ||
-        (rc = talloc_t(c, &t.gpart, (size_t)std::max(t.splits, t.colsum_blocks) * kMlpH * kMlpH))) {
+        (rc = talloc_t(c, &t.gpart, (size_t)std::max(t.splits, t.colsum_blocks) * kMlpH * kMlpH)) ||
+        (rc = talloc_t(c, &t.ncnt, na)) || (rc = talloc_t(c, &c->frames_seq, (size_t)n_frames)) ||
+        (rc = talloc_t(c, &c->frames_ord, (size_t)n_frames)) ||
+        (rc = talloc_t(c, &c->batch_loss, (size_t)2 * n_frames + 2))) {
         g_train_error = c->err;
         md_trainer_destroy(c);
         return rc;
@@ -171,6 +177,32 @@ int md_trainer_create(md_trainer** out, int n_frames, int n_atoms, const double*
     if (e == hipSuccess) e = hipMemcpy(df, hf.data(), sizeof(float) * na * 3, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(t.adam_m, 0, sizeof(float) * kMlpNP);
     if (e == hipSuccess) e = hipMemset(t.adam_v, 0, sizeof(float) * kMlpNP);
+    std::vector<int32_t> seq((size_t)n_frames);
+    for (int f = 0; f < n_frames; ++f) seq[(size_t)f] = f;
+    if (e == hipSuccess) e = hipMemcpy(c->frames_seq, seq.data(), sizeof(int32_t) * n_frames, hipMemcpyHostToDevice);
+    // pair lists of every frame: counts and their maximum, then the indices
+    int32_t* dmax = nullptr;
+    int32_t cap = 0;
+    if ((rc = talloc_t(c, &dmax, 1))) {
+        g_train_error = c->err;
+        md_trainer_destroy(c);
+        return rc;
+    }
+    if (e == hipSuccess) e = hipMemset(dmax, 0, sizeof(int32_t));
+    for (int f0 = 0; f0 < n_frames && e == hipSuccess; f0 += 64)
+        e = tr_count_pairs(t, f0, std::min(64, n_frames - f0), dmax, c->st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&cap, dmax, sizeof(int32_t), hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    if (e == hipSuccess) {
+        t.cap = std::max(cap, 1);
+        if ((rc = talloc_t(c, &t.nbr, na * (size_t)t.cap))) {
+            g_train_error = c->err;
+            md_trainer_destroy(c);
+            return rc;
+        }
+    }
+    for (int f0 = 0; f0 < n_frames && e == hipSuccess; f0 += 64)
+        e = tr_fill_pairs(t, f0, std::min(64, n_frames - f0), c->st);
     // features of every frame (they do not depend on the parameters)
     for (int f0 = 0; f0 < n_frames && e == hipSuccess; f0 += 64) e = tr_features(t, f0, std::min(64, n_frames - f0), c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
@@ -245,14 +277,19 @@ int md_trainer_loss(md_trainer* c, double w_e, double w_f, int batch_frames, dou
     t.w_f = (float)w_f;
     t.n_total = (int64_t)t.n_frames * t.n_atoms;
     if (grad) TCHK(c, hipMemsetAsync(t.grad, 0, sizeof(float) * kMlpNP, c->st));
-    double se = 0, sf = 0;
-    for (int f0 = 0; f0 < t.n_frames; f0 += batch_frames) {
+    int nb = 0;
+    for (int f0 = 0; f0 < t.n_frames; f0 += batch_frames, ++nb) {
         const int nf = std::min(batch_frames, t.n_frames - f0);
-        double a, b;
-        int rc = run_batch(c, f0, nf, grad != nullptr, &a, &b);
+        int rc = run_batch(c, c->frames_seq + f0, nf, grad != nullptr, c->batch_loss + 2 * nb);
         if (rc) return rc;
-        se += a;
-        sf += b;
+    }
+    std::vector<double> bl;
+    int rc = read_batch_loss(c, nb, bl);
+    if (rc) return rc;
+    double se = 0, sf = 0;
+    for (int q = 0; q < nb; ++q) {
+        se += bl[(size_t)2 * q];
+        sf += bl[(size_t)2 * q + 1];
     }
     const double n = (double)t.n_total;
     out[1] = se / n;
@@ -274,35 +311,35 @@ int md_trainer_epoch(md_trainer* c, int batch_frames, double lr, double w_e, dou
     TrainState& t = c->t;
     t.w_e = (float)w_e;
     t.w_f = (float)w_f;
-    // frames are taken in blocks of `batch_frames` consecutive entries of `order`;
-    // a minibatch of scattered frames is processed frame range by frame range
-    double acc[3] = {0, 0, 0};
+    // minibatch k holds the frames order[k batch_frames ...] (or sequential)
+    if (order) {
+        for (int q = 0; q < t.n_frames; ++q)
+            if (order[q] < 0 || order[q] >= t.n_frames) return tfail(c, MD_ERR_ARG, "order holds a bad frame index");
+        TCHK(c, hipMemcpyAsync(c->frames_ord, order, sizeof(int32_t) * t.n_frames, hipMemcpyHostToDevice, c->st));
+    }
+    const int32_t* frames = order ? c->frames_ord : c->frames_seq;
+    std::vector<int64_t> rows;
     int nb = 0;
-    for (int b0 = 0; b0 < t.n_frames; b0 += batch_frames) {
+    for (int b0 = 0; b0 < t.n_frames; b0 += batch_frames, ++nb) {
         const int nf = std::min(batch_frames, t.n_frames - b0);
         t.n_total = (int64_t)nf * t.n_atoms;
+        rows.push_back(t.n_total);
         TCHK(c, hipMemsetAsync(t.grad, 0, sizeof(float) * kMlpNP, c->st));
-        double se = 0, sf = 0;
-        for (int q = 0; q < nf;) {
-            // longest run of consecutive frame indices in the order
-            const int f0 = order ? order[b0 + q] : b0 + q;
-            int len = 1;
-            while (q + len < nf && (order ? order[b0 + q + len] : b0 + q + len) == f0 + len) ++len;
-            double a, b;
-            int rc = run_batch(c, f0, len, true, &a, &b);
-            if (rc) return rc;
-            se += a;
-            sf += b;
-            q += len;
-        }
+        int rc = run_batch(c, frames + b0, nf, true, c->batch_loss + 2 * nb);
+        if (rc) return rc;
         TCHK(c, tr_adam(t, (float)lr, ++c->step, c->st));
-        const double n = (double)t.n_total;
+    }
+    std::vector<double> bl;
+    int rc = read_batch_loss(c, nb, bl);
+    if (rc) return rc;
+    double acc[3] = {0, 0, 0};
+    for (int q = 0; q < nb; ++q) {
+        const double n = (double)rows[(size_t)q];
+        const double se = bl[(size_t)2 * q], sf = bl[(size_t)2 * q + 1];
         acc[1] += se / n;
         acc[2] += sf / (3.0 * n);
         acc[0] += w_e * se / n + w_f * sf / (3.0 * n);
-        ++nb;
     }
-    TCHK(c, hipStreamSynchronize(c->st));
     for (int k = 0; k < 3; ++k) out[k] = acc[k] / std::max(nb, 1);
     return MD_OK;
 }
@@ -313,8 +350,7 @@ int md_trainer_predict(md_trainer* c, int frame, double* e_out, double* f_out) {
     if (!c || frame < 0 || frame >= c->t.n_frames) return tfail(c, MD_ERR_ARG, "bad argument");
     TrainState& t = c->t;
     t.n_total = t.n_atoms;
-    double a, b;
-    int rc = run_batch(c, frame, 1, false, &a, &b);
+    int rc = run_batch(c, c->frames_seq + frame, 1, false, c->batch_loss);
     if (rc) return rc;
     const int n = t.n_atoms;
     std::vector<float> he((size_t)n), hf((size_t)n * 3);
