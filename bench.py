@@ -142,6 +142,56 @@ def replicas_256(M, steps=2000, seq_sample=48):
             "speedup": total * steps / wall_b / seq_rate}
 
 
+def frame_io(M, pos, vel, box, frames=6, steps_between=20):
+    """State files of the 1M-atom state (data/<n>.csv + bb.csv in the
+    reference's exact text): GPU text kernels per frame, the writer's
+    end-to-end rate with saves back to back, and the cost of saving every
+    `steps_between` steps next to stepping alone."""
+    import shutil
+    import tempfile
+    from moldyn_amd.frames import FrameWriter
+    n = len(pos)
+    tmp = tempfile.mkdtemp(prefix="md_frames_")
+    try:
+        e = M.Engine([n], [MASS], box, force_mode="fp32_list")
+        e.upload(pos, vel, box)
+        e.update_force()
+        th, ba = controls(M)
+        e.step(DT, 2, th, ba)
+        with FrameWriter(e, tmp) as w:
+            w.save(0)
+            w.flush()
+            t0 = time.perf_counter()
+            for k in range(1, frames + 1):
+                w.save(k)
+            w.flush()
+            wall_b2b = time.perf_counter() - t0
+            st = w.stats()
+        size = os.path.getsize(os.path.join(tmp, "data", "1.csv"))
+        # stepping alone, then stepping with a save every `steps_between` steps
+        ts = time.perf_counter()
+        for _ in range(frames):
+            e.step(DT, steps_between, th, ba)
+        plain = time.perf_counter() - ts
+        with FrameWriter(e, tmp) as w:
+            ts = time.perf_counter()
+            for k in range(frames):
+                w.save(100 + k)
+                e.step(DT, steps_between, th, ba)
+            w.flush()
+            with_io = time.perf_counter() - ts
+        e.close()
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    gpu_ms = st["gpu_ms"] / st["frames"]
+    return {"workload": f"{n}-atom state as data/<n>.csv + bb.csv (reference text), written to local disk",
+            "frame_bytes": size, "frames": frames,
+            "gpu_text_ms_per_frame": gpu_ms, "gpu_text_GBps": size / gpu_ms / 1e6,
+            "copy_ms_per_frame": st["copy_ms"] / st["frames"], "write_ms_per_frame": st["write_ms"] / st["frames"],
+            "back_to_back_frames_per_s": frames / wall_b2b, "back_to_back_MBps": frames * size / wall_b2b / 1e6,
+            f"steps_{steps_between}_plain_s": plain, f"steps_{steps_between}_with_save_s": with_io}
+
+
 def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log):
     """Surrogate: 4,096-atom LJ argon frames from thermostatted SC runs at
     lattice edges 0.38-0.7 nm and 80-400 K, labelled by the FP64 LJ path,
@@ -372,6 +422,9 @@ def main():
         log("[bench] 256 replicas")
         extras["replicas_256"] = replicas_256(M)
         log(f"[bench] replicas_256: {extras['replicas_256']}")
+        log("[bench] frame files")
+        extras["frame_io_1M"] = frame_io(M, pos, vel, box)
+        log(f"[bench] frame_io_1M: {extras['frame_io_1M']}")
         log("[bench] surrogate")
         extras["mlp_surrogate"] = mlp_surrogate(M, n_frames=args.mlp_frames, epochs=args.mlp_epochs)
         log(f"[bench] mlp_surrogate: {extras['mlp_surrogate']}")

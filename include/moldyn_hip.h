@@ -178,6 +178,23 @@ int md_init_lattice(int kind, const double start[3], const int64_t grid[3], doub
  * uses SplitMix64 + Box-Muller so inputs are reproducible). */
 int md_init_mb_velocities(double* vel, int64_t n, double temperature, double mass, uint64_t seed);
 
+/* The same initialisers on the device, into particle group (replica,
+ * type_id) of a context (a context that never held a state first gets
+ * initialize_particles' blank one: positions and velocities 0).
+ * md_init_positions: UnitCell::initialize_particles_position for
+ * start + (x, y, z) a over grid[0] x grid[1] x grid[2] cells
+ * (position.rs:24-104), with the checks of position.rs:163-171 and :35, :53
+ * (MD_ERR_ARG "OutOfBoundary" when grid[k] a exceeds the box, "TooBig" when
+ * the sites outnumber the group); particles past the sites keep their place.
+ * md_init_velocities: initialize_velocities_maxwell_boltzmann
+ * (velocity.rs:6-29) with the draws of md_init_mb_velocities, one thread per
+ * particle pair (equal to it up to the last bits of log, sin and cos).
+ * Either resets the controls as md_upload_state does; call md_update_force
+ * before stepping, as cli/src/commands.rs:102 does. */
+int md_init_positions(md_ctx* ctx, int replica, uint16_t type_id, int kind, const double start[3],
+                      const int64_t grid[3], double a);
+int md_init_velocities(md_ctx* ctx, int replica, uint16_t type_id, double temperature, uint64_t seed);
+
 /* Learned per-atom energy surrogate (no reference counterpart: the reference
  * has Lennard-Jones only).  With force_mode MD_FORCE_MLP, md_update_force and
  * md_step use E_i = e_scale * MLP(x_i) + e_shift with
@@ -261,6 +278,57 @@ int md_get_stats(md_ctx* ctx, md_stats* out);
 int md_set_profiling(md_ctx* ctx, int on);
 int md_kernel_time(md_ctx* ctx, int kernel, double* total_ms, int64_t* launches, double* span_ms,
                    int64_t* span_launches);
+
+/* ---------------------------------------------------------------- state files
+ * The reference's on-disk state (core/src/save_data.rs): per state n,
+ * data/<n>.csv with the header
+ *   id,position_x,position_y,position_z,velocity_x,velocity_y,velocity_z
+ * and one row per particle (types in order, id = type), and row n of bb.csv
+ * ("x,y,z" header) holding the box.  Every f64 is written as the reference's
+ * csv 1.3 crate writes it (the ryu crate's shortest round-trip text), so the
+ * files are byte-for-byte the reference's.  Files are created or truncated
+ * (the reference opens without truncation, which leaves stale bytes behind
+ * a shorter rewrite; on new files the output is the same). */
+
+/* Asynchronous writer of one replica of a context: md_frame_writer_save(w, n)
+ * replaces StateToSave::from(&state).save_to_file(path, n)
+ * (save_data.rs:192-213, called at cli/src/commands.rs:183-184,189-190).
+ * The text is made on the GPU on the context's stream (so it holds the state
+ * of that moment); copies and file writes run on a thread of the writer while
+ * the caller goes on, at most two states in flight (save waits for a free
+ * slot).  md_frame_writer_flush waits for every write and returns the first
+ * error.  The context must outlive the writer.  stats: frames, text bytes,
+ * GPU ms, copy ms, file-write ms, ms that save waited for a slot. */
+typedef struct md_frame_writer md_frame_writer;
+int md_frame_writer_create(md_frame_writer** out, md_ctx* ctx, const char* dir, int replica);
+int md_frame_writer_save(md_frame_writer* w, int64_t state_number);
+int md_frame_writer_flush(md_frame_writer* w);
+int md_frame_writer_stats(md_frame_writer* w, double out[6]);
+const char* md_frame_writer_last_error(md_frame_writer* w);
+void md_frame_writer_destroy(md_frame_writer* w);
+
+/* StateToSave::load_from_file(path, n) (save_data.rs:215-229): rows of
+ * data/<n>.csv in file order (columns found by header name) and row n of
+ * bb.csv into box (nullable).  With ids, x and v NULL only *n_rows is set;
+ * otherwise cap must hold *n_rows.  Errors: md_global_error(). */
+int md_frame_load(const char* dir, int64_t state_number, int64_t cap, int64_t* n_rows, uint16_t* ids, double* x,
+                  double* v, double* box);
+
+/* GPU formatter of numeric CSV tables in the same text: rows of an optional
+ * u64 key column and ncol (1..10) f64 columns, "," between fields and "\n"
+ * after each row.  md_text_table writes the rows (no header) to out (NULL:
+ * only *len is set); md_csv_write_table writes header, "\n" and the rows to
+ * path, or an empty file for no rows, as the csv crate does.  Used for
+ * bb.csv, macro.csv (DataFileMacro::save_to_file, save_data.rs:291-300) and
+ * the numbers of db.csv and potentials.json. */
+typedef struct md_text md_text;
+int md_text_create(md_text** out);
+void md_text_destroy(md_text* h);
+const char* md_text_last_error(const md_text* h);
+int md_text_table(md_text* h, int64_t n_rows, int ncol, const uint64_t* keys, const double* vals, char* out,
+                  int64_t cap, int64_t* len);
+int md_csv_write_table(md_text* h, const char* path, const char* header, int64_t n_rows, int ncol,
+                       const uint64_t* keys, const double* vals);
 
 #ifdef __cplusplus
 }

@@ -37,7 +37,10 @@ EXPORTS = (
     "md_set_profiling", "md_kernel_time", "md_mlp_set_params", "md_mlp_get_stats",
     "md_trainer_create", "md_trainer_destroy", "md_trainer_last_error", "md_trainer_feature_stats",
     "md_trainer_set_params", "md_trainer_get_params", "md_trainer_loss", "md_trainer_epoch",
-    "md_trainer_predict",
+    "md_trainer_predict", "md_frame_writer_create", "md_frame_writer_save", "md_frame_writer_flush",
+    "md_frame_writer_stats", "md_frame_writer_last_error", "md_frame_writer_destroy", "md_frame_load",
+    "md_text_create", "md_text_destroy", "md_text_last_error", "md_text_table", "md_csv_write_table",
+    "md_init_positions", "md_init_velocities",
 )
 
 
@@ -135,5 +138,24 @@ def load():
     L.md_trainer_loss.argtypes = [vp, C.c_double, C.c_double, C.c_int, dp, fp]
     L.md_trainer_epoch.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.POINTER(C.c_int32), dp]
     L.md_trainer_predict.argtypes = [vp, C.c_int, dp, dp]
+    u64p = C.POINTER(C.c_uint64)
+    L.md_init_positions.argtypes = [vp, C.c_int, C.c_uint16, C.c_int, dp, i64p, C.c_double]
+    L.md_init_velocities.argtypes = [vp, C.c_int, C.c_uint16, C.c_double, C.c_uint64]
+    L.md_frame_writer_create.argtypes = [C.POINTER(vp), vp, C.c_char_p, C.c_int]
+    L.md_frame_writer_save.argtypes = [vp, C.c_int64]
+    L.md_frame_writer_flush.argtypes = [vp]
+    L.md_frame_writer_stats.argtypes = [vp, dp]
+    L.md_frame_writer_last_error.argtypes = [vp]
+    L.md_frame_writer_last_error.restype = C.c_char_p
+    L.md_frame_writer_destroy.argtypes = [vp]
+    L.md_frame_writer_destroy.restype = None
+    L.md_frame_load.argtypes = [C.c_char_p, C.c_int64, C.c_int64, i64p, C.POINTER(C.c_uint16), dp, dp, dp]
+    L.md_text_create.argtypes = [C.POINTER(vp)]
+    L.md_text_destroy.argtypes = [vp]
+    L.md_text_destroy.restype = None
+    L.md_text_last_error.argtypes = [vp]
+    L.md_text_last_error.restype = C.c_char_p
+    L.md_text_table.argtypes = [vp, C.c_int64, C.c_int, u64p, dp, C.c_char_p, C.c_int64, i64p]
+    L.md_csv_write_table.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_int64, C.c_int, u64p, dp]
     _lib = L
     return L

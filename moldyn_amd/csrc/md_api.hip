@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "md_device.h"
+#include "md_frames.h"
 #include "moldyn_hip.h"
 
 using namespace md;
@@ -911,6 +912,34 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
 
 // ------------------------------------------------------------------ C ABI
 
+// ------------------------------------------------------------------ frame source
+
+void md_set_global_error(const std::string& msg) { g_error = msg; }
+
+int md_frame_source(md_ctx* c, int replica, md::FrameSource* out) {
+    if (!c || !out) return MD_ERR_ARG;
+    if (replica < 0 || replica >= c->R) return fail(c, MD_ERR_ARG, "bad replica");
+    md::FrameSource& s = *out;
+    s = md::FrameSource{};
+    s.st = c->st;
+    s.device = c->device;
+    s.rec[0] = (const unsigned char*)c->d.rec[0];
+    s.rec[1] = (const unsigned char*)c->d.rec[1];
+    s.stride = c->fp32 ? (int32_t)sizeof(Rec<float>) : (int32_t)sizeof(Rec<double>);
+    s.ctrl = ctrl_cur(c) + replica;
+    int64_t row = 0;
+    for (const GroupInfo& G : c->groups) {
+        if (G.rep != replica) continue;
+        s.row0[s.n_groups] = row;
+        s.slot0[s.n_groups] = G.start;
+        s.type[s.n_groups] = G.type;
+        row += G.count;
+        ++s.n_groups;
+    }
+    s.row0[s.n_groups] = row;
+    return MD_OK;
+}
+
 extern "C" {
 
 void md_lj_potential_and_force(double sigma, double eps, double r_cut, double u_cut, double r,
@@ -1028,6 +1057,9 @@ int md_get_options(const md_ctx* c, md_options* out) {
 
 }  // extern "C"
 
+// slots of a group: its tiles of bs records (a block may hold several tiles)
+static int64_t group_slots(const md_ctx* c, const GroupInfo& G) { return (G.count + c->bs - 1) / c->bs * c->bs; }
+
 // Host <-> device record transfer.  Host arrays are N x 3 (x, y, z per
 // particle) in the caller's order; slots add NaN-positioned padding.
 template <typename FT>
@@ -1038,7 +1070,7 @@ static int put_records(md_ctx* c, int buf, const double* x, const double* v, con
     if (keep_forces)
         HIPCHK(c, hipMemcpy(h.data(), c->d.rec[buf], sizeof(Rec<FT>) * N, hipMemcpyDeviceToHost));
     for (const GroupInfo& G : c->groups)
-        for (int64_t i = 0; i < (int64_t)G.nblocks * c->bs; ++i) {
+        for (int64_t i = 0; i < group_slots(c, G); ++i) {
             Rec<FT>& r = h[(size_t)(G.start + i)];
             r.type = (uint32_t)G.type;
             if (i >= G.count && !keep_forces) {
@@ -1129,26 +1161,9 @@ static int get_pv(md_ctx* c, double* pot, double* vir) {
     return MD_OK;
 }
 
-extern "C" {
-
-int md_upload_state(md_ctx* c, const double* x, const double* v, const double* box) {
-    if (!c || !x || !v || !box) return fail(c, MD_ERR_ARG, "NULL argument");
-    int rc;
-    for (int b = 0; b < 2; ++b) {
-        rc = c->fp32 ? put_records<float>(c, b, x, v, nullptr, false)
-                     : put_records<double>(c, b, x, v, nullptr, false);
-        if (rc) return rc;
-    }
-    if ((rc = put_pv(c, nullptr, nullptr))) return rc;
-    // the grid buffers keep the headers of earlier fills: continue the epoch
-    // count past every epoch used so far so that none of them can match again
-    uint32_t epoch0 = 0;
-    if (c->ctrl_valid) {
-        std::vector<RepCtrl> old;
-        if ((rc = read_ctrl(c, old))) return rc;
-        for (const RepCtrl& X : old) epoch0 = std::max(epoch0, std::max(X.fill_epoch, X.ins_epoch));
-        epoch0 += 2u;
-    }
+// Controls as after an upload of a new state with these boxes (R x 3):
+// unit lambda / mu, no lists, input in buffer 0, grid epochs from epoch0.
+static int reset_controls(md_ctx* c, const double* box, uint32_t epoch0) {
     std::vector<RepCtrl> C((size_t)c->R);
     for (int r = 0; r < c->R; ++r) {
         RepCtrl X{};
@@ -1163,13 +1178,43 @@ int md_upload_state(md_ctx* c, const double* x, const double* v, const double* b
         C[r] = X;
         for (int k = 0; k < 3; ++k) c->box0[(size_t)r * 3 + k] = box[3 * r + k];
     }
-    rc = write_ctrl(c, C);
+    int rc = write_ctrl(c, C);
     if (rc) return rc;
     c->ctrl_valid = true;
     c->sums_valid = false;
     c->stats.rebuilds = 0;
     c->stats.refresh_steps = 0;
     return MD_OK;
+}
+
+// the first grid epoch a reset may use: past every epoch used so far
+static int next_epoch(md_ctx* c, uint32_t* epoch0) {
+    *epoch0 = 0;
+    if (!c->ctrl_valid) return MD_OK;
+    std::vector<RepCtrl> old;
+    int rc = read_ctrl(c, old);
+    if (rc) return rc;
+    for (const RepCtrl& X : old) *epoch0 = std::max(*epoch0, std::max(X.fill_epoch, X.ins_epoch));
+    *epoch0 += 2u;
+    return MD_OK;
+}
+
+extern "C" {
+
+int md_upload_state(md_ctx* c, const double* x, const double* v, const double* box) {
+    if (!c || !x || !v || !box) return fail(c, MD_ERR_ARG, "NULL argument");
+    int rc;
+    for (int b = 0; b < 2; ++b) {
+        rc = c->fp32 ? put_records<float>(c, b, x, v, nullptr, false)
+                     : put_records<double>(c, b, x, v, nullptr, false);
+        if (rc) return rc;
+    }
+    if ((rc = put_pv(c, nullptr, nullptr))) return rc;
+    // the grid buffers keep the headers of earlier fills: continue the epoch
+    // count past every epoch used so far so that none of them can match again
+    uint32_t epoch0 = 0;
+    if ((rc = next_epoch(c, &epoch0))) return rc;
+    return reset_controls(c, box, epoch0);
 }
 
 int md_upload_forces(md_ctx* c, const double* force, const double* pot, const double* vir) {
@@ -1522,6 +1567,86 @@ int md_init_mb_velocities(double* vel, int64_t n, double temperature, double mas
         vel[3 * (i + h) + 2] = -vz;
     }
     return MD_OK;
+}
+
+// ------------------------------------------------------------------ device initialisers
+
+// The records to edit on the device: the current buffer and boxes; a context
+// that never held a state gets the blank one first (initialize_particles).
+static int edit_begin(md_ctx* c, int* cur, std::vector<double>& boxes) {
+    boxes.assign((size_t)c->R * 3, 0.0);
+    if (!c->ctrl_valid) {
+        HIPCHK(c, launch_init_blank(c->d, c->fp32, c->d.groups, (int)c->groups.size(), c->n_slots, c->st));
+        HIPCHK(c, hipMemsetAsync(c->d.pot, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_slots, 1), c->st));
+        HIPCHK(c, hipMemsetAsync(c->d.vir, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_slots, 1), c->st));
+        for (size_t k = 0; k < boxes.size(); ++k) boxes[k] = c->box0[k];
+        int rc = reset_controls(c, boxes.data(), 0);
+        if (rc) return rc;
+        *cur = 0;
+        return MD_OK;
+    }
+    std::vector<RepCtrl> C;
+    int rc = read_ctrl(c, C);
+    if (rc) return rc;
+    *cur = C[0].cur;
+    for (int r = 0; r < c->R; ++r)
+        for (int k = 0; k < 3; ++k) boxes[(size_t)r * 3 + k] = C[(size_t)r].box[k];
+    return MD_OK;
+}
+
+// After an edit of buffer `cur`: both buffers hold the edited state, per-atom
+// potentials and virials cleared, controls reset as by md_upload_state.
+static int edit_commit(md_ctx* c, int cur, const std::vector<double>& boxes) {
+    const size_t bytes = (size_t)std::max<int64_t>(c->n_slots, 1) * (c->fp32 ? sizeof(Rec<float>) : sizeof(Rec<double>));
+    if (cur != 0) HIPCHK(c, hipMemcpyAsync(c->d.rec[0], c->d.rec[cur], bytes, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->d.rec[1], c->d.rec[0], bytes, hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(c, hipMemsetAsync(c->d.pot, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_slots, 1), c->st));
+    HIPCHK(c, hipMemsetAsync(c->d.vir, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_slots, 1), c->st));
+    uint32_t epoch0 = 0;
+    int rc = next_epoch(c, &epoch0);
+    if (rc) return rc;
+    return reset_controls(c, boxes.data(), epoch0);
+}
+
+static const GroupInfo* find_group(md_ctx* c, int replica, int type_id) {
+    for (const GroupInfo& G : c->groups)
+        if (G.rep == replica && G.type == type_id) return &G;
+    return nullptr;
+}
+
+int md_init_positions(md_ctx* c, int replica, uint16_t type_id, int kind, const double start[3],
+                      const int64_t grid[3], double a) {
+    if (!c || !start || !grid || (kind != 0 && kind != 1)) return fail(c, MD_ERR_ARG, "bad argument");
+    const GroupInfo* G = find_group(c, replica, (int)type_id);
+    if (!G) return fail(c, MD_ERR_ARG, "InitError::ParticleIdDidNotFound: no such replica or type");
+    if (grid[0] < 0 || grid[1] < 0 || grid[2] < 0) return fail(c, MD_ERR_ARG, "negative grid size");
+    int cur = 0;
+    std::vector<double> boxes;
+    int rc = edit_begin(c, &cur, boxes);
+    if (rc) return rc;
+    const double* box = &boxes[(size_t)replica * 3];
+    // position.rs:166-171, then :35-36 / :53-54
+    if ((double)grid[0] * a > box[0] || (double)grid[1] * a > box[1] || (double)grid[2] * a > box[2])
+        return fail(c, MD_ERR_ARG, "InitError::OutOfBoundary: the lattice is larger than the box");
+    const int64_t sites = grid[0] * grid[1] * grid[2] * (kind == 1 ? 4 : 1);
+    if (sites > G->count) return fail(c, MD_ERR_ARG, "InitError::TooBig: more lattice sites than particles");
+    HIPCHK(c, launch_init_lattice(c->d.rec[cur], c->fp32, G->start, sites, kind == 1, start, grid, a, c->st));
+    return edit_commit(c, cur, boxes);
+}
+
+int md_init_velocities(md_ctx* c, int replica, uint16_t type_id, double temperature, uint64_t seed) {
+    if (!c) return MD_ERR_ARG;
+    const GroupInfo* G = find_group(c, replica, (int)type_id);
+    if (!G) return fail(c, MD_ERR_ARG, "no such replica or type");
+    int cur = 0;
+    std::vector<double> boxes;
+    int rc = edit_begin(c, &cur, boxes);
+    if (rc) return rc;
+    // velocity.rs:8-10
+    const double tp = temperature * 0.01;
+    const double sigma = sqrt(MD_K_B * tp / G->mass);
+    HIPCHK(c, launch_init_mb(c->d.rec[cur], c->fp32, G->start, G->count, sigma, seed, c->st));
+    return edit_commit(c, cur, boxes);
 }
 
 int md_get_stats(md_ctx* c, md_stats* out) {

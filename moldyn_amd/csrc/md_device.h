@@ -359,4 +359,12 @@ hipError_t launch_mlp_net(const DevState& d, hipStream_t st);
 hipError_t launch_mlp_force(const DevState& d, int bs, bool force_only, hipStream_t st);
 hipError_t launch_mlp_convert(const DevState& d, hipStream_t st);
 
+// initialisers on the device (md_init.hip)
+hipError_t launch_init_blank(const DevState& d, bool fp32, const GroupInfo* groups, int n_groups, int64_t n_slots,
+                             hipStream_t st);
+hipError_t launch_init_lattice(void* rec, bool fp32, int64_t slot0, int64_t sites, bool fcc, const double start[3],
+                               const int64_t grid[3], double a, hipStream_t st);
+hipError_t launch_init_mb(void* rec, bool fp32, int64_t slot0, int64_t n, double sigma, uint64_t seed,
+                          hipStream_t st);
+
 }  // namespace md

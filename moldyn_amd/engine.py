@@ -110,6 +110,18 @@ class Engine:
         b = _f64(box, (self.n_replicas, 3))
         self._chk(self._lib.md_upload_state(self._h, _dp(x), _dp(v), _dp(b)))
 
+    def init_lattice(self, kind, grid, a, start=(0.0, 0.0, 0.0), replica=0, type_id=0):
+        """UnitCell::initialize_particles_position on the device (md_init_positions)."""
+        st = _f64(start, (3,))
+        g = np.ascontiguousarray(grid, dtype=np.int64)
+        k = {"u": 0, "fcc": 1}[kind.lower()]
+        self._chk(self._lib.md_init_positions(self._h, int(replica), int(type_id), k, _dp(st),
+                                              g.ctypes.data_as(C.POINTER(C.c_int64)), float(a)))
+
+    def init_velocities(self, temperature, seed, replica=0, type_id=0):
+        """Maxwell-Boltzmann velocities on the device (md_init_velocities)."""
+        self._chk(self._lib.md_init_velocities(self._h, int(replica), int(type_id), float(temperature), int(seed)))
+
     def upload_forces(self, force=None, pot=None, virial=None):
         f = _f64(force, (self.n, 3)) if force is not None else None
         p = _f64(pot, (self.n,)) if pot is not None else None

@@ -320,6 +320,31 @@ def test_replicas_match_single(mode):
         one.close()
 
 
+@pytest.mark.parametrize("mode", ["fp64_list", "fp64_allpairs"])
+def test_multitype_with_several_tiles_per_block(mode):
+    """Two types in 64-slot tiles, three tiles per block: every slot of a
+    group carries its type (type 1 holds 106 atoms, two tiles, one block),
+    so the pair parameters and the t1 <= t2 rule see the right type."""
+    import moldyn_amd as M
+    rng = np.random.default_rng(5)
+    n0, n1 = 150, 106
+    box = 2.4
+    pos = rng.uniform(0, box, size=(n0 + n1, 3))
+    vel = rng.normal(0, 0.2, size=(n0 + n1, 3))
+    s = O.State([n0, n1], [AR_MASS, 39.948], [box] * 3, pos=pos, vel=vel)
+    tab = O.potential_table(2, {(0, 1): O.new_lennard_jones(0.30, 1.2), (1, 1): O.new_lennard_jones(0.36, 0.9)})
+    e = M.Engine([n0, n1], [AR_MASS, 39.948], [box] * 3, force_mode=mode, block_size=64, tiles_per_block=3)
+    e.set_pair_potential(0, 1, 0.30, 1.2)
+    e.set_pair_potential(1, 1, 0.36, 0.9)
+    e.upload(pos, vel, [box] * 3)
+    e.update_force()
+    d = e.download()
+    e.close()
+    O.update_force(s, tab)
+    assert np.array_equal(d["force"], s.force)
+    assert np.array_equal(d["pot"], s.pot) and np.array_equal(d["virial"], s.temp)
+
+
 @pytest.mark.parametrize("mode", ALL_MODES)
 def test_multitype_update_force_and_nve(mode):
     """Two types with distinct potentials: the reference's type-pair rule
