@@ -29,20 +29,46 @@ __device__ __forceinline__ double wrap1(double x, double b) {
     return x;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+// Barrier for LDS traffic only: every thread's LDS accesses before it are
+// complete and visible to the block after it.  Unlike __syncthreads it does
+// not wait for outstanding global loads and stores, which stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// DPP move of a double from the lane that CTRL selects; lanes without a
+// source lane, or in rows that ROW_MASK leaves out, get `fill`.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v, double fill) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v), f = __builtin_bit_cast(uint64_t, fill);
+    const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)f, (int)(uint32_t)b, CTRL, ROW_MASK, 0xf, false);
+    const int hi =
+        __builtin_amdgcn_update_dpp((int)(uint32_t)(f >> 32), (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+    return __builtin_bit_cast(double, (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32));
+}
+
+// Inclusive scan of a wave's 64 values in a fixed tree on the VALU: shifts by
+// 1, 2, 4 and 8 lanes within each row of 16, then row 0's last lane added to
+// row 1 and row 2's to row 3, then lane 31 added to rows 2 and 3.  Lane 63
+// ends with the sum (MAX: the maximum) of all 64 values.
+template <bool MAX>
+__device__ __forceinline__ double wave_total(double v) {
+    const double id = MAX ? -__builtin_inf() : 0.0;
+    auto op = [](double a, double b) { return MAX ? fmax(a, b) : a + b; };
+    v = op(v, dpp_f64<0x111, 0xf>(v, id));  // row_shr:1
+    v = op(v, dpp_f64<0x112, 0xf>(v, id));  // row_shr:2
+    v = op(v, dpp_f64<0x114, 0xf>(v, id));  // row_shr:4
+    v = op(v, dpp_f64<0x118, 0xf>(v, id));  // row_shr:8
+    v = op(v, dpp_f64<0x142, 0xa>(v, id));  // row_bcast:15 into rows 1, 3
+    v = op(v, dpp_f64<0x143, 0xc>(v, id));  // row_bcast:31 into rows 2, 3
     return v;
 }
 
-__device__ __forceinline__ double wave_max(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// Block reduction of NS columns in a fixed order (wave butterfly, then waves
-// in index order); the columns set in mask are stored to out[k * stride].
+// Block reduction of NS columns in a fixed order (wave scan tree, then waves
+// in index order); the columns set in mask are stored to out[k * stride]
+// after the closing barrier, so no thread waits for the stores.
 template <int BS>
 __device__ __forceinline__ void block_store(const double (&v)[NS], double* out, int64_t stride,
                                             uint32_t mask) {
@@ -52,18 +78,20 @@ __device__ __forceinline__ void block_store(const double (&v)[NS], double* out, 
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         if (!((mask >> k) & 1u)) continue;
-        const double t = k >= kFirstMax ? wave_max(v[k]) : wave_sum(v[k]);
-        if (lane == 0) red[w * NS + k] = t;
+        const double t = k >= kFirstMax ? wave_total<true>(v[k]) : wave_total<false>(v[k]);
+        if (lane == 63) red[w * NS + k] = t;
     }
-    __syncthreads();
+    lds_barrier();
     const int k = threadIdx.x;
-    if (k < NS && ((mask >> k) & 1u)) {
-        double acc = red[k];
+    const bool mine = k < NS && ((mask >> k) & 1u);
+    double acc = 0.0;
+    if (mine) {
+        acc = red[k];
         for (int i = 1; i < NW; ++i)
             acc = k >= kFirstMax ? fmax(acc, red[i * NS + k]) : acc + red[i * NS + k];
-        out[k * stride] = acc;
     }
-    __syncthreads();
+    lds_barrier();  // red may be reused by the next call
+    if (mine) out[k * stride] = acc;
 }
 
 // In-place exclusive scan of a[0..n) (n <= kBrickCells); a[n] receives the total.
@@ -84,7 +112,7 @@ __device__ void block_exclusive_scan(int* a, int n) {
         if (lane >= o) incl += y;
     }
     if (lane == 63) wsum[w] = incl;
-    __syncthreads();
+    lds_barrier();
     int before = 0;
     for (int i = 0; i < w; ++i) before += wsum[i];
     int run = before + incl - local;
@@ -94,7 +122,7 @@ __device__ void block_exclusive_scan(int* a, int n) {
         run += c;
     }
     if (t == BS - 1) a[n] = before + incl;
-    __syncthreads();
+    lds_barrier();
 }
 
 // ---------------------------------------------------------------- pair terms

@@ -151,7 +151,8 @@ __device__ void build_list_global(const DevState& d, const RepCtrl& C, const Rep
 // Lists of every particle of a replica, one brick of cells at a time: the
 // brick and a one-cell halo are staged in LDS (slot ids, positions, types),
 // then each particle of the brick scans its 27 neighbour cells there.
-// Bricks are shared out among the replica's blocks.
+// Bricks are shared out among the replica's blocks.  The barriers order LDS
+// only: a brick's list stores stay in flight while the next brick loads.
 template <typename FT, int BS, bool MULTI>
 __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams& P,
                             const RepInfo& R, int block_in_rep) {
@@ -213,7 +214,7 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
                 s_off[h] = cnt < cc ? cnt : cc;
             }
         }
-        __syncthreads();
+        lds_barrier();
         block_exclusive_scan<BS>(s_off, H);
         const int total = s_off[H];
         if (total > kBrickAtoms) {
@@ -229,14 +230,14 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
                 for (int k = 0; k < cnt; ++k)
                     build_list_global<FT, MULTI>(d, C, P, R, recs, cs[cell * cc + k]);
             }
-            __syncthreads();
+            lds_barrier();
             continue;
         }
         // staged particle -> its halo cell (LDS only), then one particle per
         // thread: slot id and position loads of all particles go out together
         for (int h = threadIdx.x; h < H; h += BS)
             for (int a = s_off[h]; a < s_off[h + 1]; ++a) s_hc[a] = (uint16_t)h;
-        __syncthreads();
+        lds_barrier();
         for (int a = threadIdx.x; a < total; a += BS) {
             const int h = s_hc[a];
             const int32_t j = cs[cell_of(h) * cc + (a - s_off[h])];
@@ -247,7 +248,7 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             s_id[a] = j;
             if (MULTI) s_ty[a] = (uint8_t)o.type;
         }
-        __syncthreads();
+        lds_barrier();
         for (int a = threadIdx.x; a < total; a += BS) {
             const int h = s_hc[a];
             const int hc[3] = {h / (hd[1] * hd[2]), (h / hd[2]) % hd[1], h % hd[2]};
@@ -292,7 +293,7 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             lst.store();
             list_close(d, i, lst.n);
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -394,7 +395,7 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
         const bool valid = t * BS + lane < G.count;
 #pragma unroll
         for (int q = 0; q < RQ; ++q) s_tile[q * BS + lane] = pre[q];
-        __syncthreads();
+        lds_barrier();  // stores of the previous tile may still be in flight
         const Rec<FT> a = ((const Rec<FT>*)s_tile)[lane];
         int nnn = 0;
         if (t + 1 < tr.t1) {
@@ -484,13 +485,13 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
         }
         visits += valid ? nn : 0;
         ((Rec<FT>*)s_tile)[lane] = res;
-        __syncthreads();
+        lds_barrier();
         {
             u4* g = (u4*)(dst + G.start + t * BS);
 #pragma unroll
             for (int q = 0; q < RQ; ++q) g[q * BS + lane] = s_tile[q * BS + lane];
         }
-        __syncthreads();
+        lds_barrier();  // s_tile is read out before the next tile overwrites it
         nn = nnn;
     }
     __shared__ int s_vis[BS / 64];
@@ -1036,12 +1037,14 @@ __device__ int64_t finalize_replica(const DevState& d, int r, int mode) {
         copy_words<BS>(ctrl_out(d, r), &s_ctrl, (int)sizeof(RepCtrl));
         return -1;
     }
-    if (finalize_reduces(mode, f0)) {
+    // the new group sums are stored at the end: a store before a barrier would
+    // make every thread wait for its completion
+    const bool reduced = finalize_reduces(mode, f0);
+    if (reduced) {
         if (single)
             block_store<BS>(acc, s_sums, 1, kColsAll);
         else
             for (int t = 0; t < T; ++t) reduce_rows<BS>(d, s_grp[t], 0, s_grp[t].nblocks, s_sums + t * NS);
-        for (int i = threadIdx.x; i < T * NS; i += BS) d.sums[g0 * NS + i] = s_sums[i];
     } else {
         for (int i = threadIdx.x; i < T * NS; i += BS) s_sums[i] = d.sums[g0 * NS + i];
     }
@@ -1073,7 +1076,10 @@ __device__ int64_t finalize_replica(const DevState& d, int r, int mode) {
             ms->oor = 0u;
         }
     }
-    if (mode == M_REDUCE) return -1;
+    if (mode == M_REDUCE) {
+        for (int i = threadIdx.x; i < T * NS; i += BS) d.sums[g0 * NS + i] = s_sums[i];
+        return -1;
+    }
     if (threadIdx.x == 0) {
         RepCtrl C = s_ctrl;
         const RepParams P = s_par;
@@ -1082,6 +1088,8 @@ __device__ int64_t finalize_replica(const DevState& d, int r, int mode) {
     }
     __syncthreads();
     copy_words<BS>(ctrl_out(d, r), &s_ctrl, (int)sizeof(RepCtrl));
+    if (reduced)
+        for (int i = threadIdx.x; i < T * NS; i += BS) d.sums[g0 * NS + i] = s_sums[i];
     return mode == M_STEP ? launch : -1;
 }
 
