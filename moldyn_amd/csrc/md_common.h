@@ -29,6 +29,42 @@ __device__ __forceinline__ double wrap1(double x, double b) {
     return x;
 }
 
+// The GPU deals blocks to its 8 XCDs in turn (block b to XCD b % 8), and each
+// XCD has its own L2.  xcd_block gives physical block b the logical block it
+// works on such that each XCD runs one contiguous range of logical blocks:
+// blocks of neighbouring particles then share an L2.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_block(int b, int n) {
+    constexpr int X = 8;
+    const int q = n / X, r = n % X;
+    const int x = b % X;
+    return x * q + (x < r ? x : r) + b / X;
+}
+
+// A value the whole wave holds alike, moved to scalar registers (it was
+// loaded through the vector path, the compiler cannot tell it is uniform).
+__device__ __forceinline__ double wave_uniform(double x) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float wave_uniform(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ PairParam wave_uniform(const PairParam& p) {
+    PairParam q;
+    q.sigma = wave_uniform(p.sigma);
+    q.eps = wave_uniform(p.eps);
+    q.r_cut = wave_uniform(p.r_cut);
+    q.u_cut = wave_uniform(p.u_cut);
+    q.r2max = wave_uniform(p.r2max);
+    q.sig2f = wave_uniform(p.sig2f);
+    q.eps24f = wave_uniform(p.eps24f);
+    q.eps4f = wave_uniform(p.eps4f);
+    q.ucutf = wave_uniform(p.ucutf);
+    return q;
+}
+
 // Barrier for LDS traffic only: every thread's LDS accesses before it are
 // complete and visible to the block after it.  Unlike __syncthreads it does
 // not wait for outstanding global loads and stores, which stay in flight.
@@ -128,65 +164,120 @@ __device__ void block_exclusive_scan(int* a, int n) {
 // ---------------------------------------------------------------- pair terms
 
 // potential.rs:201-211 with LJ of potential.rs:57-69, FP64, reference order.
-__device__ __forceinline__ void pair_exact(double dx, double dy, double dz, double r2,
-                                           const PairParam& p, double& fx, double& fy,
-                                           double& fz, double& pot, double& vir) {
+// potential.rs:57-75 and :206-211 for one pair: the terms added to the
+// force, potential and virial accumulators (in this order, by the caller)
+struct PairTerms64 {
+    double gx, gy, gz, u, t;
+};
+__device__ __forceinline__ PairTerms64 pair_exact_terms(double dx, double dy, double dz, double r2,
+                                                        const PairParam& p) {
     const double r = sqrt(r2);
     const double sr = p.sigma / r;
     const double s2 = sr * sr;
     const double s4 = s2 * s2;
     const double s6 = s2 * s4;
     const double s12 = s6 * s6;
-    const double u = 4.0 * p.eps * (s12 - s6) - p.u_cut;
+    PairTerms64 q;
+    q.u = 4.0 * p.eps * (s12 - s6) - p.u_cut;
     const double du = (24.0 * p.eps / r) * (s6 - 2.0 * s12);
-    const double gx = (dx / r) * du;
-    const double gy = (dy / r) * du;
-    const double gz = (dz / r) * du;
-    const double t = (gx * dx + gy * dy) + gz * dz;
-    fx += gx;
-    fy += gy;
-    fz += gz;
-    pot += u;
-    vir += t;
+    q.gx = (dx / r) * du;
+    q.gy = (dy / r) * du;
+    q.gz = (dz / r) * du;
+    q.t = (q.gx * dx + q.gy * dy) + q.gz * dz;
+    return q;
+}
+
+__device__ __forceinline__ void pair_exact(double dx, double dy, double dz, double r2,
+                                           const PairParam& p, double& fx, double& fy,
+                                           double& fz, double& pot, double& vir) {
+    const PairTerms64 q = pair_exact_terms(dx, dy, dz, r2, p);
+    fx += q.gx;
+    fy += q.gy;
+    fz += q.gz;
+    pot += q.u;
+    vir += q.t;
 }
 
 // Same physics in FP32 from the FP64 separation: F/r = 24 eps (s6 - 2 s12) / r^2,
-// virial F.r = (F/r) r^2, U = 4 eps (s12 - s6) - u_cut.
-__device__ __forceinline__ void pair_fp32(double dx, double dy, double dz, const PairParam& p,
-                                          float& fx, float& fy, float& fz, float& pot,
-                                          float& vir) {
-    const float ax = (float)dx, ay = (float)dy, az = (float)dz;
-    const float r2 = fmaf(ax, ax, fmaf(ay, ay, az * az));
-    const float ir2 = 1.0f / r2;
+// accumulated with fused multiply-adds: the per-pair values and the update.
+struct PairTerms32 {
+    float ax, ay, az, fr, r2, pu;
+};
+__device__ __forceinline__ PairTerms32 pair_fp32_terms(double dx, double dy, double dz, const PairParam& p) {
+    PairTerms32 q;
+    q.ax = (float)dx;
+    q.ay = (float)dy;
+    q.az = (float)dz;
+    q.r2 = fmaf(q.ax, q.ax, fmaf(q.ay, q.ay, q.az * q.az));
+    const float ir2 = 1.0f / q.r2;
     const float s2 = p.sig2f * ir2;
     const float s6 = s2 * s2 * s2;
     const float s12 = s6 * s6;
-    const float fr = p.eps24f * ir2 * fmaf(-2.0f, s12, s6);
-    fx = fmaf(ax, fr, fx);
-    fy = fmaf(ay, fr, fy);
-    fz = fmaf(az, fr, fz);
-    pot += fmaf(p.eps4f, s12 - s6, -p.ucutf);
-    vir = fmaf(fr, r2, vir);
+    q.fr = p.eps24f * ir2 * fmaf(-2.0f, s12, s6);
+    q.pu = fmaf(p.eps4f, s12 - s6, -p.ucutf);
+    return q;
+}
+
+__device__ __forceinline__ void pair_fp32_add(const PairTerms32& q, float& fx, float& fy, float& fz, float& pot,
+                                              float& vir) {
+    fx = fmaf(q.ax, q.fr, fx);
+    fy = fmaf(q.ay, q.fr, fy);
+    fz = fmaf(q.az, q.fr, fz);
+    pot += q.pu;
+    vir = fmaf(q.fr, q.r2, vir);
+}
+
+__device__ __forceinline__ void pair_fp32(double dx, double dy, double dz, const PairParam& p,
+                                          float& fx, float& fy, float& fz, float& pot,
+                                          float& vir) {
+    pair_fp32_add(pair_fp32_terms(dx, dy, dz, p), fx, fy, fz, pot, vir);
 }
 
 template <typename FT>
 struct Acc;
 template <>
 struct Acc<double> {
+    typedef PairTerms64 Terms;
     double fx = 0.0, fy = 0.0, fz = 0.0, pot = 0.0, vir = 0.0;
     __device__ __forceinline__ void add(double dx, double dy, double dz, double r2,
                                         const PairParam& p) {
         pair_exact(dx, dy, dz, r2, p, fx, fy, fz, pot, vir);
     }
+    static __device__ __forceinline__ Terms terms(double dx, double dy, double dz, double r2, const PairParam& p) {
+        return pair_exact_terms(dx, dy, dz, r2, p);
+    }
+    __device__ __forceinline__ void add_terms(const Terms& q) {
+        fx += q.gx;
+        fy += q.gy;
+        fz += q.gz;
+        pot += q.u;
+        vir += q.t;
+    }
 };
 template <>
 struct Acc<float> {
+    typedef PairTerms32 Terms;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f, pot = 0.0f, vir = 0.0f;
     __device__ __forceinline__ void add(double dx, double dy, double dz, double,
                                         const PairParam& p) {
         pair_fp32(dx, dy, dz, p, fx, fy, fz, pot, vir);
     }
+    static __device__ __forceinline__ Terms terms(double dx, double dy, double dz, double, const PairParam& p) {
+        return pair_fp32_terms(dx, dy, dz, p);
+    }
+    __device__ __forceinline__ void add_terms(const Terms& q) { pair_fp32_add(q, fx, fy, fz, pot, vir); }
 };
+
+// Inclusive scan of one int per lane over a wave (the DPP tree of wave_total).
+__device__ __forceinline__ int wave_scan_incl(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return v;
+}
 
 // ---------------------------------------------------------------- cell grid
 

@@ -299,11 +299,109 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
 
 // ---------------------------------------------------------------- fused step
 
+// LDS ordering within one wave: a compiler fence (a wave's LDS operations
+// execute in order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// The list forces of a wave's particles (potential.rs:158-216 over the Verlet
+// lists at the drifted positions).  With at most one entry per particle each
+// lane takes its own; otherwise the wave's entries are spread over its 64
+// lanes, one entry per lane and round, so the index and record loads
+// of a round go out together instead of one entry at a time per particle;
+// each lane computes its pair's terms into the wave's part of the LDS tile
+// (free once every lane holds its own record), then every particle adds its
+// own terms in list order: the order of a per-particle loop, so the sums are
+// bitwise those of one.  A neighbour's drift is recomputed from its input
+// record with the operations its own thread performs.
+template <typename FT, bool MULTI>
+__device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const RepParams& P,
+                                                    const Rec<FT>* __restrict__ src, char* wreg, int n_mine,
+                                                    int32_t e0, int64_t s, double x1, double y1, double z1, int ti, bool split,
+                                                    double c, double lam, double dt, double bx, double by,
+                                                    double bz, const PairParam& p0) {
+    typedef typename Acc<FT>::Terms Terms;
+    constexpr int W = 64;
+    static_assert(3 * W * sizeof(double) + W * sizeof(Terms) + 3 * W * sizeof(int32_t) + W <= W * sizeof(Rec<FT>),
+                  "wave scratch exceeds the wave's part of the tile");
+    double* ox = (double*)wreg;
+    double* oy = ox + W;
+    double* oz = oy + W;
+    Terms* pt = (Terms*)(oz + W);
+    int32_t* os = (int32_t*)(pt + W);
+    int32_t* oe = os + W;                  // each lane's first list entry
+    uint32_t* omap = (uint32_t*)(oe + W);  // entry of a round -> owner lane << 16 | list position
+    uint8_t* pf = (uint8_t*)(omap + W);
+    const int wl = threadIdx.x & (W - 1);
+    Acc<FT> acc;
+    auto pair = [&](int32_t j, double xi, double yi, double zi, Terms& tm) -> bool {
+        const Rec<FT> o = src[j];
+        const int tj = MULTI ? (int)o.type : 0;
+        double xj = o.x, yj = o.y, zj = o.z;
+        if (!split) {
+            const double cj = MULTI ? P.cm[tj] : c;
+            xj = wrap1(xj + ((o.vx + (double)o.fx * cj) * lam) * dt, bx);
+            yj = wrap1(yj + ((o.vy + (double)o.fy * cj) * lam) * dt, by);
+            zj = wrap1(zj + ((o.vz + (double)o.fz * cj) * lam) * dt, bz);
+        }
+        const double dx = min_image(xj - xi, bx);
+        const double dy = min_image(yj - yi, by);
+        const double dz = min_image(zj - zi, bz);
+        const double r2 = (dx * dx + dy * dy) + dz * dz;
+        const PairParam& pp = MULTI ? d.pairs[ti * d.n_types + tj] : p0;
+        if (!(r2 <= pp.r2max)) return false;
+        tm = Acc<FT>::terms(dx, dy, dz, r2, pp);
+        return true;
+    };
+    if (!__ballot(n_mine > 1)) {
+        // at most one entry per lane: every lane takes its own
+        if (n_mine == 1) {
+            Terms tm;
+            if (pair(e0, x1, y1, z1, tm)) acc.add_terms(tm);
+        }
+        return acc;
+    }
+    const int incl = wave_scan_incl(n_mine);
+    const int my_off = incl - n_mine;
+    const int total = __builtin_amdgcn_readlane(incl, W - 1);
+    wave_lds_sync();  // every lane has read its record from this part of the tile
+    ox[wl] = x1;
+    oy[wl] = y1;
+    oz[wl] = z1;
+    os[wl] = (int32_t)s;
+    oe[wl] = e0;
+    for (int c0 = 0; c0 < total; c0 += W) {
+        // this round's entries [c0, c0 + W): the owner marks its own
+        const int a0 = my_off > c0 ? my_off : c0;
+        const int a1 = my_off + n_mine < c0 + W ? my_off + n_mine : c0 + W;
+        for (int a = a0; a < a1; ++a) omap[a - c0] = ((uint32_t)wl << 16) | (uint32_t)(a - my_off);
+        wave_lds_sync();
+        Terms tm{};
+        uint8_t ok = 0;
+        if (c0 + wl < total) {
+            const uint32_t om = omap[wl];
+            const int own = (int)(om >> 16), k = (int)(om & 0xffffu);
+            const int32_t j = k == 0 ? oe[own] : d.nbr[(int64_t)k * d.n_slots + os[own]];
+            ok = pair(j, ox[own], oy[own], oz[own], tm) ? 1 : 0;
+        }
+        pt[wl] = tm;
+        pf[wl] = ok;
+        wave_lds_sync();
+        for (int a = a0; a < a1; ++a)
+            if (pf[a - c0]) acc.add_terms(pt[a - c0]);
+        wave_lds_sync();
+    }
+    return acc;
+}
+
 // One launch of the step kernel over the block's tiles.  The record and list
 // length of the next tile are loaded before the current tile is processed.
 template <typename FT, int BS, bool MULTI>
 __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
-    const int b = blockIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
     const GroupInfo G = d.groups[d.blk_group[b]];
     const int r = G.rep;
     const RepCtrl& C = ctrl_in(d, r);
@@ -332,11 +430,12 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
         prof_block(d, P_STEP, t_start, true, 0);
         return;
     }
-    const double bx = C.box[0], by = C.box[1], bz = C.box[2];
-    const double dt = P.dt, lam = C.lambda, mu = C.myu;
-    const double m = G.mass;
+    const double bx = wave_uniform(C.box[0]), by = wave_uniform(C.box[1]), bz = wave_uniform(C.box[2]);
+    const double dt = wave_uniform(P.dt), lam = wave_uniform(C.lambda), mu = wave_uniform(C.myu);
+    const double m = wave_uniform(G.mass);
     const int ti = G.type;
-    const double c = P.cm[ti];
+    const double c = wave_uniform(P.cm[ti]);
+    const double inv_m2 = wave_uniform(P.inv_m2[ti]);
     const uint32_t kick_cols = P.th_kind == 2 ? kColsKick : 0u;
     double part[NS];
 #pragma unroll
@@ -373,7 +472,7 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     const bool split = (flags & F_SPLIT_B) != 0;
     const Rec<FT>* __restrict__ src = split ? rnx : rin;
     Rec<FT>* __restrict__ dst = split ? (Rec<FT>*)d.rec[cur] : rnx;
-    const PairParam p0 = d.pairs[0];
+    const PairParam p0 = wave_uniform(d.pairs[0]);
     int visits = 0;  // list entries read by this thread (profiling)
     // A tile's records move between HBM and LDS as whole 16-byte words with
     // consecutive lanes on consecutive words (fully coalesced); each thread
@@ -384,11 +483,13 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     __shared__ u4 s_tile[BS * RQ];
     u4 pre[RQ];
     int nn = 0;
+    int32_t e0 = 0;  // first list entry of the thread's particle (entry 0 of every slot exists)
     if (tr.t0 < tr.t1) {
         const u4* g = (const u4*)(src + G.start + tr.t0 * BS);
 #pragma unroll
         for (int q = 0; q < RQ; ++q) pre[q] = g[q * BS + lane];
         nn = tr.t0 * BS + lane < G.count ? d.ncount[base + tr.t0 * BS] : 0;
+        e0 = d.nbr[base + tr.t0 * BS];
     }
     for (int64_t t = tr.t0; t < tr.t1; ++t) {
         const int64_t s = base + t * BS;
@@ -398,15 +499,22 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
         lds_barrier();  // stores of the previous tile may still be in flight
         const Rec<FT> a = ((const Rec<FT>*)s_tile)[lane];
         int nnn = 0;
+        int32_t ne0 = 0;
         if (t + 1 < tr.t1) {
             const u4* g = (const u4*)(src + G.start + (t + 1) * BS);
 #pragma unroll
             for (int q = 0; q < RQ; ++q) pre[q] = g[q * BS + lane];
             nnn = (t + 1) * BS + lane < G.count ? d.ncount[s + BS] : 0;
+            ne0 = d.nbr[s + BS];
         }
-        Rec<FT> res = a;
+        // a padding slot (past the group's count) keeps the padding record
+        Rec<FT> res{};
+        res.x = res.y = res.z = __builtin_nan("");
+        res.vx = res.vy = res.vz = 0.0;
+        res.fx = res.fy = res.fz = (FT)0;
+        res.type = (uint32_t)ti;
+        double x1 = 0.0, y1 = 0.0, z1 = 0.0, v1x = 0.0, v1y = 0.0, v1z = 0.0;
         if (valid) {
-            double x1, y1, z1, v1x, v1y, v1z;
             if (split) {
                 x1 = a.x;
                 y1 = a.y;
@@ -432,28 +540,11 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
                 z1 = wrap1(a.z + v1z * dt, bz);
                 part[S_VMAX] = fmax(part[S_VMAX], (v1x * v1x + v1y * v1y) + v1z * v1z);
             }
-            // potential.rs:158-216 over the Verlet list at the drifted positions;
-            // a neighbour's drift is recomputed from its input record with the
-            // same operations its own thread performs, so it is bitwise the same
-            Acc<FT> acc;
-            for (int k = 0; k < nn; ++k) {
-                const int32_t j = d.nbr[(int64_t)k * d.n_slots + s];
-                const Rec<FT> o = src[j];
-                const int tj = MULTI ? (int)o.type : 0;
-                double xj = o.x, yj = o.y, zj = o.z;
-                if (!split) {
-                    const double cj = MULTI ? P.cm[tj] : c;
-                    xj = wrap1(xj + ((o.vx + (double)o.fx * cj) * lam) * dt, bx);
-                    yj = wrap1(yj + ((o.vy + (double)o.fy * cj) * lam) * dt, by);
-                    zj = wrap1(zj + ((o.vz + (double)o.fz * cj) * lam) * dt, bz);
-                }
-                const double dx = min_image(xj - x1, bx);
-                const double dy = min_image(yj - y1, by);
-                const double dz = min_image(zj - z1, bz);
-                const double r2 = (dx * dx + dy * dy) + dz * dz;
-                const PairParam& pp = MULTI ? d.pairs[ti * d.n_types + tj] : p0;
-                if (r2 <= pp.r2max) acc.add(dx, dy, dz, r2, pp);
-            }
+        }
+        const Acc<FT> acc = wave_list_forces<FT, MULTI>(
+            d, P, src, (char*)s_tile + (threadIdx.x >> 6) * 64 * sizeof(Rec<FT>), valid ? nn : 0, e0, s, x1, y1, z1,
+            ti, split, c, lam, dt, bx, by, bz, p0);
+        if (valid) {
             // integrator.rs:47-53 second half kick, barostat.rs:45-48 scale
             Rec<FT> out;
             out.vx = v1x + (double)acc.fx * c;
@@ -465,7 +556,7 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
             out.fx = acc.fx;
             out.fy = acc.fy;
             out.fz = acc.fz;
-            out.type = a.type;
+            out.type = (uint32_t)ti;
             res = out;
             if (flags & F_WRITE_PV) {
                 d.pot[s] = (double)acc.pot;
@@ -480,7 +571,7 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
             part[S_WV] = fmax(part[S_WV], (out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz);
             part[S_WA] = fmax(part[S_WA],
                               (((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
-                               (double)acc.fz * (double)acc.fz) * P.inv_m2[ti]);
+                               (double)acc.fz * (double)acc.fz) * inv_m2);
             if (flags & F_INSERT) grid_insert(d, C, R, s, out.x, out.y, out.z);
         }
         visits += valid ? nn : 0;
@@ -493,6 +584,7 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
         }
         lds_barrier();  // s_tile is read out before the next tile overwrites it
         nn = nnn;
+        e0 = ne0;
     }
     __shared__ int s_vis[BS / 64];
     if (d.prof) {
