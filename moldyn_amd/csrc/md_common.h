@@ -279,6 +279,47 @@ __device__ __forceinline__ int wave_scan_incl(int v) {
     return v;
 }
 
+// Two in-place exclusive scans of a[0..n) and b[0..n) (n <= kBrickCells) with
+// shared barriers; a[n] and b[n] receive the totals.
+template <int BS>
+__device__ void block_exclusive_scan2(int* a, int* b, int n) {
+    constexpr int NW = BS / 64;
+    __shared__ int wsa[NW], wsb[NW];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int chunk = (n + BS - 1) / BS;
+    const int lo = t * chunk < n ? t * chunk : n;
+    const int hi = lo + chunk < n ? lo + chunk : n;
+    int la = 0, lb = 0;
+    for (int i = lo; i < hi; ++i) {
+        la += a[i];
+        lb += b[i];
+    }
+    const int ia = wave_scan_incl(la), ib = wave_scan_incl(lb);
+    if (lane == 63) {
+        wsa[w] = ia;
+        wsb[w] = ib;
+    }
+    lds_barrier();
+    int pa = 0, pb = 0;
+    for (int i = 0; i < w; ++i) {
+        pa += wsa[i];
+        pb += wsb[i];
+    }
+    int ra = pa + ia - la, rb = pb + ib - lb;
+    for (int i = lo; i < hi; ++i) {
+        const int ca = a[i], cb = b[i];
+        a[i] = ra;
+        b[i] = rb;
+        ra += ca;
+        rb += cb;
+    }
+    if (t == BS - 1) {
+        a[n] = pa + ia;
+        b[n] = pb + ib;
+    }
+    lds_barrier();
+}
+
 // ---------------------------------------------------------------- cell grid
 
 __device__ __forceinline__ int cell_coord(double x, double b, int m) {

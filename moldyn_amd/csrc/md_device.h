@@ -270,7 +270,8 @@ constexpr int kFinBS = 512;
 constexpr int kBrickEdge = 14;
 constexpr int kBrickCells = 4096;  // (14 + 2)^3
 constexpr int kBrickAtoms = 1024;
-constexpr int kBrickTarget = 600;  // expected staged particles a brick is sized for
+constexpr int kBrickTarget = 900;  // expected staged particles a brick is sized for
+constexpr int kBrickOwn = 440;     // expected centre particles: one per thread of a 512-thread block
 constexpr int kListReg = 16;       // list entries kept (sorted) in registers while building
 
 // Cell grid of a fill: per axis floor(L / (r_list (1 + 1e-9))) cells in
@@ -304,17 +305,23 @@ __host__ __device__ inline void grid_dims(const double* gb, double rlist, int64_
     for (int k = 0; k < 3; ++k) gd[k] = g[k];
 }
 
-// Brick shape of a grid: the largest cubic edge E <= 8 cells whose halo
-// ((E + 2)^3 cells, or the whole axis where that is shorter) holds about
-// kBrickTarget particles at the grid's mean occupancy.
+// Brick shape of a grid: the largest cubic edge E <= kBrickEdge cells whose
+// halo ((E + 2)^3 cells, or the whole axis where that is shorter) holds about
+// kBrickTarget particles and whose centre (E^3 cells) about kBrickOwn at the
+// grid's mean occupancy.
 __host__ __device__ inline void brick_dims(const int* gd, int64_t n_atoms, int* edge, int* whole) {
     const double per_cell = (double)n_atoms / ((double)gd[0] * gd[1] * gd[2]);
     int best = 1;
     for (int e = 1; e <= kBrickEdge; ++e) {
-        double halo = 1.0;
+        double halo = 1.0, own = 1.0;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) halo *= (gd[k] <= e + 2) ? gd[k] : (e + 2);
-        if (halo <= (double)kBrickCells && halo * per_cell <= (double)kBrickTarget) best = e;
+        for (int k = 0; k < 3; ++k) {
+            halo *= (gd[k] <= e + 2) ? gd[k] : (e + 2);
+            own *= (gd[k] <= e + 2) ? gd[k] : e;
+        }
+        if (halo <= (double)kBrickCells && halo * per_cell <= (double)kBrickTarget &&
+            own * per_cell <= (double)kBrickOwn)
+            best = e;
     }
     int w = 0;
 #pragma unroll

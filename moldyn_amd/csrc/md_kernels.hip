@@ -148,9 +148,22 @@ __device__ void build_list_global(const DevState& d, const RepCtrl& C, const Rep
     list_close(d, slot, n);
 }
 
+// FP32 minimum image of one component (the prefilter of the list build).
+__device__ __forceinline__ float min_image_f(float r, float b) {
+    if (r < -0.5f * b) {
+        r += b;
+    } else if (r > 0.5f * b) {
+        r -= b;
+    }
+    return r;
+}
+
 // Lists of every particle of a replica, one brick of cells at a time: the
 // brick and a one-cell halo are staged in LDS (slot ids, positions, types),
-// then each particle of the brick scans its 27 neighbour cells there.
+// then each particle of the brick scans its 27 neighbour cells there, as nine
+// runs of consecutive staged particles (the cells of a z row).  A
+// conservative FP32 distance test rejects most candidates before the FP64 test
+// of the reference's minimum image decides.
 // Bricks are shared out among the replica's blocks.  The barriers order LDS
 // only: a brick's list stores stay in flight while the next brick loads.
 template <typename FT, int BS, bool MULTI>
@@ -167,6 +180,16 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
     const int cc = d.cell_cap;
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double rl2 = P.rlist * P.rlist;
+    // FP32 prefilter: positions below 1.01 L_max carry an FP32 error below
+    // 2^-24 (1.01 L_max) each, so a minimum-image component in FP32 is within
+    // eps = 5.05 * 2^-24 L_max of the FP64 one (two roundings of the
+    // positions, the difference, the box and the shift), and |r| within
+    // 2 sqrt(3) eps (a branch taken the other way near L/2 doubles it).  The
+    // squared sum adds 3 roundings; 2^-20 covers them.
+    const float bxf = (float)bx, byf = (float)by, bzf = (float)bz;
+    const double eps = 5.05 * 0x1p-24 * fmax(bx, fmax(by, bz));
+    const double rlm = P.rlist + 2.0 * 1.7320508075688772 * eps;
+    const float rl2f = __double2float_ru(rlm * rlm * (1.0 + 0x1p-20));
     const int64_t N = d.n_slots;
     int nbk[3], hd[3];
     for (int k = 0; k < 3; ++k) {
@@ -176,8 +199,11 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
     }
     const int nbricks = nbk[0] * nbk[1] * nbk[2];
     const int H = hd[0] * hd[1] * hd[2];
-    __shared__ int s_off[kBrickCells + 1];
+    __shared__ int s_off[kBrickCells + 1];  // staged particles before halo cell h
+    __shared__ int s_cen[kBrickCells + 1];  // the brick's own (centre) particles before cell h
+    __shared__ uint16_t s_cidx[kBrickAtoms];  // centre particle k -> its staging index
     __shared__ double s_x[kBrickAtoms], s_y[kBrickAtoms], s_z[kBrickAtoms];
+    __shared__ float s_xf[kBrickAtoms], s_yf[kBrickAtoms], s_zf[kBrickAtoms];
     __shared__ int32_t s_id[kBrickAtoms];
     __shared__ uint16_t s_hc[kBrickAtoms];
     __shared__ uint8_t s_ty[MULTI ? kBrickAtoms : 1];
@@ -211,12 +237,18 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             const int h = threadIdx.x + q * BS;
             if (h < H) {
                 const int cnt = (uint32_t)(hw[q] >> 32) == ep ? (int)(uint32_t)hw[q] : 0;
+                const int h0 = h / (hd[1] * hd[2]), h1 = (h / hd[2]) % hd[1], h2 = h % hd[2];
+                const bool own = ((whole & 1) || (h0 >= 1 && h0 <= ncen[0])) &&
+                                 ((whole & 2) || (h1 >= 1 && h1 <= ncen[1])) &&
+                                 ((whole & 4) || (h2 >= 1 && h2 <= ncen[2]));
                 s_off[h] = cnt < cc ? cnt : cc;
+                s_cen[h] = own ? (cnt < cc ? cnt : cc) : 0;
             }
         }
         lds_barrier();
-        block_exclusive_scan<BS>(s_off, H);
+        block_exclusive_scan2<BS>(s_off, s_cen, H);
         const int total = s_off[H];
+        const int n_own = s_cen[H];
         if (total > kBrickAtoms) {
             // too dense for LDS: lists of the brick's particles straight from the grid
             if (threadIdx.x == 0) atomicOr(&d.stats->brick_fallback, 1u);
@@ -233,10 +265,17 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             lds_barrier();
             continue;
         }
-        // staged particle -> its halo cell (LDS only), then one particle per
-        // thread: slot id and position loads of all particles go out together
-        for (int h = threadIdx.x; h < H; h += BS)
-            for (int a = s_off[h]; a < s_off[h + 1]; ++a) s_hc[a] = (uint16_t)h;
+        // staged particle -> its halo cell, centre particle -> staging index
+        // (LDS only), then one particle per thread: slot id and position
+        // loads of all particles go out together
+        for (int h = threadIdx.x; h < H; h += BS) {
+            const int a0 = s_off[h], a1 = s_off[h + 1], k0 = s_cen[h];
+            const bool own = s_cen[h + 1] > k0;
+            for (int a = a0; a < a1; ++a) {
+                s_hc[a] = (uint16_t)h;
+                if (own) s_cidx[k0 + (a - a0)] = (uint16_t)a;
+            }
+        }
         lds_barrier();
         for (int a = threadIdx.x; a < total; a += BS) {
             const int h = s_hc[a];
@@ -245,49 +284,76 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             s_x[a] = o.x;
             s_y[a] = o.y;
             s_z[a] = o.z;
+            s_xf[a] = (float)o.x;
+            s_yf[a] = (float)o.y;
+            s_zf[a] = (float)o.z;
             s_id[a] = j;
             if (MULTI) s_ty[a] = (uint8_t)o.type;
         }
         lds_barrier();
-        for (int a = threadIdx.x; a < total; a += BS) {
+        for (int k = threadIdx.x; k < n_own; k += BS) {
+            const int a = s_cidx[k];
             const int h = s_hc[a];
             const int hc[3] = {h / (hd[1] * hd[2]), (h / hd[2]) % hd[1], h % hd[2]};
-            bool center = true;
-            int lo[3], hi[3];
-            for (int k = 0; k < 3; ++k) {
-                if ((whole >> k) & 1) {
-                    lo[k] = gd[k] >= 3 ? -1 : 0;
-                    hi[k] = gd[k] >= 2 ? 1 : 0;
-                } else {
-                    center = center && hc[k] >= 1 && hc[k] <= ncen[k];
-                    lo[k] = -1;
-                    hi[k] = 1;
-                }
+            int lo[2], hi[2];
+            for (int q = 0; q < 2; ++q) {
+                const bool w = (whole >> q) & 1;
+                lo[q] = w ? (gd[q] >= 3 ? -1 : 0) : -1;
+                hi[q] = w ? (gd[q] >= 2 ? 1 : 0) : 1;
             }
-            if (!center) continue;
+            // the stencil's z cells as at most two runs of consecutive halo
+            // cells, whose particles are consecutive in the staging order
+            int za0, za1, zb0 = 0, zb1 = 0;
+            if (!((whole >> 2) & 1)) {
+                za0 = hc[2] - 1;
+                za1 = hc[2] + 2;
+            } else if (gd[2] <= 3) {
+                za0 = 0;
+                za1 = gd[2];
+            } else if (hc[2] == 0) {
+                za0 = 0;
+                za1 = 2;
+                zb0 = gd[2] - 1;
+                zb1 = gd[2];
+            } else if (hc[2] == gd[2] - 1) {
+                za0 = 0;
+                za1 = 1;
+                zb0 = gd[2] - 2;
+                zb1 = gd[2];
+            } else {
+                za0 = hc[2] - 1;
+                za1 = hc[2] + 2;
+            }
             const int32_t i = s_id[a];
             const int ti = MULTI ? (int)s_ty[a] : 0;
             const double xi = s_x[a], yi = s_y[a], zi = s_z[a];
+            const float xf = s_xf[a], yf = s_yf[a], zf = s_zf[a];
             RegList lst(d.nbr + i, N, d.list_cap);
+            auto scan = [&](int b0, int b1) {
+                for (int b2 = b0; b2 < b1; ++b2) {
+                    // FP32 test first: it never rejects a pair the FP64 test keeps
+                    const float ex = min_image_f(s_xf[b2] - xf, bxf);
+                    const float ey = min_image_f(s_yf[b2] - yf, byf);
+                    const float ez = min_image_f(s_zf[b2] - zf, bzf);
+                    if (!((ex * ex + ey * ey) + ez * ez <= rl2f)) continue;
+                    const int32_t j = s_id[b2];
+                    if (j == i) continue;
+                    if (MULTI && (int)s_ty[b2] < ti) continue;
+                    const double dx = min_image(s_x[b2] - xi, bx);
+                    const double dy = min_image(s_y[b2] - yi, by);
+                    const double dz = min_image(s_z[b2] - zi, bz);
+                    const double r2 = (dx * dx + dy * dy) + dz * dz;
+                    if (!(r2 <= rl2)) continue;
+                    lst.push(j);
+                }
+            };
             for (int o0 = lo[0]; o0 <= hi[0]; ++o0) {
                 const int g0 = ((whole & 1) ? (hc[0] + o0 + gd[0]) % gd[0] : hc[0] + o0);
                 for (int o1 = lo[1]; o1 <= hi[1]; ++o1) {
                     const int g1 = ((whole & 2) ? (hc[1] + o1 + gd[1]) % gd[1] : hc[1] + o1);
-                    for (int o2 = lo[2]; o2 <= hi[2]; ++o2) {
-                        const int g2 = ((whole & 4) ? (hc[2] + o2 + gd[2]) % gd[2] : hc[2] + o2);
-                        const int h2 = (g0 * hd[1] + g1) * hd[2] + g2;
-                        for (int b2 = s_off[h2]; b2 < s_off[h2 + 1]; ++b2) {
-                            const int32_t j = s_id[b2];
-                            if (j == i) continue;
-                            if (MULTI && (int)s_ty[b2] < ti) continue;
-                            const double dx = min_image(s_x[b2] - xi, bx);
-                            const double dy = min_image(s_y[b2] - yi, by);
-                            const double dz = min_image(s_z[b2] - zi, bz);
-                            const double r2 = (dx * dx + dy * dy) + dz * dz;
-                            if (!(r2 <= rl2)) continue;
-                            lst.push(j);
-                        }
-                    }
+                    const int row = (g0 * hd[1] + g1) * hd[2];
+                    scan(s_off[row + za0], s_off[row + za1]);
+                    if (zb1 > zb0) scan(s_off[row + zb0], s_off[row + zb1]);
                 }
             }
             lst.store();
