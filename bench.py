@@ -359,7 +359,8 @@ def main():
     avg_step_s = step_ms / max(step_n, 1) * 1e-3
     # mean list length over the timed launches (entries read / atoms / launches)
     n_l = st["list_visits"] / (n * max(span_n, 1)) if span_n else st["mean_list_len"]
-    bytes_per_launch = n * (124.0 + 28.0 * n_l)
+    # step records in and out (48 B each), 4 B index + 24 B position per list entry
+    bytes_per_launch = n * (96.0 + 28.0 * n_l)
     achieved = bytes_per_launch / avg_step_s / 1e9 if step_n else 0.0
     log(f"[bench] k_step avg {avg_step_s * 1e6:.2f} us over {step_n} launches, "
         f"k_fin avg {fin_ms / max(fin_n, 1) * 1e3:.2f} us over {fin_n}; <n_l> {n_l:.4f}")

@@ -320,7 +320,8 @@ static int auto_capacities(md_ctx* c) {
 static int alloc_lists(md_ctx* c) {
     free_list(c->list_allocs);
     drop_graph(c);
-    c->ctrl_valid = false;  // cleared headers: no old epoch can match
+    // the cleared headers match no epoch; the control blocks keep counting
+    // theirs, so a later upload still starts past every epoch in use
     void* p;
     int rc;
     if (!c->list_mode) {
@@ -408,6 +409,7 @@ static int alloc_base(md_ctx* c) {
     c->mlp = c->opt.force_mode == MD_FORCE_MLP;
     DevState& d = c->d;
     d = DevState{};
+    c->ctrl_valid = false;  // new control blocks: no state, no epochs yet
     const int64_t N = c->n_slots;
     const size_t nd = sizeof(double) * (size_t)std::max<int64_t>(N, 1);
     void* p;
@@ -415,6 +417,22 @@ static int alloc_base(md_ctx* c) {
     for (int b = 0; b < 2; ++b) {
         if ((rc = dalloc(c, c->base_allocs, &p, rec_size(c) * (size_t)std::max<int64_t>(N, 1)))) return rc;
         d.rec[b] = p;
+    }
+    if (c->list_mode && !c->mlp) {
+        // the list path steps on position + kicked velocity records
+        for (int b = 0; b < 2; ++b) {
+            if ((rc = dalloc(c, c->base_allocs, &p, sizeof(CRec) * (size_t)std::max<int64_t>(N, 1)))) return rc;
+            d.crec[b] = (CRec*)p;
+        }
+        std::vector<uint8_t> ty((size_t)std::max<int64_t>(N, 1), 0);
+        for (const GroupInfo& G : c->groups) {
+            const int64_t n = (G.count + c->bs - 1) / c->bs * c->bs;
+            for (int64_t i = 0; i < n; ++i) ty[(size_t)(G.start + i)] = (uint8_t)G.type;
+        }
+        if ((rc = dalloc(c, c->base_allocs, &p, ty.size()))) return rc;
+        HIPCHK(c, hipMemcpy(p, ty.data(), ty.size(), hipMemcpyHostToDevice));
+        d.slot_type = (const uint8_t*)p;
+        d.compact = 1;
     }
     if ((rc = dalloc(c, c->base_allocs, &p, nd))) return rc;
     d.pot = (double*)p;
@@ -641,13 +659,14 @@ static int ensure_graph(md_ctx* c, int len) {
 // ------------------------------------------------------------------ snapshot
 
 static size_t state_bytes(md_ctx* c) {
-    // records, pot, vir, the group sums the first step's controls derive from,
-    // and the surrogate's counters
-    return (rec_size(c) + 2 * sizeof(double)) * (size_t)c->n_slots + sizeof(double) * NS * c->groups.size() +
+    // both record buffers (replicas after a split step sit at different
+    // parities), pot, vir, the group sums the first step's controls derive
+    // from, and the surrogate's counters
+    return (2 * rec_size(c) + 2 * sizeof(double)) * (size_t)c->n_slots + sizeof(double) * NS * c->groups.size() +
            (c->d.mlp_stat ? sizeof(MlpStat) * c->R : 0);
 }
 
-static int snapshot(md_ctx* c, bool save, int buf) {
+static int snapshot(md_ctx* c, bool save) {
     const size_t n = sizeof(double) * (size_t)c->n_slots;
     if (!c->snap) {
         c->snap_bytes = std::max<size_t>(state_bytes(c), 256);
@@ -655,11 +674,11 @@ static int snapshot(md_ctx* c, bool save, int buf) {
         HIPCHK(c, hipMalloc((void**)&c->snap_ctrl, sizeof(RepCtrl) * c->R));
     }
     char* s = (char*)c->snap;
-    void* arrs[5] = {c->d.rec[buf], c->d.pot, c->d.vir, c->d.sums, c->d.mlp_stat};
-    const size_t sz[5] = {rec_size(c) * (size_t)c->n_slots, n, n, sizeof(double) * NS * c->groups.size(),
-                          c->d.mlp_stat ? sizeof(MlpStat) * c->R : 0};
+    void* arrs[6] = {c->d.rec[0], c->d.rec[1], c->d.pot, c->d.vir, c->d.sums, c->d.mlp_stat};
+    const size_t sz[6] = {rec_size(c) * (size_t)c->n_slots, rec_size(c) * (size_t)c->n_slots, n, n,
+                          sizeof(double) * NS * c->groups.size(), c->d.mlp_stat ? sizeof(MlpStat) * c->R : 0};
     size_t off = 0;
-    for (int k = 0; k < 5; ++k) {
+    for (int k = 0; k < 6; ++k) {
         if (sz[k] == 0) continue;
         if (save)
             HIPCHK(c, hipMemcpyAsync(s + off, arrs[k], sz[k], hipMemcpyDeviceToDevice, c->st));
@@ -815,9 +834,8 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
     md_stats st{};
     while (done < n) {
         const int64_t cn = std::min(n - done, chunk);
-        int buf = C[0].cur;
         if (c->list_mode) {
-            if ((rc = snapshot(c, true, buf))) return rc;
+            if ((rc = snapshot(c, true))) return rc;
         }
         for (int r = 0; r < c->R; ++r) {
             P[r].target_step = step0 + done + cn;
@@ -825,6 +843,11 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
         }
         if ((rc = write_params(c, P))) return rc;
         if ((rc = reset_stats(c))) return rc;
+        if (c->d.compact) {
+            // the chunk steps on step records; its last step writes the full ones back
+            c->d.par = c->par;
+            HIPCHK(c, launch_pack(c->d, c->fp32, c->bs, c->st));
+        }
         HIPCHK(c, finalize(c, M_PRIME));
         int64_t k = 0;
         if (K > 1 && cn >= K) {
@@ -848,7 +871,7 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
             int grown = 0;
             if ((rc = grow_capacity(c, s, &grown))) return rc;
             if (++retries > 8) return fail(c, MD_ERR_OVERFLOW, "capacity kept overflowing");
-            if ((rc = snapshot(c, false, buf))) return rc;
+            if ((rc = snapshot(c, false))) return rc;
             if ((rc = read_ctrl(c, C))) return rc;
             for (int r = 0; r < c->R; ++r) C[r].flags &= ~(F_HAVE_LIST | F_GRID_FRESH);
             if ((rc = write_ctrl(c, C))) return rc;

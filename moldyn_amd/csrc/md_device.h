@@ -51,6 +51,18 @@ struct alignas(16) Rec<double> {
 static_assert(sizeof(Rec<float>) == 64, "record size");
 static_assert(sizeof(Rec<double>) == 80, "record size");
 
+// The list path's step record inside md_step: the position and the kicked
+// velocity u = v + F dt/(2m), the first half kick of the next step
+// (integrator.rs:28-34), rounded as that kick rounds it.  The next step and
+// every neighbour's drift need u alone, not v and F.  In the first half of a
+// split step the velocity field holds the scaled velocity instead.  The full
+// records (Rec) stay the state between calls: a chunk packs them at its
+// start, and its last step writes them back.
+struct alignas(16) CRec {
+    double x, y, z, ux, uy, uz;
+};
+static_assert(sizeof(CRec) == 48, "step record size");
+
 // per-launch control flags of a replica
 enum : uint32_t {
     F_INSERT = 1u << 1,     // insert this launch's output positions into grid `ins_epoch`
@@ -187,6 +199,9 @@ struct MlpStat {
 
 struct DevState {
     void* rec[2];            // Rec<float> or Rec<double>, n_slots each
+    CRec* crec[2];           // step records of the list path (null on the other paths)
+    const uint8_t* slot_type;  // type of every slot (padding slots: the group's type)
+    int32_t compact;         // the list path steps on crec (k_fin builds from crec)
     double* pot;
     double* vir;
     int32_t* nbr;
@@ -345,6 +360,7 @@ constexpr uint32_t kColsAll = (1u << NS) - 1u;
 
 // host-side launchers (md_kernels.hip)
 hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st);
+hipError_t launch_pack(const DevState& d, bool fp32, int bs, hipStream_t st);
 hipError_t launch_build(const DevState& d, bool fp32, hipStream_t st);
 hipError_t launch_prof_init(const DevState& d, hipStream_t st);
 // finalize (grid n_reps); with `build` the grid also covers the list-build

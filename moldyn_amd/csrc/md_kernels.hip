@@ -28,6 +28,13 @@
 
 namespace md {
 
+// Type of the particle in slot j: full records carry it, step records do not.
+template <typename FT>
+__device__ __forceinline__ int rec_type(const DevState&, const Rec<FT>* r, int64_t j) {
+    return (int)r[j].type;
+}
+__device__ __forceinline__ int rec_type(const DevState& d, const CRec*, int64_t j) { return (int)d.slot_type[j]; }
+
 // Sorted insertion of j into the list column of slot i (column-major).
 __device__ __forceinline__ void list_push(int32_t* __restrict__ col, int64_t N, int n, int cap,
                                           int32_t j) {
@@ -95,16 +102,16 @@ struct RegList {
 // does not fit in LDS): every j of the same replica, type >= own type
 // (potential.rs:171-180), j != slot, within rlist under the reference
 // minimum image, sorted ascending so that sums follow the reference's j order.
-template <typename FT, bool MULTI>
+template <typename RT, bool MULTI>
 __device__ void build_list_global(const DevState& d, const RepCtrl& C, const RepParams& P,
-                                  const RepInfo& R, const Rec<FT>* __restrict__ recs,
+                                  const RepInfo& R, const RT* __restrict__ recs,
                                   int64_t slot) {
     const uint32_t ep = C.fill_epoch;
     const int buf = ep & 1;
     const int* gd = C.gdim[buf];
     const double* gb = C.gbox[buf];
-    const Rec<FT> me = recs[slot];
-    const int ti = (int)me.type;
+    const RT me = recs[slot];
+    const int ti = MULTI ? rec_type(d, recs, slot) : 0;
     const int cx = cell_coord(me.x, gb[0], gd[0]);
     const int cy = cell_coord(me.y, gb[1], gd[1]);
     const int cz = cell_coord(me.z, gb[2], gd[2]);
@@ -132,8 +139,8 @@ __device__ void build_list_global(const DevState& d, const RepCtrl& C, const Rep
                 for (int k = 0; k < cnt; ++k) {
                     const int32_t j = cs[cell * cc + k];
                     if ((int64_t)j == slot) continue;
-                    const Rec<FT> o = recs[j];
-                    if (MULTI && (int)o.type < ti) continue;
+                    const RT o = recs[j];
+                    if (MULTI && rec_type(d, recs, j) < ti) continue;
                     const double dx = min_image(o.x - me.x, bx);
                     const double dy = min_image(o.y - me.y, by);
                     const double dz = min_image(o.z - me.z, bz);
@@ -158,6 +165,23 @@ __device__ __forceinline__ float min_image_f(float r, float b) {
     return r;
 }
 
+// LDS of a brick: one object per kernel, shared by the builders of both
+// record types.
+struct BrickLds {
+    int off[kBrickCells + 1];  // staged particles before halo cell h
+    int cen[kBrickCells + 1];  // the brick's own (centre) particles before cell h
+    double x[kBrickAtoms], y[kBrickAtoms], z[kBrickAtoms];
+    float xf[kBrickAtoms], yf[kBrickAtoms], zf[kBrickAtoms];
+    int32_t id[kBrickAtoms];
+    uint16_t cidx[kBrickAtoms];  // centre particle k -> its staging index
+    uint16_t hc[kBrickAtoms];    // staged particle -> its halo cell
+    uint8_t ty[kBrickAtoms];
+};
+__device__ __forceinline__ BrickLds& brick_lds() {
+    __shared__ BrickLds l;
+    return l;
+}
+
 // Lists of every particle of a replica, one brick of cells at a time: the
 // brick and a one-cell halo are staged in LDS (slot ids, positions, types),
 // then each particle of the brick scans its 27 neighbour cells there, as nine
@@ -166,15 +190,14 @@ __device__ __forceinline__ float min_image_f(float r, float b) {
 // of the reference's minimum image decides.
 // Bricks are shared out among the replica's blocks.  The barriers order LDS
 // only: a brick's list stores stay in flight while the next brick loads.
-template <typename FT, int BS, bool MULTI>
+template <typename RT, int BS, bool MULTI>
 __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams& P,
-                            const RepInfo& R, int block_in_rep) {
+                            const RepInfo& R, const RT* __restrict__ recs, int block_in_rep) {
     const uint32_t ep = C.fill_epoch;
     const int buf = ep & 1;
     const int* gd = C.gdim[buf];
     const int* be = C.bedge[buf];
     const int whole = C.bwhole[buf];
-    const Rec<FT>* __restrict__ recs = (const Rec<FT>*)d.rec[C.build_src];
     const unsigned long long* __restrict__ hdr = d.cell_hdr[buf];
     const int32_t* __restrict__ cs = d.cell_slots[buf];
     const int cc = d.cell_cap;
@@ -199,14 +222,19 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
     }
     const int nbricks = nbk[0] * nbk[1] * nbk[2];
     const int H = hd[0] * hd[1] * hd[2];
-    __shared__ int s_off[kBrickCells + 1];  // staged particles before halo cell h
-    __shared__ int s_cen[kBrickCells + 1];  // the brick's own (centre) particles before cell h
-    __shared__ uint16_t s_cidx[kBrickAtoms];  // centre particle k -> its staging index
-    __shared__ double s_x[kBrickAtoms], s_y[kBrickAtoms], s_z[kBrickAtoms];
-    __shared__ float s_xf[kBrickAtoms], s_yf[kBrickAtoms], s_zf[kBrickAtoms];
-    __shared__ int32_t s_id[kBrickAtoms];
-    __shared__ uint16_t s_hc[kBrickAtoms];
-    __shared__ uint8_t s_ty[MULTI ? kBrickAtoms : 1];
+    BrickLds& L = brick_lds();
+    int* const s_off = L.off;
+    int* const s_cen = L.cen;
+    uint16_t* const s_cidx = L.cidx;
+    double* const s_x = L.x;
+    double* const s_y = L.y;
+    double* const s_z = L.z;
+    float* const s_xf = L.xf;
+    float* const s_yf = L.yf;
+    float* const s_zf = L.zf;
+    int32_t* const s_id = L.id;
+    uint16_t* const s_hc = L.hc;
+    uint8_t* const s_ty = L.ty;
     for (int bi = block_in_rep; bi < nbricks; bi += R.bld_count) {
         const int bc[3] = {bi / (nbk[1] * nbk[2]), (bi / nbk[2]) % nbk[1], bi % nbk[2]};
         int org[3], ncen[3];
@@ -260,7 +288,7 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
                 const int64_t cell = cell_of(h);
                 const int cnt = s_off[h + 1] - s_off[h];
                 for (int k = 0; k < cnt; ++k)
-                    build_list_global<FT, MULTI>(d, C, P, R, recs, cs[cell * cc + k]);
+                    build_list_global<RT, MULTI>(d, C, P, R, recs, cs[cell * cc + k]);
             }
             lds_barrier();
             continue;
@@ -280,7 +308,7 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
         for (int a = threadIdx.x; a < total; a += BS) {
             const int h = s_hc[a];
             const int32_t j = cs[cell_of(h) * cc + (a - s_off[h])];
-            const Rec<FT>& o = recs[j];
+            const RT& o = recs[j];
             s_x[a] = o.x;
             s_y[a] = o.y;
             s_z[a] = o.z;
@@ -288,7 +316,7 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             s_yf[a] = (float)o.y;
             s_zf[a] = (float)o.z;
             s_id[a] = j;
-            if (MULTI) s_ty[a] = (uint8_t)o.type;
+            if (MULTI) s_ty[a] = (uint8_t)rec_type(d, recs, j);
         }
         lds_barrier();
         for (int k = threadIdx.x; k < n_own; k += BS) {
@@ -373,26 +401,30 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
+// LDS a wave's list forces use (owner positions, slots and first entries,
+// pair terms and flags, the owner map of a round).
+template <typename FT>
+constexpr int wave_scratch_bytes() {
+    return 64 * (3 * (int)sizeof(double) + (int)sizeof(typename Acc<FT>::Terms) + 3 * (int)sizeof(int32_t) + 1);
+}
+
 // The list forces of a wave's particles (potential.rs:158-216 over the Verlet
 // lists at the drifted positions).  With at most one entry per particle each
 // lane takes its own; otherwise the wave's entries are spread over its 64
 // lanes, one entry per lane and round, so the index and record loads
 // of a round go out together instead of one entry at a time per particle;
-// each lane computes its pair's terms into the wave's part of the LDS tile
-// (free once every lane holds its own record), then every particle adds its
-// own terms in list order: the order of a per-particle loop, so the sums are
-// bitwise those of one.  A neighbour's drift is recomputed from its input
-// record with the operations its own thread performs.
+// each lane computes its pair's terms into the wave's LDS scratch, then every
+// particle adds its own terms in list order: the order of a per-particle
+// loop, so the sums are bitwise those of one.  A neighbour's drift is
+// recomputed from its step record (position and kicked velocity) with the
+// operations its own thread performs.
 template <typename FT, bool MULTI>
-__device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const RepParams& P,
-                                                    const Rec<FT>* __restrict__ src, char* wreg, int n_mine,
-                                                    int32_t e0, int64_t s, double x1, double y1, double z1, int ti, bool split,
-                                                    double c, double lam, double dt, double bx, double by,
-                                                    double bz, const PairParam& p0) {
+__device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRec* __restrict__ src, char* wreg,
+                                                    int n_mine, int32_t e0, int64_t s, double x1, double y1,
+                                                    double z1, int ti, bool split, double lam, double dt,
+                                                    double bx, double by, double bz, const PairParam& p0) {
     typedef typename Acc<FT>::Terms Terms;
     constexpr int W = 64;
-    static_assert(3 * W * sizeof(double) + W * sizeof(Terms) + 3 * W * sizeof(int32_t) + W <= W * sizeof(Rec<FT>),
-                  "wave scratch exceeds the wave's part of the tile");
     double* ox = (double*)wreg;
     double* oy = ox + W;
     double* oz = oy + W;
@@ -404,14 +436,13 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const Rep
     const int wl = threadIdx.x & (W - 1);
     Acc<FT> acc;
     auto pair = [&](int32_t j, double xi, double yi, double zi, Terms& tm) -> bool {
-        const Rec<FT> o = src[j];
-        const int tj = MULTI ? (int)o.type : 0;
+        const CRec o = src[j];
+        const int tj = MULTI ? (int)d.slot_type[j] : 0;
         double xj = o.x, yj = o.y, zj = o.z;
         if (!split) {
-            const double cj = MULTI ? P.cm[tj] : c;
-            xj = wrap1(xj + ((o.vx + (double)o.fx * cj) * lam) * dt, bx);
-            yj = wrap1(yj + ((o.vy + (double)o.fy * cj) * lam) * dt, by);
-            zj = wrap1(zj + ((o.vz + (double)o.fz * cj) * lam) * dt, bz);
+            xj = wrap1(xj + (o.ux * lam) * dt, bx);
+            yj = wrap1(yj + (o.uy * lam) * dt, by);
+            zj = wrap1(zj + (o.uz * lam) * dt, bz);
         }
         const double dx = min_image(xj - xi, bx);
         const double dy = min_image(yj - yi, by);
@@ -433,7 +464,7 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const Rep
     const int incl = wave_scan_incl(n_mine);
     const int my_off = incl - n_mine;
     const int total = __builtin_amdgcn_readlane(incl, W - 1);
-    wave_lds_sync();  // every lane has read its record from this part of the tile
+    wave_lds_sync();  // the previous tile's rounds are done with the scratch
     ox[wl] = x1;
     oy[wl] = y1;
     oz[wl] = z1;
@@ -463,8 +494,11 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const Rep
     return acc;
 }
 
-// One launch of the step kernel over the block's tiles.  The record and list
-// length of the next tile are loaded before the current tile is processed.
+// One launch of the step kernel over the block's tiles, on the step records
+// (position and kicked velocity).  The record, list length and first list
+// entry of the next tile are loaded before the current tile is processed.
+// The last step of a chunk also writes the full records (x, v, F) and the
+// per-particle energies.
 template <typename FT, int BS, bool MULTI>
 __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -483,8 +517,8 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     const int lane = threadIdx.x;
     const int64_t base = G.start + lane;
     const int cur = C.cur, nx = cur ^ 1;
-    const Rec<FT>* __restrict__ rin = (const Rec<FT>*)d.rec[cur];
-    Rec<FT>* __restrict__ rnx = (Rec<FT>*)d.rec[nx];
+    const CRec* __restrict__ rin = d.crec[cur];
+    CRec* __restrict__ rnx = d.crec[nx];
     double* const part_out = d.partial + b;
     const int64_t pstride = d.n_blocks;
     if (flags & F_NULL) {
@@ -507,27 +541,26 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
     if (flags & F_SPLIT_A) {
-        // first half of a split step: kick, scale, drift, wrap into cur ^ 1 and
-        // fill the grid with the drifted positions (the drift exceeds the skin)
+        // first half of a split step: kick, scale, drift, wrap into cur ^ 1
+        // (the velocity field now holds the scaled velocity) and fill the grid
+        // with the drifted positions (the drift exceeds the skin)
         for (int64_t t = tr.t0; t < tr.t1; ++t) {
             const int64_t s = base + t * BS;
             if (t * BS + lane >= G.count) continue;
-            Rec<FT> a = rin[s];
-            const double vkx = a.vx + (double)a.fx * c;
-            const double vky = a.vy + (double)a.fy * c;
-            const double vkz = a.vz + (double)a.fz * c;
+            CRec a = rin[s];
+            const double vkx = a.ux, vky = a.uy, vkz = a.uz;
             part[S_KVX] += vkx * m;
             part[S_KVY] += vky * m;
             part[S_KVZ] += vkz * m;
             part[S_KV2] += m * ((vkx * vkx + vky * vky) + vkz * vkz);
-            a.vx = vkx * lam;
-            a.vy = vky * lam;
-            a.vz = vkz * lam;
-            a.x = wrap1(a.x + a.vx * dt, bx);
-            a.y = wrap1(a.y + a.vy * dt, by);
-            a.z = wrap1(a.z + a.vz * dt, bz);
+            a.ux = vkx * lam;
+            a.uy = vky * lam;
+            a.uz = vkz * lam;
+            a.x = wrap1(a.x + a.ux * dt, bx);
+            a.y = wrap1(a.y + a.uy * dt, by);
+            a.z = wrap1(a.z + a.uz * dt, bz);
             rnx[s] = a;
-            part[S_VMAX] = fmax(part[S_VMAX], (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
+            part[S_VMAX] = fmax(part[S_VMAX], (a.ux * a.ux + a.uy * a.uy) + a.uz * a.uz);
             grid_insert(d, C, R, s, a.x, a.y, a.z);
         }
         block_store<BS>(part, part_out, pstride, kick_cols | kColsDrift);
@@ -536,8 +569,9 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     }
     // second half of a split step: drifted state in cur ^ 1, results to cur
     const bool split = (flags & F_SPLIT_B) != 0;
-    const Rec<FT>* __restrict__ src = split ? rnx : rin;
-    Rec<FT>* __restrict__ dst = split ? (Rec<FT>*)d.rec[cur] : rnx;
+    const CRec* __restrict__ src = split ? rnx : rin;
+    CRec* __restrict__ dst = split ? d.crec[cur] : rnx;
+    Rec<FT>* __restrict__ full = (Rec<FT>*)(split ? d.rec[cur] : d.rec[nx]);
     const PairParam p0 = wave_uniform(d.pairs[0]);
     int visits = 0;  // list entries read by this thread (profiling)
     // A tile's records move between HBM and LDS as whole 16-byte words with
@@ -545,8 +579,11 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     // then reads and writes its own record in LDS.  The next tile's words
     // load into registers while the current tile is processed.
     typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-    constexpr int RQ = (int)(sizeof(Rec<FT>) / 16);
+    constexpr int RQ = (int)(sizeof(CRec) / 16);
+    constexpr int WS = wave_scratch_bytes<FT>();
     __shared__ u4 s_tile[BS * RQ];
+    __shared__ u4 s_ws[(BS / 64) * WS / 16];
+    char* const wreg = (char*)s_ws + (threadIdx.x >> 6) * WS;
     u4 pre[RQ];
     int nn = 0;
     int32_t e0 = 0;  // first list entry of the thread's particle (entry 0 of every slot exists)
@@ -563,7 +600,7 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
 #pragma unroll
         for (int q = 0; q < RQ; ++q) s_tile[q * BS + lane] = pre[q];
         lds_barrier();  // stores of the previous tile may still be in flight
-        const Rec<FT> a = ((const Rec<FT>*)s_tile)[lane];
+        const CRec a = ((const CRec*)s_tile)[lane];
         int nnn = 0;
         int32_t ne0 = 0;
         if (t + 1 < tr.t1) {
@@ -574,25 +611,21 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
             ne0 = d.nbr[s + BS];
         }
         // a padding slot (past the group's count) keeps the padding record
-        Rec<FT> res{};
+        CRec res;
         res.x = res.y = res.z = __builtin_nan("");
-        res.vx = res.vy = res.vz = 0.0;
-        res.fx = res.fy = res.fz = (FT)0;
-        res.type = (uint32_t)ti;
+        res.ux = res.uy = res.uz = 0.0;
         double x1 = 0.0, y1 = 0.0, z1 = 0.0, v1x = 0.0, v1y = 0.0, v1z = 0.0;
         if (valid) {
             if (split) {
                 x1 = a.x;
                 y1 = a.y;
                 z1 = a.z;
-                v1x = a.vx;
-                v1y = a.vy;
-                v1z = a.vz;
+                v1x = a.ux;
+                v1y = a.uy;
+                v1z = a.uz;
             } else {
-                // integrator.rs:28-34 first half kick
-                const double vkx = a.vx + (double)a.fx * c;
-                const double vky = a.vy + (double)a.fy * c;
-                const double vkz = a.vz + (double)a.fz * c;
+                // integrator.rs:28-34 first half kick: the record holds it
+                const double vkx = a.ux, vky = a.uy, vkz = a.uz;
                 part[S_KVX] += vkx * m;
                 part[S_KVY] += vky * m;
                 part[S_KVZ] += vkz * m;
@@ -607,41 +640,55 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
                 part[S_VMAX] = fmax(part[S_VMAX], (v1x * v1x + v1y * v1y) + v1z * v1z);
             }
         }
-        const Acc<FT> acc = wave_list_forces<FT, MULTI>(
-            d, P, src, (char*)s_tile + (threadIdx.x >> 6) * 64 * sizeof(Rec<FT>), valid ? nn : 0, e0, s, x1, y1, z1,
-            ti, split, c, lam, dt, bx, by, bz, p0);
+        const Acc<FT> acc = wave_list_forces<FT, MULTI>(d, src, wreg, valid ? nn : 0, e0, s, x1, y1, z1, ti,
+                                                        split, lam, dt, bx, by, bz, p0);
         if (valid) {
             // integrator.rs:47-53 second half kick, barostat.rs:45-48 scale
-            Rec<FT> out;
-            out.vx = v1x + (double)acc.fx * c;
-            out.vy = v1y + (double)acc.fy * c;
-            out.vz = v1z + (double)acc.fz * c;
-            out.x = x1 * mu;
-            out.y = y1 * mu;
-            out.z = z1 * mu;
-            out.fx = acc.fx;
-            out.fy = acc.fy;
-            out.fz = acc.fz;
-            out.type = (uint32_t)ti;
-            res = out;
+            const double vx = v1x + (double)acc.fx * c;
+            const double vy = v1y + (double)acc.fy * c;
+            const double vz = v1z + (double)acc.fz * c;
+            res.x = x1 * mu;
+            res.y = y1 * mu;
+            res.z = z1 * mu;
+            // the next step's first half kick, as integrator.rs:28-34 rounds it
+            res.ux = vx + (double)acc.fx * c;
+            res.uy = vy + (double)acc.fy * c;
+            res.uz = vz + (double)acc.fz * c;
             if (flags & F_WRITE_PV) {
                 d.pot[s] = (double)acc.pot;
                 d.vir[s] = (double)acc.vir;
+                Rec<FT> o{};
+                o.x = res.x;
+                o.y = res.y;
+                o.z = res.z;
+                o.vx = vx;
+                o.vy = vy;
+                o.vz = vz;
+                o.fx = acc.fx;
+                o.fy = acc.fy;
+                o.fz = acc.fz;
+                o.type = (uint32_t)ti;
+                full[s] = o;
             }
-            part[S_MVX] += out.vx * m;
-            part[S_MVY] += out.vy * m;
-            part[S_MVZ] += out.vz * m;
-            part[S_MV2] += m * ((out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz);
+            part[S_MVX] += vx * m;
+            part[S_MVY] += vy * m;
+            part[S_MVZ] += vz * m;
+            part[S_MV2] += m * ((vx * vx + vy * vy) + vz * vz);
             part[S_POT] += (double)acc.pot;
             part[S_VIR] += (double)acc.vir;
-            part[S_WV] = fmax(part[S_WV], (out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz);
+            part[S_WV] = fmax(part[S_WV], (vx * vx + vy * vy) + vz * vz);
             part[S_WA] = fmax(part[S_WA],
                               (((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
                                (double)acc.fz * (double)acc.fz) * inv_m2);
-            if (flags & F_INSERT) grid_insert(d, C, R, s, out.x, out.y, out.z);
+            if (flags & F_INSERT) grid_insert(d, C, R, s, res.x, res.y, res.z);
+        } else if (flags & F_WRITE_PV) {
+            Rec<FT> o{};
+            o.x = o.y = o.z = __builtin_nan("");
+            o.type = (uint32_t)ti;
+            full[s] = o;
         }
         visits += valid ? nn : 0;
-        ((Rec<FT>*)s_tile)[lane] = res;
+        ((CRec*)s_tile)[lane] = res;
         lds_barrier();
         {
             u4* g = (u4*)(dst + G.start + t * BS);
@@ -667,6 +714,32 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     }
 }
 
+// Step records of the list path from the full records of each replica's
+// current buffer: the position and the first half kick of the next step
+// (integrator.rs:28-34).
+template <typename FT, int BS>
+__global__ __launch_bounds__(BS) void k_pack(DevState d) {
+    const int b = blockIdx.x;
+    const GroupInfo G = d.groups[d.blk_group[b]];
+    const double c = d.params[G.rep].cm[G.type];
+    const int buf = ctrl_in(d, G.rep).cur;
+    const TileRange tr = block_tiles(d, G, b, BS);
+    const Rec<FT>* __restrict__ in = (const Rec<FT>*)d.rec[buf];
+    CRec* __restrict__ out = d.crec[buf];
+    for (int64_t t = tr.t0; t < tr.t1; ++t) {
+        const int64_t s = G.start + t * BS + threadIdx.x;
+        const Rec<FT> a = in[s];
+        CRec o;
+        o.x = a.x;
+        o.y = a.y;
+        o.z = a.z;
+        o.ux = a.vx + (double)a.fx * c;
+        o.uy = a.vy + (double)a.fy * c;
+        o.uz = a.vz + (double)a.fz * c;
+        out[s] = o;
+    }
+}
+
 // Stand-alone list build (md_update_force on the list paths): replicas whose
 // control block is flagged F_BUILD build their lists from grid `fill_epoch`.
 template <typename FT, bool MULTI>
@@ -676,7 +749,8 @@ __global__ __launch_bounds__(kFinBS) void k_build(DevState d) {
     const RepCtrl& C = ctrl_in(d, r);
     if ((C.flags & (F_BUILD | F_EXIT)) != F_BUILD) return;
     const RepInfo R = d.reps[r];
-    brick_build<FT, kFinBS, MULTI>(d, C, d.params[r], R, b - R.bld_first);
+    brick_build<Rec<FT>, kFinBS, MULTI>(d, C, d.params[r], R, (const Rec<FT>*)d.rec[C.build_src],
+                                        b - R.bld_first);
 }
 
 // Force evaluation at the current positions through the current lists
@@ -1310,7 +1384,11 @@ __global__ __launch_bounds__(kFinBS) void k_fin(DevState d, int mode) {
         const RepCtrl& C = ctrl_in(d, r);
         if ((C.flags & (F_BUILD | F_EXIT)) == F_BUILD) {
             const RepInfo R = d.reps[r];
-            brick_build<FT, kFinBS, MULTI>(d, C, d.params[r], R, b - R.bld_first);
+            if (d.compact)
+                brick_build<CRec, kFinBS, MULTI>(d, C, d.params[r], R, d.crec[C.build_src], b - R.bld_first);
+            else
+                brick_build<Rec<FT>, kFinBS, MULTI>(d, C, d.params[r], R, (const Rec<FT>*)d.rec[C.build_src],
+                                                    b - R.bld_first);
             launch = C.launches;
         }
     }
@@ -1386,6 +1464,17 @@ __global__ __launch_bounds__(256) void k_prof_init(DevState d) {
 hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st) {
     const bool multi = d.n_types > 1;
 #define L(B) MD_LAUNCH_FT_MULTI(k_step, B)
+    MD_BS_SWITCH(bs, L)
+#undef L
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const DevState& d, bool fp32, int bs, hipStream_t st) {
+#define L(B)                                                                         \
+    if (fp32)                                                                        \
+        hipLaunchKernelGGL((k_pack<float, B>), dim3(d.n_blocks), dim3(B), 0, st, d); \
+    else                                                                             \
+        hipLaunchKernelGGL((k_pack<double, B>), dim3(d.n_blocks), dim3(B), 0, st, d);
     MD_BS_SWITCH(bs, L)
 #undef L
     return hipGetLastError();

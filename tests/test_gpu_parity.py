@@ -274,6 +274,68 @@ def test_overflow_replay_under_thermostat_and_barostat(mode):
     assert np.array_equal(a["box"], b["box"]) and la == lb and ma == mb
 
 
+@pytest.mark.parametrize("mode,npt", [("fp64_list", False), ("fp32_list", False), ("fp32_list", True)])
+def test_chunk_boundaries(mode, npt):
+    """md_step runs in chunks of at least 512 steps; the list path steps on
+    position + kicked-velocity records packed at a chunk's start and written
+    back to the full records by its last step.  1100 steps in one call
+    (chunks of 512, 512, 76) equal 600 + 500 in two calls, and on the FP64
+    path the oracle's 1100 steps, bit for bit."""
+    import moldyn_amd as M
+    s = liquid_state(4, temperature=150.0)
+    outs = []
+    for parts in ([1100], [600, 500]):
+        e = engine_for(s, mode)
+        e.update_force()
+        for k in parts:
+            th = M.thermostat("berendsen", tau=1.0, target_t=150.0) if npt else None
+            ba = M.barostat("berendsen", beta=1.0, tau=5.0, target_p=5.0) if npt else None
+            e.step(0.002, k, th, ba)
+        outs.append(e.download())
+        e.close()
+    a, b = outs
+    for key in ("x", "v", "force", "pot", "virial", "box"):
+        assert np.array_equal(a[key], b[key]), key
+    if mode == "fp64_list":
+        ref = s.copy()
+        tab = O.potential_table(1)
+        O.update_force(ref, tab)
+        for _ in range(1100):
+            O.verlet_step(ref, 0.002, tab)
+        assert np.array_equal(a["x"], ref.pos) and np.array_equal(a["v"], ref.vel)
+        assert np.array_equal(a["force"], ref.force)
+
+
+@pytest.mark.parametrize("mode", ["fp64_list", "fp32_list"])
+def test_reupload_after_capacity_growth(mode):
+    """Capacity overflows reallocate and clear the cell grid; the grid epochs
+    keep counting, so a second run from a re-upload into the same engine
+    meets no stale cell of the first and equals it bit for bit, and a run
+    in a fresh engine with roomy capacities."""
+    import moldyn_amd as M
+    s = sc_state(8, a=0.55, temperature=120.0, seed=7, jitter=0.04)
+
+    def run(e):
+        e.upload(s.pos, s.vel, s.box)
+        e.update_force()
+        th = M.thermostat("berendsen", tau=1.0, target_t=150.0)
+        ba = M.barostat("berendsen", beta=1.0, tau=5.0, target_p=5.0)
+        e.step(0.002, 300, th, ba)
+        return e.download(), e.stats()
+
+    e = engine_for(s, mode, skin=0.05, list_capacity=2, cell_capacity=1)
+    a, sa = run(e)
+    b, sb = run(e)
+    e.close()
+    f = engine_for(s, mode, skin=0.05, list_capacity=64, cell_capacity=64)
+    c, _ = run(f)
+    f.close()
+    assert sa["list_capacity"] > 2 and sa["cell_capacity"] > 1  # the grid was reallocated
+    for key in ("x", "v", "force", "box"):
+        assert np.array_equal(a[key], b[key]), key
+        assert np.array_equal(a[key], c[key]), key
+
+
 def test_fp64_list_equals_allpairs_trajectory():
     s = liquid_state(4, temperature=150.0)
     a = engine_for(s, "fp64_allpairs")
