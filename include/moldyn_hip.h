@@ -262,6 +262,7 @@ typedef struct {
     int32_t n_slots;        /* padded particle slots on the device */
     int64_t grows;          /* capacity increases between chunks (no replay) */
     int64_t list_visits;    /* neighbour-list entries read by the step kernel while profiling */
+    int64_t split_steps;    /* steps whose drift exceeded the skin (drift, rebuild, then forces) */
 } md_stats;
 int md_get_stats(md_ctx* ctx, md_stats* out);
 /* Per-kernel device time of the launches of md_step: the step kernel

@@ -66,7 +66,8 @@ class MdStats(C.Structure):
                 ("refresh_steps", C.c_int64), ("retries", C.c_int64),
                 ("list_capacity", C.c_int32), ("cell_capacity", C.c_int32),
                 ("mean_list_len", C.c_double), ("max_list_len", C.c_int32),
-                ("n_slots", C.c_int32), ("grows", C.c_int64), ("list_visits", C.c_int64)]
+                ("n_slots", C.c_int32), ("grows", C.c_int64), ("list_visits", C.c_int64),
+                ("split_steps", C.c_int64)]
 
 
 class MdError(RuntimeError):

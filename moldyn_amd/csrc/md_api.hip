@@ -87,7 +87,7 @@ struct md_ctx {
     int64_t kspan_count[P_NUM] = {0, 0, 0};
     int64_t list_visits = 0;
     md_stats stats{};
-    int64_t last_rebuilds = 0, last_refresh = 0;
+    int64_t last_rebuilds = 0, last_refresh = 0, last_splits = 0;
     bool ctrl_valid = false;  // ctrl holds epochs of fills into the current grid buffers
     std::string err;
 };
@@ -910,15 +910,18 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
         if (b && b->kind > 0 && (per_rep || r == 0)) b->myu = C[r].myu_used;
     }
     st.steps = n;
-    int64_t rb = 0, nl = 0;
+    int64_t rb = 0, nl = 0, ns = 0;
     for (int r = 0; r < c->R; ++r) {
         rb += C[r].rebuilds;
         nl += C[r].nulls;
+        ns += C[r].splits;
     }
     st.rebuilds = rb - c->stats.rebuilds;  // counters in ctrl are cumulative
     st.refresh_steps = nl - c->stats.refresh_steps;
+    st.split_steps = ns - c->stats.split_steps;
     c->stats.rebuilds = rb;
     c->stats.refresh_steps = nl;
+    c->stats.split_steps = ns;
     c->stats.steps = st.steps;
     c->stats.launches = st.launches;
     c->stats.retries = st.retries;
@@ -928,6 +931,7 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
     c->stats.n_slots = (int32_t)c->n_slots;
     c->last_rebuilds = st.rebuilds;
     c->last_refresh = st.refresh_steps;
+    c->last_splits = st.split_steps;
     c->sums_valid = true;
     if (bad) return fail(c, MD_ERR_BAROSTAT, "non-finite temperature/pressure or mu^3 <= 0 during the run");
     return MD_OK;
@@ -1207,6 +1211,7 @@ static int reset_controls(md_ctx* c, const double* box, uint32_t epoch0) {
     c->sums_valid = false;
     c->stats.rebuilds = 0;
     c->stats.refresh_steps = 0;
+    c->stats.split_steps = 0;
     return MD_OK;
 }
 
@@ -1678,6 +1683,7 @@ int md_get_stats(md_ctx* c, md_stats* out) {
     s.steps = c->stats.steps;
     s.rebuilds = c->last_rebuilds;
     s.refresh_steps = c->last_refresh;
+    s.split_steps = c->last_splits;
     s.list_capacity = c->d.list_cap;
     s.cell_capacity = c->d.cell_cap;
     s.n_slots = (int32_t)c->n_slots;

@@ -336,6 +336,37 @@ def test_reupload_after_capacity_growth(mode):
         assert np.array_equal(a[key], c[key]), key
 
 
+@pytest.mark.parametrize("mode,npt", [("fp64_list", False), ("fp64_list", True), ("fp32_list", True)])
+def test_split_steps(mode, npt):
+    """A skin thinner than one step's drift: each step drifts first, rebuilds
+    its lists at the drifted positions, then computes forces and the second
+    half kick (three launches).  The trajectory equals the one with a roomy
+    skin bit for bit, and the oracle's on the FP64 NVE path."""
+    import moldyn_amd as M
+    s = liquid_state(4, temperature=150.0)
+    outs = []
+    for skin in (0.002, 0.3):
+        e = engine_for(s, mode, skin=skin)
+        e.update_force()
+        th = M.thermostat("berendsen", tau=1.0, target_t=200.0) if npt else None
+        ba = M.barostat("berendsen", beta=1.0, tau=5.0, target_p=5.0) if npt else None
+        e.step(0.002, 60, th, ba)
+        outs.append((e.download(), e.stats()))
+        e.close()
+    (a, sa), (b, sb) = outs
+    assert sa["split_steps"] >= 10 and sb["split_steps"] == 0
+    for key in ("x", "v", "force", "box"):
+        assert np.array_equal(a[key], b[key]), key
+    if not npt:
+        ref = s.copy()
+        tab = O.potential_table(1)
+        O.update_force(ref, tab)
+        for _ in range(60):
+            O.verlet_step(ref, 0.002, tab)
+        assert np.array_equal(a["x"], ref.pos) and np.array_equal(a["v"], ref.vel)
+        assert np.array_equal(a["force"], ref.force)
+
+
 def test_fp64_list_equals_allpairs_trajectory():
     s = liquid_state(4, temperature=150.0)
     a = engine_for(s, "fp64_allpairs")
