@@ -190,9 +190,19 @@ __device__ __forceinline__ BrickLds& brick_lds() {
 // of the reference's minimum image decides.
 // Bricks are shared out among the replica's blocks.  The barriers order LDS
 // only: a brick's list stores stay in flight while the next brick loads.
+// Bricks of the grid a replica's lists are built from.
+__device__ __forceinline__ int brick_count(const RepCtrl& C) {
+    const int buf = C.fill_epoch & 1;
+    int n = 1;
+    for (int k = 0; k < 3; ++k)
+        n *= ((C.bwhole[buf] >> k) & 1) ? 1 : (C.gdim[buf][k] + C.bedge[buf][k] - 1) / C.bedge[buf][k];
+    return n;
+}
+
+// Bricks bi0, bi0 + stride, ... of a replica (all threads of the block).
 template <typename RT, int BS, bool MULTI>
 __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams& P,
-                            const RepInfo& R, const RT* __restrict__ recs, int block_in_rep) {
+                            const RepInfo& R, const RT* __restrict__ recs, int bi0, int stride) {
     const uint32_t ep = C.fill_epoch;
     const int buf = ep & 1;
     const int* gd = C.gdim[buf];
@@ -235,7 +245,8 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
     int32_t* const s_id = L.id;
     uint16_t* const s_hc = L.hc;
     uint8_t* const s_ty = L.ty;
-    for (int bi = block_in_rep; bi < nbricks; bi += R.bld_count) {
+    for (int64_t b64 = bi0; b64 < nbricks; b64 += stride) {  // 64-bit: a stride may be INT_MAX
+        const int bi = (int)b64;
         const int bc[3] = {bi / (nbk[1] * nbk[2]), (bi / nbk[2]) % nbk[1], bi % nbk[2]};
         int org[3], ncen[3];
         for (int k = 0; k < 3; ++k) {
@@ -750,7 +761,7 @@ __global__ __launch_bounds__(kFinBS) void k_build(DevState d) {
     if ((C.flags & (F_BUILD | F_EXIT)) != F_BUILD) return;
     const RepInfo R = d.reps[r];
     brick_build<Rec<FT>, kFinBS, MULTI>(d, C, d.params[r], R, (const Rec<FT>*)d.rec[C.build_src],
-                                        b - R.bld_first);
+                                        b - R.bld_first, R.bld_count);
 }
 
 // Force evaluation at the current positions through the current lists
@@ -1369,7 +1380,7 @@ __device__ void prof_fold(const DevState& d, int64_t launch, bool drift) {
     }
 }
 
-template <typename FT, bool MULTI, bool BUILD>
+template <typename FT, bool MULTI, bool BUILD, bool REPS>
 __global__ __launch_bounds__(kFinBS) void k_fin(DevState d, int mode) {
     const int b = blockIdx.x;
     const uint64_t t_start = prof_now(d);
@@ -1379,17 +1390,66 @@ __global__ __launch_bounds__(kFinBS) void k_fin(DevState d, int mode) {
         prof_fold(d, ctrl_in(d, 0).launches, !BUILD);
     int64_t launch = -1;
     if (b < d.n_reps) launch = finalize_replica(d, b, mode);
-    if (BUILD && b < d.n_bld) {
-        const int r = d.bld_rep[b];
-        const RepCtrl& C = ctrl_in(d, r);
+    if (BUILD && !REPS && b < d.n_bld) {
+        // one system: the build blocks share its bricks
+        const RepCtrl& C = ctrl_in(d, 0);
         if ((C.flags & (F_BUILD | F_EXIT)) == F_BUILD) {
-            const RepInfo R = d.reps[r];
+            const RepInfo R = d.reps[0];
             if (d.compact)
-                brick_build<CRec, kFinBS, MULTI>(d, C, d.params[r], R, d.crec[C.build_src], b - R.bld_first);
+                brick_build<CRec, kFinBS, MULTI>(d, C, d.params[0], R, d.crec[C.build_src], b, d.n_bld);
             else
-                brick_build<Rec<FT>, kFinBS, MULTI>(d, C, d.params[r], R, (const Rec<FT>*)d.rec[C.build_src],
-                                                    b - R.bld_first);
+                brick_build<Rec<FT>, kFinBS, MULTI>(d, C, d.params[0], R, (const Rec<FT>*)d.rec[C.build_src], b,
+                                                    d.n_bld);
             launch = C.launches;
+        }
+    } else if (BUILD && b < d.n_bld) {
+        // several replicas: the bricks of every replica building in this
+        // launch, numbered replica after replica, are shared out among all
+        // build blocks: block b takes bricks b, b + n_bld, ... (replicas
+        // rebuild at different launches, so a share fixed per replica would
+        // leave most blocks idle while a large replica builds).  A block's
+        // bricks within one replica are one strided run: one builder call
+        // per (replica, block).
+        __shared__ int s_rb[kFinBS + 1];
+        __shared__ int s_work[kFinBS];
+        __shared__ int s_nwork;
+        const int nb = d.n_bld;
+        int64_t base = 0;  // bricks of the replicas before this chunk
+        for (int r0 = 0; r0 < d.n_reps; r0 += kFinBS) {
+            const int n = d.n_reps - r0 < kFinBS ? d.n_reps - r0 : kFinBS;
+            if ((int)threadIdx.x < n) {
+                const RepCtrl& C = ctrl_in(d, r0 + threadIdx.x);
+                s_rb[threadIdx.x] = (C.flags & (F_BUILD | F_EXIT)) == F_BUILD ? brick_count(C) : 0;
+            }
+            if (threadIdx.x == 0) s_nwork = 0;
+            lds_barrier();
+            block_exclusive_scan<kFinBS>(s_rb, n);
+            const int tot = s_rb[n];
+            // replicas holding bricks of this block (in any order: each
+            // brick's lists depend on the brick alone)
+            if ((int)threadIdx.x < n) {
+                const int64_t lo = base + s_rb[threadIdx.x], hi = base + s_rb[threadIdx.x + 1];
+                const int64_t g0 = lo + ((b - lo % nb) % nb + nb) % nb;
+                if (g0 < hi) s_work[atomicAdd(&s_nwork, 1)] = threadIdx.x;
+            }
+            lds_barrier();
+            const int nwork = s_nwork;
+            for (int w = 0; w < nwork; ++w) {
+                const int idx = s_work[w];
+                const int64_t lo = base + s_rb[idx];
+                const int r = r0 + idx;
+                const int bi0 = (int)(((b - lo % nb) % nb + nb) % nb);
+                const RepCtrl& C = ctrl_in(d, r);
+                const RepInfo R = d.reps[r];
+                if (d.compact)
+                    brick_build<CRec, kFinBS, MULTI>(d, C, d.params[r], R, d.crec[C.build_src], bi0, nb);
+                else
+                    brick_build<Rec<FT>, kFinBS, MULTI>(d, C, d.params[r], R, (const Rec<FT>*)d.rec[C.build_src],
+                                                        bi0, nb);
+                launch = C.launches;
+            }
+            base += tot;
+            lds_barrier();  // s_rb and s_work are read before the next chunk rewrites them
         }
     }
     if (launch >= 0) {
@@ -1511,15 +1571,24 @@ hipError_t launch_finalize(const DevState& d, int mode, bool build, bool fp32, h
     const bool multi = d.n_types > 1;
     if (build && mode == M_STEP && d.n_bld > 0) {
         const int grid = d.n_bld > d.n_reps ? d.n_bld : d.n_reps;
+        const bool reps = d.n_reps > 1;
+#define KF(FT, M)                                                                                    \
+    do {                                                                                             \
+        if (reps)                                                                                    \
+            hipLaunchKernelGGL((k_fin<FT, M, true, true>), dim3(grid), dim3(kFinBS), 0, st, d, mode);  \
+        else                                                                                         \
+            hipLaunchKernelGGL((k_fin<FT, M, true, false>), dim3(grid), dim3(kFinBS), 0, st, d, mode); \
+    } while (0)
         if (fp32) {
-            if (multi) hipLaunchKernelGGL((k_fin<float, true, true>), dim3(grid), dim3(kFinBS), 0, st, d, mode);
-            else hipLaunchKernelGGL((k_fin<float, false, true>), dim3(grid), dim3(kFinBS), 0, st, d, mode);
+            if (multi) KF(float, true);
+            else KF(float, false);
         } else {
-            if (multi) hipLaunchKernelGGL((k_fin<double, true, true>), dim3(grid), dim3(kFinBS), 0, st, d, mode);
-            else hipLaunchKernelGGL((k_fin<double, false, true>), dim3(grid), dim3(kFinBS), 0, st, d, mode);
+            if (multi) KF(double, true);
+            else KF(double, false);
         }
+#undef KF
     } else {
-        hipLaunchKernelGGL((k_fin<double, false, false>), dim3(d.n_reps), dim3(kFinBS), 0, st, d, mode);
+        hipLaunchKernelGGL((k_fin<double, false, false, false>), dim3(d.n_reps), dim3(kFinBS), 0, st, d, mode);
     }
     return hipGetLastError();
 }
