@@ -106,11 +106,31 @@ def replicas_256(M, steps=2000, seq_sample=48):
         boxes.append([side * a[r]] * 3)
     boxes = np.array(boxes)
     L = M.engine.L
-    e = M.Engine(counts[:, None], [MASS], boxes, force_mode="fp32_list")
-    e.upload(np.concatenate(pos), np.concatenate(vel), boxes)
+
+    def controls():
+        th = (L.MdThermostat * R)(*[M.thermostat("berendsen", 10.0, float(t0[r])) for r in range(R)])
+        ba = (L.MdBarostat * R)(*[M.barostat("berendsen", 1.0, 5.0, float(p0[r])) for r in range(R)])
+        return th, ba
+
+    # the skin for this batch by the autotuner, on the state half-way through
+    # the run (the barostats compress most replicas, which lengthens the
+    # lists; dilute gases favour a wide skin, denser ones a thinner)
+    from moldyn_amd.autotune import autotune
+    xs, vs = np.concatenate(pos), np.concatenate(vel)
+    e0 = M.Engine(counts[:, None], [MASS], boxes, force_mode="fp32_list")
+    e0.upload(xs, vs, boxes)
+    e0.update_force()
+    th0, ba0 = controls()
+    e0.step(DT, 20 + steps // 2, th0, ba0)
+    mid = e0.download(("x", "v", "box"))
+    e0.close()
+    best, _ = autotune(counts[:, None], [MASS], mid["box"], mid["x"], mid["v"], DT, controls, steps=200,
+                       warmup=20, blocks=(256,), tiles=(0,), cells=(2.0,))
+    skin = best["skin"]
+    e = M.Engine(counts[:, None], [MASS], boxes, force_mode="fp32_list", skin=skin)
+    e.upload(xs, vs, boxes)
     e.update_force()
-    th = (L.MdThermostat * R)(*[M.thermostat("berendsen", 10.0, float(t0[r])) for r in range(R)])
-    ba = (L.MdBarostat * R)(*[M.barostat("berendsen", 1.0, 5.0, float(p0[r])) for r in range(R)])
+    th, ba = controls()
     e.step(DT, 20, th, ba)
     tb = time.perf_counter()
     e.step(DT, steps, th, ba)
@@ -121,7 +141,7 @@ def replicas_256(M, steps=2000, seq_sample=48):
     idx = rng.choice(R, seq_sample, replace=False)
     wall_s, atoms_s = 0.0, 0
     for r in idx:
-        one = M.Engine([int(counts[r])], [MASS], boxes[r], force_mode="fp32_list")
+        one = M.Engine([int(counts[r])], [MASS], boxes[r], force_mode="fp32_list", skin=skin)
         one.upload(pos[r], vel[r], boxes[r])
         one.update_force()
         th1 = M.thermostat("berendsen", 10.0, float(t0[r]))
@@ -135,6 +155,7 @@ def replicas_256(M, steps=2000, seq_sample=48):
     seq_rate = atoms_s * steps / wall_s
     return {"workload": "256 argon replicas, N log-uniform in [500, 32768] (seed 4), a in [3.0, 4.2], "
                         "Berendsen T0 in [80, 400] K, P0 in [0.5, 5.0], FP32 forces",
+            "skin_nm": skin,
             "atoms": total, "steps": steps, "batched_atom_steps_per_s": total * steps / wall_b,
             "batched_ms_per_step": 1e3 * wall_b / steps,
             "sequential_atom_steps_per_s": seq_rate,
