@@ -75,6 +75,7 @@ struct md_ctx {
     // graphs, by the parity they start at
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     int g_len[2] = {0, 0};
+    int g_dense[2] = {0, 0};  // the kernel instances (DevState::dense) a graph holds
     // profiling: per-launch device timestamps
     bool prof = false;
     unsigned long long* prof_ring = nullptr;
@@ -635,7 +636,7 @@ static hipError_t launch_once(md_ctx* c) {
 // replay ends at the parity it started at).
 static int ensure_graph(md_ctx* c, int len) {
     const int p0 = c->par;
-    if (c->gexec[p0] && c->g_len[p0] == len) return MD_OK;
+    if (c->gexec[p0] && c->g_len[p0] == len && c->g_dense[p0] == c->d.dense) return MD_OK;
     if (c->gexec[p0]) (void)hipGraphExecDestroy(c->gexec[p0]);
     c->gexec[p0] = nullptr;
     c->g_len[p0] = 0;
@@ -653,6 +654,7 @@ static int ensure_graph(md_ctx* c, int len) {
     if (e != hipSuccess) return fail(c, MD_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
     c->gexec[p0] = x;
     c->g_len[p0] = len;
+    c->g_dense[p0] = c->d.dense;
     return MD_OK;
 }
 
@@ -836,6 +838,16 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
         const int64_t cn = std::min(n - done, chunk);
         if (c->list_mode) {
             if ((rc = snapshot(c, true))) return rc;
+            // dense systems (long lists expected at the current densities)
+            // step and build with the long-list kernel instances
+            double ne = 0.0;
+            for (int r = 0; r < c->R; ++r) {
+                const double v = C[r].box[0] * C[r].box[1] * C[r].box[2];
+                if (v > 0.0)
+                    ne = std::max(ne, (double)c->reps[r].n_atoms / v * (4.0 / 3.0 * M_PI) * c->rlist * c->rlist *
+                                          c->rlist);
+            }
+            c->d.dense = ne >= (double)kLaneWalk ? 1 : 0;
         }
         for (int r = 0; r < c->R; ++r) {
             P[r].target_step = step0 + done + cn;

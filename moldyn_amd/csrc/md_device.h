@@ -202,6 +202,7 @@ struct DevState {
     CRec* crec[2];           // step records of the list path (null on the other paths)
     const uint8_t* slot_type;  // type of every slot (padding slots: the group's type)
     int32_t compact;         // the list path steps on crec (k_fin builds from crec)
+    int32_t dense;           // long lists expected: lanes walk their own lists, builds sort long lists in LDS
     double* pot;
     double* vir;
     int32_t* nbr;
@@ -288,6 +289,8 @@ constexpr int kBrickAtoms = 1024;
 constexpr int kBrickTarget = 900;  // expected staged particles a brick is sized for
 constexpr int kBrickOwn = 440;     // expected centre particles: one per thread of a 512-thread block
 constexpr int kListReg = 16;       // list entries kept (sorted) in registers while building
+constexpr int kBrickSeg = 16384;   // LDS entries for lists longer than kListReg (per brick)
+constexpr int kLaneWalk = 6;       // mean list entries per particle from which lanes walk their own lists
 
 // Cell grid of a fill: per axis floor(L / (r_list (1 + 1e-9))) cells in
 // [1, 1024], coarsened until the replica's cell budget holds.  Cell edges stay

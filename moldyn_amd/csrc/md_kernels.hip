@@ -59,16 +59,37 @@ __device__ __forceinline__ void list_close(const DevState& d, int64_t slot, int 
 }
 
 // A list under construction: the first kListReg entries stay sorted in
-// registers (compile-time indexed insertion), longer lists continue in global
-// memory.  close() stores the register part and the count.
+// registers (compile-time indexed insertion).  A longer list continues in
+// global memory (sorted insertion), or with SEG in a column of an LDS segment
+// the builder hands over (room for the list capacity): the register part is
+// copied there and later entries are appended, and
+// store() sorts the appended entries into place once, in a loop every lane
+// runs at the same time (a sorted insertion per entry would run whenever any
+// lane of the wave found a neighbour).  store() writes the register or LDS
+// part to global memory.
+template <bool SEG>
 struct RegList {
     int32_t v[kListReg];
     int n;
     int32_t* __restrict__ col;
     int64_t N;
     int cap;
+    int32_t* seg = nullptr;  // LDS column of cap entries at a stride of seg_stride (SEG)
+    int seg_stride = 0;
     __device__ __forceinline__ RegList(int32_t* c, int64_t stride, int capacity)
         : n(0), col(c), N(stride), cap(capacity) {}
+    // insertion sort of seg[kListReg, m) into the sorted seg[0, kListReg)
+    __device__ __forceinline__ void seg_sort(int m) {
+        for (int q = kListReg; q < m; ++q) {
+            const int32_t x = seg[q * seg_stride];
+            int p = q;
+            while (p > 0 && seg[(p - 1) * seg_stride] > x) {
+                seg[p * seg_stride] = seg[(p - 1) * seg_stride];
+                --p;
+            }
+            seg[p * seg_stride] = x;
+        }
+    }
     __device__ __forceinline__ void push(int32_t j) {
         if (n < kListReg) {
 #pragma unroll
@@ -79,6 +100,14 @@ struct RegList {
                     v[p] = keep ? v[p] : (here ? j : v[p > 0 ? p - 1 : 0]);
                 }
             }
+        } else if (SEG && seg) {
+            if (n == kListReg) {
+#pragma unroll
+                for (int p = 0; p < kListReg; ++p) seg[p * seg_stride] = v[p];
+            }
+            // the column holds cap entries; past them a list is only counted
+            // (list_close reports the overflow, the chunk is replayed)
+            if (n < cap) seg[n * seg_stride] = j;
         } else {
             if (n == kListReg) {
 #pragma unroll
@@ -94,6 +123,10 @@ struct RegList {
 #pragma unroll
             for (int p = 0; p < kListReg; ++p)
                 if (p < n && p < cap) col[(int64_t)p * N] = v[p];
+        } else if (SEG && seg) {
+            const int m = n < cap ? n : cap;
+            seg_sort(m);
+            for (int p = 0; p < m; ++p) col[(int64_t)p * N] = seg[p * seg_stride];
         }
     }
 };
@@ -165,6 +198,7 @@ __device__ __forceinline__ float min_image_f(float r, float b) {
     return r;
 }
 
+
 // LDS of a brick: one object per kernel, shared by the builders of both
 // record types.
 struct BrickLds {
@@ -176,6 +210,7 @@ struct BrickLds {
     uint16_t cidx[kBrickAtoms];  // centre particle k -> its staging index
     uint16_t hc[kBrickAtoms];    // staged particle -> its halo cell
     uint8_t ty[kBrickAtoms];
+    int32_t seg[kBrickSeg];  // long lists of the centre particles, one column each
 };
 __device__ __forceinline__ BrickLds& brick_lds() {
     __shared__ BrickLds l;
@@ -200,7 +235,8 @@ __device__ __forceinline__ int brick_count(const RepCtrl& C) {
 }
 
 // Bricks bi0, bi0 + stride, ... of a replica (all threads of the block).
-template <typename RT, int BS, bool MULTI>
+// SEG: lists longer than kListReg continue in LDS columns (dense systems).
+template <typename RT, int BS, bool MULTI, bool SEG>
 __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams& P,
                             const RepInfo& R, const RT* __restrict__ recs, int bi0, int stride) {
     const uint32_t ep = C.fill_epoch;
@@ -330,6 +366,7 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             if (MULTI) s_ty[a] = (uint8_t)rec_type(d, recs, j);
         }
         lds_barrier();
+        const int seg_cap = n_own > 0 ? kBrickSeg / n_own : 0;
         for (int k = threadIdx.x; k < n_own; k += BS) {
             const int a = s_cidx[k];
             const int h = s_hc[a];
@@ -367,7 +404,11 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             const int ti = MULTI ? (int)s_ty[a] : 0;
             const double xi = s_x[a], yi = s_y[a], zi = s_z[a];
             const float xf = s_xf[a], yf = s_yf[a], zf = s_zf[a];
-            RegList lst(d.nbr + i, N, d.list_cap);
+            RegList<SEG> lst(d.nbr + i, N, d.list_cap);
+            if (SEG && seg_cap >= d.list_cap && d.list_cap > kListReg) {
+                lst.seg = L.seg + k;
+                lst.seg_stride = n_own;
+            }
             auto scan = [&](int b0, int b1) {
                 for (int b2 = b0; b2 < b1; ++b2) {
                     // FP32 test first: it never rejects a pair the FP64 test keeps
@@ -429,7 +470,7 @@ constexpr int wave_scratch_bytes() {
 // loop, so the sums are bitwise those of one.  A neighbour's drift is
 // recomputed from its step record (position and kicked velocity) with the
 // operations its own thread performs.
-template <typename FT, bool MULTI>
+template <typename FT, bool MULTI, bool WALK>
 __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRec* __restrict__ src, char* wreg,
                                                     int n_mine, int32_t e0, int64_t s, double x1, double y1,
                                                     double z1, int ti, bool split, double lam, double dt,
@@ -464,7 +505,7 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
         tm = Acc<FT>::terms(dx, dy, dz, r2, pp);
         return true;
     };
-    if (!__ballot(n_mine > 1)) {
+    if (!WALK && !__ballot(n_mine > 1)) {
         // at most one entry per lane: every lane takes its own
         if (n_mine == 1) {
             Terms tm;
@@ -473,8 +514,19 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
         return acc;
     }
     const int incl = wave_scan_incl(n_mine);
-    const int my_off = incl - n_mine;
     const int total = __builtin_amdgcn_readlane(incl, W - 1);
+    if (WALK && (!__ballot(n_mine > 1) || total >= kLaneWalk * W)) {
+        // dense systems: at most one entry per lane, or long lists: each
+        // lane walks its own list; entry k of the wave's particles is one
+        // coalesced row of the column-major lists
+        for (int k = 0; k < n_mine; ++k) {
+            const int32_t j = k == 0 ? e0 : d.nbr[(int64_t)k * d.n_slots + s];
+            Terms tm;
+            if (pair(j, x1, y1, z1, tm)) acc.add_terms(tm);
+        }
+        return acc;
+    }
+    const int my_off = incl - n_mine;
     wave_lds_sync();  // the previous tile's rounds are done with the scratch
     ox[wl] = x1;
     oy[wl] = y1;
@@ -510,7 +562,7 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
 // entry of the next tile are loaded before the current tile is processed.
 // The last step of a chunk also writes the full records (x, v, F) and the
 // per-particle energies.
-template <typename FT, int BS, bool MULTI>
+template <typename FT, int BS, bool MULTI, bool DENSE>
 __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const GroupInfo G = d.groups[d.blk_group[b]];
@@ -651,7 +703,7 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
                 part[S_VMAX] = fmax(part[S_VMAX], (v1x * v1x + v1y * v1y) + v1z * v1z);
             }
         }
-        const Acc<FT> acc = wave_list_forces<FT, MULTI>(d, src, wreg, valid ? nn : 0, e0, s, x1, y1, z1, ti,
+        const Acc<FT> acc = wave_list_forces<FT, MULTI, DENSE>(d, src, wreg, valid ? nn : 0, e0, s, x1, y1, z1, ti,
                                                         split, lam, dt, bx, by, bz, p0);
         if (valid) {
             // integrator.rs:47-53 second half kick, barostat.rs:45-48 scale
@@ -760,7 +812,7 @@ __global__ __launch_bounds__(kFinBS) void k_build(DevState d) {
     const RepCtrl& C = ctrl_in(d, r);
     if ((C.flags & (F_BUILD | F_EXIT)) != F_BUILD) return;
     const RepInfo R = d.reps[r];
-    brick_build<Rec<FT>, kFinBS, MULTI>(d, C, d.params[r], R, (const Rec<FT>*)d.rec[C.build_src],
+    brick_build<Rec<FT>, kFinBS, MULTI, true>(d, C, d.params[r], R, (const Rec<FT>*)d.rec[C.build_src],
                                         b - R.bld_first, R.bld_count);
 }
 
@@ -1380,7 +1432,7 @@ __device__ void prof_fold(const DevState& d, int64_t launch, bool drift) {
     }
 }
 
-template <typename FT, bool MULTI, bool BUILD, bool REPS>
+template <typename FT, bool MULTI, bool BUILD, bool REPS, bool DENSE>
 __global__ __launch_bounds__(kFinBS) void k_fin(DevState d, int mode) {
     const int b = blockIdx.x;
     const uint64_t t_start = prof_now(d);
@@ -1396,9 +1448,9 @@ __global__ __launch_bounds__(kFinBS) void k_fin(DevState d, int mode) {
         if ((C.flags & (F_BUILD | F_EXIT)) == F_BUILD) {
             const RepInfo R = d.reps[0];
             if (d.compact)
-                brick_build<CRec, kFinBS, MULTI>(d, C, d.params[0], R, d.crec[C.build_src], b, d.n_bld);
+                brick_build<CRec, kFinBS, MULTI, DENSE>(d, C, d.params[0], R, d.crec[C.build_src], b, d.n_bld);
             else
-                brick_build<Rec<FT>, kFinBS, MULTI>(d, C, d.params[0], R, (const Rec<FT>*)d.rec[C.build_src], b,
+                brick_build<Rec<FT>, kFinBS, MULTI, DENSE>(d, C, d.params[0], R, (const Rec<FT>*)d.rec[C.build_src], b,
                                                     d.n_bld);
             launch = C.launches;
         }
@@ -1424,7 +1476,7 @@ __global__ __launch_bounds__(kFinBS) void k_fin(DevState d, int mode) {
             if (threadIdx.x == 0) s_nwork = 0;
             lds_barrier();
             block_exclusive_scan<kFinBS>(s_rb, n);
-            const int tot = s_rb[n];
+            const int tot = __builtin_amdgcn_readfirstlane(s_rb[n]);  // uniform: scalar registers
             // replicas holding bricks of this block (in any order: each
             // brick's lists depend on the brick alone)
             if ((int)threadIdx.x < n) {
@@ -1433,19 +1485,19 @@ __global__ __launch_bounds__(kFinBS) void k_fin(DevState d, int mode) {
                 if (g0 < hi) s_work[atomicAdd(&s_nwork, 1)] = threadIdx.x;
             }
             lds_barrier();
-            const int nwork = s_nwork;
+            const int nwork = __builtin_amdgcn_readfirstlane(s_nwork);
             for (int w = 0; w < nwork; ++w) {
-                const int idx = s_work[w];
-                const int64_t lo = base + s_rb[idx];
+                const int idx = __builtin_amdgcn_readfirstlane(s_work[w]);
+                const int64_t lo = base + __builtin_amdgcn_readfirstlane(s_rb[idx]);
                 const int r = r0 + idx;
                 const int bi0 = (int)(((b - lo % nb) % nb + nb) % nb);
                 const RepCtrl& C = ctrl_in(d, r);
                 const RepInfo R = d.reps[r];
                 if (d.compact)
-                    brick_build<CRec, kFinBS, MULTI>(d, C, d.params[r], R, d.crec[C.build_src], bi0, nb);
+                    brick_build<CRec, kFinBS, MULTI, DENSE>(d, C, d.params[r], R, d.crec[C.build_src], bi0, nb);
                 else
-                    brick_build<Rec<FT>, kFinBS, MULTI>(d, C, d.params[r], R, (const Rec<FT>*)d.rec[C.build_src],
-                                                        bi0, nb);
+                    brick_build<Rec<FT>, kFinBS, MULTI, DENSE>(d, C, d.params[r], R,
+                                                               (const Rec<FT>*)d.rec[C.build_src], bi0, nb);
                 launch = C.launches;
             }
             base += tot;
@@ -1523,9 +1575,24 @@ __global__ __launch_bounds__(256) void k_prof_init(DevState d) {
 
 hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st) {
     const bool multi = d.n_types > 1;
-#define L(B) MD_LAUNCH_FT_MULTI(k_step, B)
+#define KS(FT, B, M)                                                                        \
+    do {                                                                                    \
+        if (d.dense)                                                                        \
+            hipLaunchKernelGGL((k_step<FT, B, M, true>), dim3(d.n_blocks), dim3(B), 0, st, d);  \
+        else                                                                                \
+            hipLaunchKernelGGL((k_step<FT, B, M, false>), dim3(d.n_blocks), dim3(B), 0, st, d); \
+    } while (0)
+#define L(B)                           \
+    if (fp32) {                        \
+        if (multi) KS(float, B, true);  \
+        else KS(float, B, false);       \
+    } else {                           \
+        if (multi) KS(double, B, true); \
+        else KS(double, B, false);      \
+    }
     MD_BS_SWITCH(bs, L)
 #undef L
+#undef KS
     return hipGetLastError();
 }
 
@@ -1572,12 +1639,19 @@ hipError_t launch_finalize(const DevState& d, int mode, bool build, bool fp32, h
     if (build && mode == M_STEP && d.n_bld > 0) {
         const int grid = d.n_bld > d.n_reps ? d.n_bld : d.n_reps;
         const bool reps = d.n_reps > 1;
-#define KF(FT, M)                                                                                    \
-    do {                                                                                             \
-        if (reps)                                                                                    \
-            hipLaunchKernelGGL((k_fin<FT, M, true, true>), dim3(grid), dim3(kFinBS), 0, st, d, mode);  \
-        else                                                                                         \
-            hipLaunchKernelGGL((k_fin<FT, M, true, false>), dim3(grid), dim3(kFinBS), 0, st, d, mode); \
+#define KF2(FT, M, DN)                                                                                \
+    do {                                                                                               \
+        if (reps)                                                                                      \
+            hipLaunchKernelGGL((k_fin<FT, M, true, true, DN>), dim3(grid), dim3(kFinBS), 0, st, d, mode);  \
+        else                                                                                           \
+            hipLaunchKernelGGL((k_fin<FT, M, true, false, DN>), dim3(grid), dim3(kFinBS), 0, st, d, mode); \
+    } while (0)
+#define KF(FT, M)                 \
+    do {                          \
+        if (d.dense)              \
+            KF2(FT, M, true);     \
+        else                      \
+            KF2(FT, M, false);    \
     } while (0)
         if (fp32) {
             if (multi) KF(float, true);
@@ -1587,8 +1661,10 @@ hipError_t launch_finalize(const DevState& d, int mode, bool build, bool fp32, h
             else KF(double, false);
         }
 #undef KF
+#undef KF2
     } else {
-        hipLaunchKernelGGL((k_fin<double, false, false, false>), dim3(d.n_reps), dim3(kFinBS), 0, st, d, mode);
+        hipLaunchKernelGGL((k_fin<double, false, false, false, false>), dim3(d.n_reps), dim3(kFinBS), 0, st, d,
+                           mode);
     }
     return hipGetLastError();
 }
