@@ -58,15 +58,17 @@ __device__ __forceinline__ void list_close(const DevState& d, int64_t slot, int 
     }
 }
 
-// A list under construction: the first kListReg entries stay sorted in
-// registers (compile-time indexed insertion).  A longer list continues in
-// global memory (sorted insertion), or with SEG in a column of an LDS segment
-// the builder hands over (room for the list capacity): the register part is
-// copied there and later entries are appended, and
-// store() sorts the appended entries into place once, in a loop every lane
-// runs at the same time (a sorted insertion per entry would run whenever any
-// lane of the wave found a neighbour).  store() writes the register or LDS
-// part to global memory.
+// A list under construction.  Without an LDS column, the first kListReg
+// entries stay sorted in registers (compile-time indexed insertion) and a
+// longer list continues in global memory (sorted insertion); store() writes
+// the register part.  With SEG and a column the builder hands over (room for
+// the list capacity, at a stride of seg_stride; dense systems), the scan
+// appends the staging index of every candidate the FP32 test accepts, and
+// the builder then runs the FP64 test, compacts and sorts the column once,
+// in loops every lane runs at the same time: in a dense brick some lane of
+// the wave accepts a candidate at nearly every step of the scan, and work
+// per accepted entry inside the scan would make the whole wave execute it
+// each time.
 template <bool SEG>
 struct RegList {
     int32_t v[kListReg];
@@ -78,9 +80,15 @@ struct RegList {
     int seg_stride = 0;
     __device__ __forceinline__ RegList(int32_t* c, int64_t stride, int capacity)
         : n(0), col(c), N(stride), cap(capacity) {}
-    // insertion sort of seg[kListReg, m) into the sorted seg[0, kListReg)
-    __device__ __forceinline__ void seg_sort(int m) {
-        for (int q = kListReg; q < m; ++q) {
+    // SEG: append a candidate's staging index to the column (past the
+    // capacity a list is only counted: list_close reports the overflow)
+    __device__ __forceinline__ void push_raw(int32_t b2) {
+        if (n < cap) seg[n * seg_stride] = b2;
+        ++n;
+    }
+    // SEG: sort the column's first m entries (slot ids) and store them
+    __device__ __forceinline__ void store_seg(int m) {
+        for (int q = 1; q < m; ++q) {
             const int32_t x = seg[q * seg_stride];
             int p = q;
             while (p > 0 && seg[(p - 1) * seg_stride] > x) {
@@ -89,6 +97,7 @@ struct RegList {
             }
             seg[p * seg_stride] = x;
         }
+        for (int p = 0; p < m; ++p) col[(int64_t)p * N] = seg[p * seg_stride];
     }
     __device__ __forceinline__ void push(int32_t j) {
         if (n < kListReg) {
@@ -100,14 +109,6 @@ struct RegList {
                     v[p] = keep ? v[p] : (here ? j : v[p > 0 ? p - 1 : 0]);
                 }
             }
-        } else if (SEG && seg) {
-            if (n == kListReg) {
-#pragma unroll
-                for (int p = 0; p < kListReg; ++p) seg[p * seg_stride] = v[p];
-            }
-            // the column holds cap entries; past them a list is only counted
-            // (list_close reports the overflow, the chunk is replayed)
-            if (n < cap) seg[n * seg_stride] = j;
         } else {
             if (n == kListReg) {
 #pragma unroll
@@ -123,10 +124,6 @@ struct RegList {
 #pragma unroll
             for (int p = 0; p < kListReg; ++p)
                 if (p < n && p < cap) col[(int64_t)p * N] = v[p];
-        } else if (SEG && seg) {
-            const int m = n < cap ? n : cap;
-            seg_sort(m);
-            for (int p = 0; p < m; ++p) col[(int64_t)p * N] = seg[p * seg_stride];
         }
     }
 };
@@ -405,7 +402,7 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
             const double xi = s_x[a], yi = s_y[a], zi = s_z[a];
             const float xf = s_xf[a], yf = s_yf[a], zf = s_zf[a];
             RegList<SEG> lst(d.nbr + i, N, d.list_cap);
-            if (SEG && seg_cap >= d.list_cap && d.list_cap > kListReg) {
+            if (SEG && seg_cap >= d.list_cap) {
                 lst.seg = L.seg + k;
                 lst.seg_stride = n_own;
             }
@@ -419,6 +416,10 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
                     const int32_t j = s_id[b2];
                     if (j == i) continue;
                     if (MULTI && (int)s_ty[b2] < ti) continue;
+                    if (SEG && lst.seg) {
+                        lst.push_raw(b2);  // the FP64 test runs once the scan is done
+                        continue;
+                    }
                     const double dx = min_image(s_x[b2] - xi, bx);
                     const double dy = min_image(s_y[b2] - yi, by);
                     const double dz = min_image(s_z[b2] - zi, bz);
@@ -436,8 +437,27 @@ __device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams
                     if (zb1 > zb0) scan(s_off[row + zb0], s_off[row + zb1]);
                 }
             }
-            lst.store();
-            list_close(d, i, lst.n);
+            if (SEG && lst.seg) {
+                // the FP64 test (the reference's minimum image) over the
+                // FP32-accepted candidates, in a loop every lane runs at once
+                int kept = 0;
+                const int m = lst.n < d.list_cap ? lst.n : d.list_cap;
+                for (int q = 0; q < m; ++q) {
+                    const int b2 = lst.seg[q * lst.seg_stride];
+                    const double dx = min_image(s_x[b2] - xi, bx);
+                    const double dy = min_image(s_y[b2] - yi, by);
+                    const double dz = min_image(s_z[b2] - zi, bz);
+                    const double r2 = (dx * dx + dy * dy) + dz * dz;
+                    if (r2 <= rl2) lst.seg[kept++ * lst.seg_stride] = s_id[b2];
+                }
+                lst.store_seg(kept);
+                // more FP32-accepted candidates than the capacity counts as
+                // an overflow (the chunk is replayed with a larger one)
+                list_close(d, i, lst.n > d.list_cap ? lst.n : kept);
+            } else {
+                lst.store();
+                list_close(d, i, lst.n);
+            }
         }
         lds_barrier();
     }
