@@ -481,22 +481,27 @@ constexpr int wave_scratch_bytes() {
 }
 
 // The list forces of a wave's particles (potential.rs:158-216 over the Verlet
-// lists at the drifted positions).  With at most one entry per particle each
-// lane takes its own; otherwise the wave's entries are spread over its 64
-// lanes, one entry per lane and round, so the index and record loads
-// of a round go out together instead of one entry at a time per particle;
-// each lane computes its pair's terms into the wave's LDS scratch, then every
-// particle adds its own terms in list order: the order of a per-particle
-// loop, so the sums are bitwise those of one.  A neighbour's drift is
-// recomputed from its step record (position and kicked velocity) with the
-// operations its own thread performs.
+// lists at the drifted positions).  In sparse systems (!WALK) each lane takes
+// its particle's first entry, whose record the caller gathered a tile ahead
+// (o0, type tj0); the remaining entries of the wave are spread over its 64
+// lanes, one entry per lane and round, so the index and record loads of a
+// round go out together instead of one entry at a time per particle.  In
+// dense systems (WALK) the rounds cover every entry (first entry e0), or,
+// with long lists, each lane walks its own list.  Each lane computes its
+// pair's terms into the wave's LDS scratch, then every particle adds its own
+// terms in list order: the order of a per-particle loop, so the sums are
+// bitwise those of one.  A neighbour's drift is recomputed from its step
+// record (position and kicked velocity) with the operations its own thread
+// performs.
 template <typename FT, bool MULTI, bool WALK>
 __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRec* __restrict__ src, char* wreg,
-                                                    int n_mine, int32_t e0, int64_t s, double x1, double y1,
-                                                    double z1, int ti, bool split, double lam, double dt,
-                                                    double bx, double by, double bz, const PairParam& p0) {
+                                                    int n_mine, const CRec& o0, int tj0, int32_t e0, int64_t s,
+                                                    double x1, double y1, double z1, int ti, bool split, double lam,
+                                                    double dt, double bx, double by, double bz,
+                                                    const PairParam& p0) {
     typedef typename Acc<FT>::Terms Terms;
     constexpr int W = 64;
+    constexpr int K0 = WALK ? 0 : 1;  // first list position the rounds cover
     double* ox = (double*)wreg;
     double* oy = ox + W;
     double* oz = oy + W;
@@ -507,9 +512,7 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
     uint8_t* pf = (uint8_t*)(omap + W);
     const int wl = threadIdx.x & (W - 1);
     Acc<FT> acc;
-    auto pair = [&](int32_t j, double xi, double yi, double zi, Terms& tm) -> bool {
-        const CRec o = src[j];
-        const int tj = MULTI ? (int)d.slot_type[j] : 0;
+    auto pair = [&](const CRec& o, int tj, double xi, double yi, double zi, Terms& tm) -> bool {
         double xj = o.x, yj = o.y, zj = o.z;
         if (!split) {
             xj = wrap1(xj + (o.ux * lam) * dt, bx);
@@ -525,28 +528,30 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
         tm = Acc<FT>::terms(dx, dy, dz, r2, pp);
         return true;
     };
-    if (!WALK && !__ballot(n_mine > 1)) {
-        // at most one entry per lane: every lane takes its own
-        if (n_mine == 1) {
-            Terms tm;
-            if (pair(e0, x1, y1, z1, tm)) acc.add_terms(tm);
-        }
-        return acc;
+    auto gathered = [&](int32_t j, double xi, double yi, double zi, Terms& tm) -> bool {
+        const CRec o = src[j];  // one copy: the record loads as whole 16-byte words
+        return pair(o, MULTI ? (int)d.slot_type[j] : 0, xi, yi, zi, tm);
+    };
+    if (!WALK && n_mine >= 1) {
+        Terms tm;
+        if (pair(o0, tj0, x1, y1, z1, tm)) acc.add_terms(tm);
     }
-    const int incl = wave_scan_incl(n_mine);
+    const int n_rest = n_mine > K0 ? n_mine - K0 : 0;
+    if (!WALK && !__ballot(n_rest > 0)) return acc;
+    const int incl = wave_scan_incl(n_rest);
     const int total = __builtin_amdgcn_readlane(incl, W - 1);
     if (WALK && (!__ballot(n_mine > 1) || total >= kLaneWalk * W)) {
-        // dense systems: at most one entry per lane, or long lists: each
-        // lane walks its own list; entry k of the wave's particles is one
-        // coalesced row of the column-major lists
+        // at most one entry per lane, or long lists: each lane walks its own
+        // list; entry k of the wave's particles is one coalesced row of the
+        // column-major lists
         for (int k = 0; k < n_mine; ++k) {
             const int32_t j = k == 0 ? e0 : d.nbr[(int64_t)k * d.n_slots + s];
             Terms tm;
-            if (pair(j, x1, y1, z1, tm)) acc.add_terms(tm);
+            if (gathered(j, x1, y1, z1, tm)) acc.add_terms(tm);
         }
         return acc;
     }
-    const int my_off = incl - n_mine;
+    const int my_off = incl - n_rest;
     wave_lds_sync();  // the previous tile's rounds are done with the scratch
     ox[wl] = x1;
     oy[wl] = y1;
@@ -556,16 +561,16 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
     for (int c0 = 0; c0 < total; c0 += W) {
         // this round's entries [c0, c0 + W): the owner marks its own
         const int a0 = my_off > c0 ? my_off : c0;
-        const int a1 = my_off + n_mine < c0 + W ? my_off + n_mine : c0 + W;
-        for (int a = a0; a < a1; ++a) omap[a - c0] = ((uint32_t)wl << 16) | (uint32_t)(a - my_off);
+        const int a1 = my_off + n_rest < c0 + W ? my_off + n_rest : c0 + W;
+        for (int a = a0; a < a1; ++a) omap[a - c0] = ((uint32_t)wl << 16) | (uint32_t)(a - my_off + K0);
         wave_lds_sync();
         Terms tm{};
         uint8_t ok = 0;
         if (c0 + wl < total) {
             const uint32_t om = omap[wl];
             const int own = (int)(om >> 16), k = (int)(om & 0xffffu);
-            const int32_t j = k == 0 ? oe[own] : d.nbr[(int64_t)k * d.n_slots + os[own]];
-            ok = pair(j, ox[own], oy[own], oz[own], tm) ? 1 : 0;
+            const int32_t j = (WALK && k == 0) ? oe[own] : d.nbr[(int64_t)k * d.n_slots + os[own]];
+            ok = gathered(j, ox[own], oy[own], oz[own], tm) ? 1 : 0;
         }
         pt[wl] = tm;
         pf[wl] = ok;
@@ -579,7 +584,9 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
 
 // One launch of the step kernel over the block's tiles, on the step records
 // (position and kicked velocity).  The record, list length and first list
-// entry of the next tile are loaded before the current tile is processed.
+// entry of the next tile are loaded before the current tile is processed; in
+// sparse systems the record of that first entry is gathered as the tile
+// starts.
 // The last step of a chunk also writes the full records (x, v, F) and the
 // per-particle energies.
 template <typename FT, int BS, bool MULTI, bool DENSE>
@@ -669,7 +676,9 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     char* const wreg = (char*)s_ws + (threadIdx.x >> 6) * WS;
     u4 pre[RQ];
     int nn = 0;
-    int32_t e0 = 0;  // first list entry of the thread's particle (entry 0 of every slot exists)
+    // the step record (and type) of the first list entry of the thread's
+    // particle, gathered a tile ahead (entry 0 of every slot exists)
+    int32_t e0 = 0;
     if (tr.t0 < tr.t1) {
         const u4* g = (const u4*)(src + G.start + tr.t0 * BS);
 #pragma unroll
@@ -680,6 +689,17 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     for (int64_t t = tr.t0; t < tr.t1; ++t) {
         const int64_t s = base + t * BS;
         const bool valid = t * BS + lane < G.count;
+        // sparse systems: the record of the particle's first list entry
+        CRec o0;
+        int tj0 = 0;
+        if constexpr (!DENSE) {
+            if (valid && nn > 0) {
+                o0 = src[e0];
+                if (MULTI) tj0 = d.slot_type[e0];
+            } else {
+                o0.x = o0.y = o0.z = o0.ux = o0.uy = o0.uz = 0.0;
+            }
+        }
 #pragma unroll
         for (int q = 0; q < RQ; ++q) s_tile[q * BS + lane] = pre[q];
         lds_barrier();  // stores of the previous tile may still be in flight
@@ -723,8 +743,8 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
                 part[S_VMAX] = fmax(part[S_VMAX], (v1x * v1x + v1y * v1y) + v1z * v1z);
             }
         }
-        const Acc<FT> acc = wave_list_forces<FT, MULTI, DENSE>(d, src, wreg, valid ? nn : 0, e0, s, x1, y1, z1, ti,
-                                                        split, lam, dt, bx, by, bz, p0);
+        const Acc<FT> acc = wave_list_forces<FT, MULTI, DENSE>(d, src, wreg, valid ? nn : 0, o0, tj0, e0, s, x1, y1,
+                                                               z1, ti, split, lam, dt, bx, by, bz, p0);
         if (valid) {
             // integrator.rs:47-53 second half kick, barostat.rs:45-48 scale
             const double vx = v1x + (double)acc.fx * c;
