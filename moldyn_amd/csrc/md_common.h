@@ -132,7 +132,7 @@ __device__ __forceinline__ void block_store(const double (&v)[NS], double* out, 
 
 // In-place exclusive scan of a[0..n) (n <= kBrickCells); a[n] receives the total.
 template <int BS>
-__device__ void block_exclusive_scan(int* a, int n) {
+__device__ __forceinline__ void block_exclusive_scan(int* a, int n) {
     constexpr int NW = BS / 64;
     __shared__ int wsum[NW];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;

@@ -234,7 +234,7 @@ __device__ __forceinline__ int brick_count(const RepCtrl& C) {
 // Bricks bi0, bi0 + stride, ... of a replica (all threads of the block).
 // SEG: lists longer than kListReg continue in LDS columns (dense systems).
 template <typename RT, int BS, bool MULTI, bool SEG>
-__device__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams& P,
+__device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C, const RepParams& P,
                             const RepInfo& R, const RT* __restrict__ recs, int bi0, int stride) {
     const uint32_t ep = C.fill_epoch;
     const int buf = ep & 1;
@@ -473,11 +473,11 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
-// LDS a wave's list forces use (owner positions, slots and first entries,
-// pair terms and flags, the owner map of a round).
+// LDS a wave's list forces use (owner positions and slots, pair terms and
+// flags, the owner map of a round).
 template <typename FT>
 constexpr int wave_scratch_bytes() {
-    return 64 * (3 * (int)sizeof(double) + (int)sizeof(typename Acc<FT>::Terms) + 3 * (int)sizeof(int32_t) + 1);
+    return 64 * (3 * (int)sizeof(double) + (int)sizeof(typename Acc<FT>::Terms) + 2 * (int)sizeof(int32_t) + 1);
 }
 
 // The list forces of a wave's particles (potential.rs:158-216 over the Verlet
@@ -486,7 +486,7 @@ constexpr int wave_scratch_bytes() {
 // (o0, type tj0); the remaining entries of the wave are spread over its 64
 // lanes, one entry per lane and round, so the index and record loads of a
 // round go out together instead of one entry at a time per particle.  In
-// dense systems (WALK) the rounds cover every entry (first entry e0), or,
+// dense systems (WALK) the rounds cover every entry, or,
 // with long lists, each lane walks its own list.  Each lane computes its
 // pair's terms into the wave's LDS scratch, then every particle adds its own
 // terms in list order: the order of a per-particle loop, so the sums are
@@ -507,8 +507,7 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
     double* oz = oy + W;
     Terms* pt = (Terms*)(oz + W);
     int32_t* os = (int32_t*)(pt + W);
-    int32_t* oe = os + W;                  // each lane's first list entry
-    uint32_t* omap = (uint32_t*)(oe + W);  // entry of a round -> owner lane << 16 | list position
+    uint32_t* omap = (uint32_t*)(os + W);  // entry of a round -> owner lane << 16 | list position
     uint8_t* pf = (uint8_t*)(omap + W);
     const int wl = threadIdx.x & (W - 1);
     Acc<FT> acc;
@@ -557,7 +556,6 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
     oy[wl] = y1;
     oz[wl] = z1;
     os[wl] = (int32_t)s;
-    oe[wl] = e0;
     for (int c0 = 0; c0 < total; c0 += W) {
         // this round's entries [c0, c0 + W): the owner marks its own
         const int a0 = my_off > c0 ? my_off : c0;
@@ -569,7 +567,10 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
         if (c0 + wl < total) {
             const uint32_t om = omap[wl];
             const int own = (int)(om >> 16), k = (int)(om & 0xffffu);
-            const int32_t j = (WALK && k == 0) ? oe[own] : d.nbr[(int64_t)k * d.n_slots + os[own]];
+            // the index from the list rows (row 0 too): a choice between the
+            // first entry in LDS and a row in global memory would compile to
+            // one flat load, which waits on both memory counters
+            const int32_t j = d.nbr[(int64_t)k * d.n_slots + os[own]];
             ok = gathered(j, ox[own], oy[own], oz[own], tm) ? 1 : 0;
         }
         pt[wl] = tm;
@@ -1328,7 +1329,7 @@ __device__ __forceinline__ void copy_words(void* dst, const void* src, int bytes
 // thread advances the controls on register copies of the LDS-staged inputs,
 // and the block writes the control block of the next launch.  Returns the
 // launch index when the replica did work in this launch, else -1.
-__device__ int64_t finalize_replica(const DevState& d, int r, int mode) {
+__device__ __forceinline__ int64_t finalize_replica(const DevState& d, int r, int mode) {
     constexpr int BS = kFinBS;
     const int T = d.n_types;
     const int64_t g0 = (int64_t)r * T;  // groups are replica-major
@@ -1430,7 +1431,7 @@ __device__ int64_t finalize_replica(const DevState& d, int r, int mode) {
 
 // Fold the per-block records of this launch's per-slot kernels into the
 // ring entry of the launch (one block).
-__device__ void prof_fold(const DevState& d, int64_t launch, bool drift) {
+__device__ __forceinline__ void prof_fold(const DevState& d, int64_t launch, bool drift) {
     constexpr int BS = kFinBS;
     __shared__ unsigned long long s_mn[BS / 64], s_mx[BS / 64], s_en[BS / 64];
     for (int slot = 0; slot < (drift ? 2 : 1); ++slot) {
