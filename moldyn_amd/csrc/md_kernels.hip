@@ -187,12 +187,9 @@ __device__ void build_list_global(const DevState& d, const RepCtrl& C, const Rep
 
 // FP32 minimum image of one component (the prefilter of the list build).
 __device__ __forceinline__ float min_image_f(float r, float b) {
-    if (r < -0.5f * b) {
-        r += b;
-    } else if (r > 0.5f * b) {
-        r -= b;
-    }
-    return r;
+    // selects, not branches: the candidate loop stays one basic block
+    const float h = 0.5f * b;
+    return r < -h ? r + b : (r > h ? r - b : r);
 }
 
 
@@ -202,7 +199,7 @@ struct BrickLds {
     int off[kBrickCells + 1];  // staged particles before halo cell h
     int cen[kBrickCells + 1];  // the brick's own (centre) particles before cell h
     double x[kBrickAtoms], y[kBrickAtoms], z[kBrickAtoms];
-    float xf[kBrickAtoms], yf[kBrickAtoms], zf[kBrickAtoms];
+    float4 pf[kBrickAtoms];  // FP32 copy of the position (w unused): one 16-byte read per candidate
     int32_t id[kBrickAtoms];
     uint16_t cidx[kBrickAtoms];  // centre particle k -> its staging index
     uint16_t hc[kBrickAtoms];    // staged particle -> its halo cell
@@ -272,9 +269,7 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
     double* const s_x = L.x;
     double* const s_y = L.y;
     double* const s_z = L.z;
-    float* const s_xf = L.xf;
-    float* const s_yf = L.yf;
-    float* const s_zf = L.zf;
+    float4* const s_pf = L.pf;
     int32_t* const s_id = L.id;
     uint16_t* const s_hc = L.hc;
     uint8_t* const s_ty = L.ty;
@@ -356,9 +351,7 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             s_x[a] = o.x;
             s_y[a] = o.y;
             s_z[a] = o.z;
-            s_xf[a] = (float)o.x;
-            s_yf[a] = (float)o.y;
-            s_zf[a] = (float)o.z;
+            s_pf[a] = make_float4((float)o.x, (float)o.y, (float)o.z, 0.0f);
             s_id[a] = j;
             if (MULTI) s_ty[a] = (uint8_t)rec_type(d, recs, j);
         }
@@ -400,7 +393,8 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             const int32_t i = s_id[a];
             const int ti = MULTI ? (int)s_ty[a] : 0;
             const double xi = s_x[a], yi = s_y[a], zi = s_z[a];
-            const float xf = s_xf[a], yf = s_yf[a], zf = s_zf[a];
+            const float4 pi = s_pf[a];
+            const float xf = pi.x, yf = pi.y, zf = pi.z;
             RegList<SEG> lst(d.nbr + i, N, d.list_cap);
             if (SEG && seg_cap >= d.list_cap) {
                 lst.seg = L.seg + k;
@@ -409,9 +403,10 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             auto scan = [&](int b0, int b1) {
                 for (int b2 = b0; b2 < b1; ++b2) {
                     // FP32 test first: it never rejects a pair the FP64 test keeps
-                    const float ex = min_image_f(s_xf[b2] - xf, bxf);
-                    const float ey = min_image_f(s_yf[b2] - yf, byf);
-                    const float ez = min_image_f(s_zf[b2] - zf, bzf);
+                    const float4 pj = s_pf[b2];
+                    const float ex = min_image_f(pj.x - xf, bxf);
+                    const float ey = min_image_f(pj.y - yf, byf);
+                    const float ez = min_image_f(pj.z - zf, bzf);
                     if (!((ex * ex + ey * ey) + ez * ez <= rl2f)) continue;
                     const int32_t j = s_id[b2];
                     if (j == i) continue;
