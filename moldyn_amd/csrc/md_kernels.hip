@@ -193,6 +193,14 @@ __device__ __forceinline__ float min_image_f(float r, float b) {
 }
 
 
+// The reference's minimum image (potential.rs:186-200) as selects, for the
+// list builder's exact tests: the same values as min_image, without a branch
+// per component in the candidate loops.
+__device__ __forceinline__ double min_image_sel(double r, double b) {
+    const double h = b / 2.0;
+    return r < -h ? r + b : (r > h ? r - b : r);
+}
+
 // LDS of a brick: one object per kernel, shared by the builders of both
 // record types.
 struct BrickLds {
@@ -415,9 +423,9 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                         lst.push_raw(b2);  // the FP64 test runs once the scan is done
                         continue;
                     }
-                    const double dx = min_image(s_x[b2] - xi, bx);
-                    const double dy = min_image(s_y[b2] - yi, by);
-                    const double dz = min_image(s_z[b2] - zi, bz);
+                    const double dx = min_image_sel(s_x[b2] - xi, bx);
+                    const double dy = min_image_sel(s_y[b2] - yi, by);
+                    const double dz = min_image_sel(s_z[b2] - zi, bz);
                     const double r2 = (dx * dx + dy * dy) + dz * dz;
                     if (!(r2 <= rl2)) continue;
                     lst.push(j);
@@ -439,9 +447,9 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                 const int m = lst.n < d.list_cap ? lst.n : d.list_cap;
                 for (int q = 0; q < m; ++q) {
                     const int b2 = lst.seg[q * lst.seg_stride];
-                    const double dx = min_image(s_x[b2] - xi, bx);
-                    const double dy = min_image(s_y[b2] - yi, by);
-                    const double dz = min_image(s_z[b2] - zi, bz);
+                    const double dx = min_image_sel(s_x[b2] - xi, bx);
+                    const double dy = min_image_sel(s_y[b2] - yi, by);
+                    const double dz = min_image_sel(s_z[b2] - zi, bz);
                     const double r2 = (dx * dx + dy * dy) + dz * dz;
                     if (r2 <= rl2) lst.seg[kept++ * lst.seg_stride] = s_id[b2];
                 }
