@@ -329,28 +329,38 @@ __host__ __device__ inline void grid_dims(const double* gb, double rlist, int64_
 // grid's mean occupancy.
 __host__ __device__ inline void brick_dims(const int* gd, int64_t n_atoms, int* edge, int* whole) {
     const double per_cell = (double)n_atoms / ((double)gd[0] * gd[1] * gd[2]);
-    int best = 1;
+    const int g0 = gd[0], g1 = gd[1], g2 = gd[2];
+    // edges e (cubic), then e with the last one or two axes a cell longer:
+    // in dense systems a cubic brick that fits the staging capacity leaves
+    // most threads without a centre particle, and one cell more along an
+    // axis or two doubles the centre (scalars only: the device control
+    // update keeps its state in registers)
+    int b0 = 1, b1 = 1, b2 = 1;
+    double best_own = -1.0, best_halo = 0.0;
     for (int e = 1; e <= kBrickEdge; ++e) {
-        double halo = 1.0, own = 1.0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            halo *= (gd[k] <= e + 2) ? gd[k] : (e + 2);
-            own *= (gd[k] <= e + 2) ? gd[k] : e;
+        for (int x = 0; x < 3; ++x) {
+            const int e0 = e, e1 = e + (x >= 2 ? 1 : 0), e2 = e + (x >= 1 ? 1 : 0);
+            const double halo = (double)(g0 <= e0 + 2 ? g0 : e0 + 2) * (double)(g1 <= e1 + 2 ? g1 : e1 + 2) *
+                                (double)(g2 <= e2 + 2 ? g2 : e2 + 2);
+            const double own = (double)(g0 <= e0 + 2 ? g0 : e0) * (double)(g1 <= e1 + 2 ? g1 : e1) *
+                               (double)(g2 <= e2 + 2 ? g2 : e2);
+            if (halo <= (double)kBrickCells && halo * per_cell <= (double)kBrickTarget &&
+                own * per_cell <= (double)kBrickOwn && (own > best_own || (own == best_own && halo < best_halo))) {
+                best_own = own;
+                best_halo = halo;
+                b0 = e0;
+                b1 = e1;
+                b2 = e2;
+            }
         }
-        if (halo <= (double)kBrickCells && halo * per_cell <= (double)kBrickTarget &&
-            own * per_cell <= (double)kBrickOwn)
-            best = e;
     }
     int w = 0;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        if (gd[k] <= best + 2) {
-            edge[k] = gd[k];
-            w |= 1 << k;
-        } else {
-            edge[k] = best;
-        }
-    }
+    edge[0] = g0 <= b0 + 2 ? g0 : b0;
+    edge[1] = g1 <= b1 + 2 ? g1 : b1;
+    edge[2] = g2 <= b2 + 2 ? g2 : b2;
+    w |= g0 <= b0 + 2 ? 1 : 0;
+    w |= g1 <= b1 + 2 ? 2 : 0;
+    w |= g2 <= b2 + 2 ? 4 : 0;
     *whole = w;
 }
 

@@ -1124,7 +1124,7 @@ __device__ __forceinline__ void fill_geometry(double (&gbox)[3], int32_t (&gdim)
 
 // Geometry of the next grid fill (buffer ins_epoch & 1), chosen with fixed
 // indices so that the control block can live in registers.
-__device__ void set_fill(RepCtrl& C, const RepParams& P, const RepInfo& R, const double* gb) {
+__device__ __forceinline__ void set_fill(RepCtrl& C, const RepParams& P, const RepInfo& R, const double* gb) {
     C.ins_epoch = C.fill_epoch + 1u;
     if (C.ins_epoch & 1)
         fill_geometry(C.gbox[1], C.gdim[1], C.bedge[1], C.bwhole[1], P, R, gb);
@@ -1134,7 +1134,7 @@ __device__ void set_fill(RepCtrl& C, const RepParams& P, const RepInfo& R, const
 
 // Advance a replica's controls after a launch (M_STEP) or prepare the first
 // step of a call (M_PRIME).  S holds the group sums of the replica's types.
-__device__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const GroupInfo* G,
+__device__ __forceinline__ void advance(RepCtrl& C, const RepParams& P, const RepInfo& R, const GroupInfo* G,
                         const double* S, int mode) {
     const int T = R.n_types;
     const double dt = P.dt;
@@ -1298,7 +1298,7 @@ __device__ __forceinline__ bool finalize_reduces(int mode, uint32_t f0) {
 // Sum rows [lo, hi) of a group's column-major block partials in a fixed order
 // (thread-strided, two rows in flight per thread, then the block tree).
 template <int BS>
-__device__ void reduce_rows(const DevState& d, const GroupInfo& G, int lo, int hi, double* out) {
+__device__ __forceinline__ void reduce_rows(const DevState& d, const GroupInfo& G, int lo, int hi, double* out) {
     const int64_t nb = d.n_blocks;
     double acc[NS];
 #pragma unroll
