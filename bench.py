@@ -124,7 +124,9 @@ def replicas_256(M, steps=2000, seq_sample=48):
     e0.step(DT, 20 + steps // 2, th0, ba0)
     mid = e0.download(("x", "v", "box"))
     e0.close()
-    best, _ = autotune(counts[:, None], [MASS], mid["box"], mid["x"], mid["v"], DT, controls, steps=200,
+    # 1000-step trials: about ten replicas rebuild per launch, so shorter
+    # windows rank the skins by which replicas happened to rebuild in them
+    best, _ = autotune(counts[:, None], [MASS], mid["box"], mid["x"], mid["v"], DT, controls, steps=1000,
                        warmup=20, blocks=(256,), tiles=(0,), cells=(2.0,))
     skin = best["skin"]
     e = M.Engine(counts[:, None], [MASS], boxes, force_mode="fp32_list", skin=skin)
