@@ -542,6 +542,40 @@ def test_large_gas_interior_bricks(mode):
         assert np.array_equal(d["virial"] != 0, s.temp != 0)
 
 
+@pytest.mark.parametrize("shape", ["liquid", "gas"])
+def test_elongated_boxes(shape):
+    """Boxes much longer along one axis: grids whose short axes are whole
+    bricks while the long one is tiled, and brick edges that differ by a
+    cell.  FP64 list forces equal the oracle's; with a thin skin (frequent
+    rebuilds) 200 steps equal the all-pairs path bitwise."""
+    rng = np.random.default_rng(17)
+    if shape == "liquid":
+        a = 0.5256
+        pos = O.lattice_fcc([3, 4, 24], a)
+        pos = pos + rng.uniform(-0.02, 0.02, pos.shape)
+        box = [3 * a, 4 * a, 24 * a]
+    else:
+        box = [6.0, 9.0, 40.0]
+        pos = rng.uniform(0.0, 1.0, (6000, 3)) * np.array(box)
+    n = len(pos)
+    s = O.State([n], [AR_MASS], box, pos=pos, vel=O.mb_velocities(n, 150.0, AR_MASS, 3))
+    O.apply_boundary_conditions(s)
+    e = engine_for(s, "fp64_list", skin=0.05)
+    e.update_force()
+    f = e.download(("force",))["force"]
+    ref = s.copy()
+    O.update_force(ref)  # all pairs, one minimum image (the short axes are below 2 r_cut)
+    assert np.count_nonzero(ref.force) > n
+    assert np.array_equal(f, ref.force)
+    a_ = engine_for(s, "fp64_allpairs")
+    a_.update_force()
+    for eng in (a_, e):
+        eng.step(0.002, 200)
+    da, db = a_.download(), e.download()
+    assert e.stats()["rebuilds"] >= 5
+    assert np.array_equal(da["x"], db["x"]) and np.array_equal(da["v"], db["v"])
+
+
 def test_dense_bricks_fall_back_to_grid_scan():
     """Liquid argon with ~32 atoms per cell: brick halos exceed the LDS
     staging capacity and lists are built straight from the grid."""
