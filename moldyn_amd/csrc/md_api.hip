@@ -541,14 +541,25 @@ static int write_params(md_ctx* c, const std::vector<RepParams>& P) {
     return MD_OK;
 }
 
+// Group sums of the current state.  The thermal partials are taken about
+// the centre-of-mass velocity of the sums stored before; after a new state
+// those belong to another state, so a second pass takes them about the new
+// centre of mass (no cancellation whatever the centre of mass does).
+static int reduce_state_sums(md_ctx* c, int passes) {
+    c->d.par = c->par;
+    for (int k = 0; k < passes; ++k) {
+        if (c->n_blocks > 0) HIPCHK(c, launch_reduce_state(c->d, c->fp32, c->bs, c->st));
+        HIPCHK(c, finalize(c, M_REDUCE));
+    }
+    return MD_OK;
+}
+
 static int ensure_sums(md_ctx* c) {
     if (c->sums_valid) return MD_OK;
     std::vector<RepParams> P((size_t)c->R, base_params(c));
     int rc;
     if ((rc = write_params(c, P))) return rc;
-    c->d.par = c->par;
-    if (c->n_blocks > 0) HIPCHK(c, launch_reduce_state(c->d, c->fp32, c->bs, c->st));
-    HIPCHK(c, finalize(c, M_REDUCE));
+    if ((rc = reduce_state_sums(c, 2))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->sums_valid = true;
     return MD_OK;
@@ -758,6 +769,7 @@ static int list_force_eval(md_ctx* c) {
             }
         }
         HIPCHK(c, finalize(c, M_REDUCE));
+        if ((rc = reduce_state_sums(c, 1))) return rc;
         RepStats s;
         if ((rc = read_stats(c, s))) return rc;
         int grown = 0;
@@ -1099,15 +1111,51 @@ int md_get_options(const md_ctx* c, md_options* out) {
 // slots of a group: its tiles of bs records (a block may hold several tiles)
 static int64_t group_slots(const md_ctx* c, const GroupInfo& G) { return (G.count + c->bs - 1) / c->bs * c->bs; }
 
+// Record buffer holding each replica's current state.  Replicas may sit at
+// different buffers: a split step ends in the buffer it started from while
+// an ordinary step ends in the other one.
+static int replica_bufs(md_ctx* c, std::vector<int>& cur) {
+    std::vector<RepCtrl> C;
+    int rc = read_ctrl(c, C);
+    if (rc) return rc;
+    cur.resize(c->R);
+    for (int r = 0; r < c->R; ++r) cur[r] = C[r].cur & 1;
+    return MD_OK;
+}
+
+// Copy each replica's slot range between the host image and its own buffer.
+static int copy_replica_ranges(md_ctx* c, void* host, size_t rsize, const std::vector<int>& cur, bool to_device) {
+    for (int r = 0; r < c->R; ++r) {
+        const RepInfo& R = c->reps[r];
+        if (R.end <= R.start) continue;
+        char* h = (char*)host + rsize * (size_t)R.start;
+        char* dv = (char*)c->d.rec[cur[r]] + rsize * (size_t)R.start;
+        const size_t bytes = rsize * (size_t)(R.end - R.start);
+        if (to_device)
+            HIPCHK(c, hipMemcpy(dv, h, bytes, hipMemcpyHostToDevice));
+        else
+            HIPCHK(c, hipMemcpy(h, dv, bytes, hipMemcpyDeviceToHost));
+    }
+    return MD_OK;
+}
+
 // Host <-> device record transfer.  Host arrays are N x 3 (x, y, z per
-// particle) in the caller's order; slots add NaN-positioned padding.
+// particle) in the caller's order; slots add NaN-positioned padding.  With
+// `cur` the records go to (and, keeping forces, come from) each replica's
+// current buffer; without it to buffer `buf` of every replica.
 template <typename FT>
 static int put_records(md_ctx* c, int buf, const double* x, const double* v, const double* f,
-                       bool keep_forces) {
+                       bool keep_forces, const std::vector<int>* cur = nullptr) {
     const size_t N = (size_t)std::max<int64_t>(c->n_slots, 1);
     std::vector<Rec<FT>> h(N);
-    if (keep_forces)
-        HIPCHK(c, hipMemcpy(h.data(), c->d.rec[buf], sizeof(Rec<FT>) * N, hipMemcpyDeviceToHost));
+    if (keep_forces) {
+        if (cur) {
+            int rc = copy_replica_ranges(c, h.data(), sizeof(Rec<FT>), *cur, false);
+            if (rc) return rc;
+        } else {
+            HIPCHK(c, hipMemcpy(h.data(), c->d.rec[buf], sizeof(Rec<FT>) * N, hipMemcpyDeviceToHost));
+        }
+    }
     for (const GroupInfo& G : c->groups)
         for (int64_t i = 0; i < group_slots(c, G); ++i) {
             Rec<FT>& r = h[(size_t)(G.start + i)];
@@ -1140,15 +1188,18 @@ static int put_records(md_ctx* c, int buf, const double* x, const double* v, con
                 r.fx = r.fy = r.fz = (FT)0;
             }
         }
+    if (cur) return copy_replica_ranges(c, h.data(), sizeof(Rec<FT>), *cur, true);
     HIPCHK(c, hipMemcpy(c->d.rec[buf], h.data(), sizeof(Rec<FT>) * N, hipMemcpyHostToDevice));
     return MD_OK;
 }
 
+// Records of each replica's current buffer (`cur`, one entry per replica).
 template <typename FT>
-static int get_records(md_ctx* c, int buf, double* x, double* v, double* f) {
+static int get_records(md_ctx* c, const std::vector<int>& cur, double* x, double* v, double* f) {
     const size_t N = (size_t)std::max<int64_t>(c->n_slots, 1);
     std::vector<Rec<FT>> h(N);
-    HIPCHK(c, hipMemcpy(h.data(), c->d.rec[buf], sizeof(Rec<FT>) * N, hipMemcpyDeviceToHost));
+    int rc = copy_replica_ranges(c, h.data(), sizeof(Rec<FT>), cur, false);
+    if (rc) return rc;
     int64_t a = 0;
     for (const GroupInfo& G : c->groups)
         for (int64_t i = 0; i < G.count; ++i, ++a) {
@@ -1259,17 +1310,16 @@ int md_upload_state(md_ctx* c, const double* x, const double* v, const double* b
 
 int md_upload_forces(md_ctx* c, const double* force, const double* pot, const double* vir) {
     if (!c) return MD_ERR_ARG;
-    std::vector<RepCtrl> C;
-    int rc = read_ctrl(c, C);
+    std::vector<int> cur;
+    int rc = replica_bufs(c, cur);
     if (rc) return rc;
-    const int buf = C[0].cur;
     std::vector<double> zero;
     if (!force) {
         zero.assign((size_t)c->n_atoms * 3 + 1, 0.0);
         force = zero.data();
     }
-    rc = c->fp32 ? put_records<float>(c, buf, nullptr, nullptr, force, true)
-                 : put_records<double>(c, buf, nullptr, nullptr, force, true);
+    rc = c->fp32 ? put_records<float>(c, 0, nullptr, nullptr, force, true, &cur)
+                 : put_records<double>(c, 0, nullptr, nullptr, force, true, &cur);
     if (rc) return rc;
     if ((rc = put_pv(c, pot, vir))) return rc;
     c->sums_valid = false;
@@ -1282,9 +1332,10 @@ int md_download_state(md_ctx* c, double* x, double* v, double* force, double* po
     std::vector<RepCtrl> C;
     int rc = read_ctrl(c, C);
     if (rc) return rc;
-    const int buf = C[0].cur;
     if (x || v || force) {
-        rc = c->fp32 ? get_records<float>(c, buf, x, v, force) : get_records<double>(c, buf, x, v, force);
+        std::vector<int> cur(c->R);
+        for (int r = 0; r < c->R; ++r) cur[r] = C[r].cur & 1;
+        rc = c->fp32 ? get_records<float>(c, cur, x, v, force) : get_records<double>(c, cur, x, v, force);
         if (rc) return rc;
     }
     if (pot || vir)
@@ -1408,6 +1459,7 @@ int md_update_force(md_ctx* c) {
     c->d.par = c->par;
     if (c->n_blocks > 0) HIPCHK(c, launch_allpairs(c->d, c->bs, true, c->st));
     HIPCHK(c, finalize(c, M_REDUCE));
+    if ((rc = reduce_state_sums(c, 1))) return rc;
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->sums_valid = true;
     return MD_OK;
@@ -1445,7 +1497,8 @@ static int type_thermo(md_ctx* c, int rep, int type, double* temperature, double
     const double* s = &S[(size_t)g * NS];
     const double M = G.sum_m;
     const double cx = s[S_MVX] / M, cy = s[S_MVY] / M, cz = s[S_MVZ] / M;
-    const double th2 = s[S_MV2] - M * ((cx * cx + cy * cy) + cz * cz);
+    // the stored sums hold sum m |v - v_com|^2 (energy.rs:25-37)
+    const double th2 = s[S_MV2];
     const double thermal = th2 / 2.0;
     const double t = (2.0 * thermal) / (3.0 * (double)G.count * MD_K_B) * 100.0;
     const double* b = C[rep].box;
@@ -1454,7 +1507,8 @@ static int type_thermo(md_ctx* c, int rep, int type, double* temperature, double
     if (temperature) *temperature = t;
     if (pressure) *pressure = p;
     if (out) {
-        out[0] = s[S_MV2] / 2.0;
+        // energy.rs:14-22: sum m |v|^2 = sum m |v - v_com|^2 + M |v_com|^2
+        out[0] = (th2 + M * ((cx * cx + cy * cy) + cz * cz)) / 2.0;
         out[1] = s[S_POT] / 2.0;
         out[2] = thermal;
         out[3] = t;
@@ -1611,35 +1665,42 @@ int md_init_mb_velocities(double* vel, int64_t n, double temperature, double mas
 
 // ------------------------------------------------------------------ device initialisers
 
-// The records to edit on the device: the current buffer and boxes; a context
-// that never held a state gets the blank one first (initialize_particles).
-static int edit_begin(md_ctx* c, int* cur, std::vector<double>& boxes) {
+// The records to edit on the device: each replica's current buffer and the
+// boxes; a context that never held a state gets the blank one first
+// (initialize_particles).
+static int edit_begin(md_ctx* c, std::vector<int>& cur, std::vector<double>& boxes) {
     boxes.assign((size_t)c->R * 3, 0.0);
+    cur.assign(c->R, 0);
     if (!c->ctrl_valid) {
         HIPCHK(c, launch_init_blank(c->d, c->fp32, c->d.groups, (int)c->groups.size(), c->n_slots, c->st));
         HIPCHK(c, hipMemsetAsync(c->d.pot, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_slots, 1), c->st));
         HIPCHK(c, hipMemsetAsync(c->d.vir, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_slots, 1), c->st));
         for (size_t k = 0; k < boxes.size(); ++k) boxes[k] = c->box0[k];
-        int rc = reset_controls(c, boxes.data(), 0);
-        if (rc) return rc;
-        *cur = 0;
-        return MD_OK;
+        return reset_controls(c, boxes.data(), 0);
     }
     std::vector<RepCtrl> C;
     int rc = read_ctrl(c, C);
     if (rc) return rc;
-    *cur = C[0].cur;
-    for (int r = 0; r < c->R; ++r)
+    for (int r = 0; r < c->R; ++r) {
+        cur[r] = C[(size_t)r].cur & 1;
         for (int k = 0; k < 3; ++k) boxes[(size_t)r * 3 + k] = C[(size_t)r].box[k];
+    }
     return MD_OK;
 }
 
-// After an edit of buffer `cur`: both buffers hold the edited state, per-atom
-// potentials and virials cleared, controls reset as by md_upload_state.
-static int edit_commit(md_ctx* c, int cur, const std::vector<double>& boxes) {
-    const size_t bytes = (size_t)std::max<int64_t>(c->n_slots, 1) * (c->fp32 ? sizeof(Rec<float>) : sizeof(Rec<double>));
-    if (cur != 0) HIPCHK(c, hipMemcpyAsync(c->d.rec[0], c->d.rec[cur], bytes, hipMemcpyDeviceToDevice, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->d.rec[1], c->d.rec[0], bytes, hipMemcpyDeviceToDevice, c->st));
+// After an edit of each replica's buffer `cur[r]`: both buffers hold the
+// edited state, per-atom potentials and virials cleared, controls reset as
+// by md_upload_state.
+static int edit_commit(md_ctx* c, const std::vector<int>& cur, const std::vector<double>& boxes) {
+    const size_t rsize = c->fp32 ? sizeof(Rec<float>) : sizeof(Rec<double>);
+    for (int r = 0; r < c->R; ++r) {
+        const RepInfo& R = c->reps[r];
+        if (R.end <= R.start) continue;
+        const size_t off = rsize * (size_t)R.start, bytes = rsize * (size_t)(R.end - R.start);
+        char* src = (char*)c->d.rec[cur[r]] + off;
+        char* dst = (char*)c->d.rec[cur[r] ^ 1] + off;
+        HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->st));
+    }
     HIPCHK(c, hipMemsetAsync(c->d.pot, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_slots, 1), c->st));
     HIPCHK(c, hipMemsetAsync(c->d.vir, 0, sizeof(double) * (size_t)std::max<int64_t>(c->n_slots, 1), c->st));
     uint32_t epoch0 = 0;
@@ -1660,9 +1721,9 @@ int md_init_positions(md_ctx* c, int replica, uint16_t type_id, int kind, const 
     const GroupInfo* G = find_group(c, replica, (int)type_id);
     if (!G) return fail(c, MD_ERR_ARG, "InitError::ParticleIdDidNotFound: no such replica or type");
     if (grid[0] < 0 || grid[1] < 0 || grid[2] < 0) return fail(c, MD_ERR_ARG, "negative grid size");
-    int cur = 0;
+    std::vector<int> cur;
     std::vector<double> boxes;
-    int rc = edit_begin(c, &cur, boxes);
+    int rc = edit_begin(c, cur, boxes);
     if (rc) return rc;
     const double* box = &boxes[(size_t)replica * 3];
     // position.rs:166-171, then :35-36 / :53-54
@@ -1670,7 +1731,7 @@ int md_init_positions(md_ctx* c, int replica, uint16_t type_id, int kind, const 
         return fail(c, MD_ERR_ARG, "InitError::OutOfBoundary: the lattice is larger than the box");
     const int64_t sites = grid[0] * grid[1] * grid[2] * (kind == 1 ? 4 : 1);
     if (sites > G->count) return fail(c, MD_ERR_ARG, "InitError::TooBig: more lattice sites than particles");
-    HIPCHK(c, launch_init_lattice(c->d.rec[cur], c->fp32, G->start, sites, kind == 1, start, grid, a, c->st));
+    HIPCHK(c, launch_init_lattice(c->d.rec[cur[replica]], c->fp32, G->start, sites, kind == 1, start, grid, a, c->st));
     return edit_commit(c, cur, boxes);
 }
 
@@ -1678,14 +1739,14 @@ int md_init_velocities(md_ctx* c, int replica, uint16_t type_id, double temperat
     if (!c) return MD_ERR_ARG;
     const GroupInfo* G = find_group(c, replica, (int)type_id);
     if (!G) return fail(c, MD_ERR_ARG, "no such replica or type");
-    int cur = 0;
+    std::vector<int> cur;
     std::vector<double> boxes;
-    int rc = edit_begin(c, &cur, boxes);
+    int rc = edit_begin(c, cur, boxes);
     if (rc) return rc;
     // velocity.rs:8-10
     const double tp = temperature * 0.01;
     const double sigma = sqrt(MD_K_B * tp / G->mass);
-    HIPCHK(c, launch_init_mb(c->d.rec[cur], c->fp32, G->start, G->count, sigma, seed, c->st));
+    HIPCHK(c, launch_init_mb(c->d.rec[cur[replica]], c->fp32, G->start, G->count, sigma, seed, c->st));
     return edit_commit(c, cur, boxes);
 }
 

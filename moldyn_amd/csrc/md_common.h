@@ -161,6 +161,34 @@ __device__ __forceinline__ void block_exclusive_scan(int* a, int n) {
     lds_barrier();
 }
 
+// ---------------------------------------------------------------- thermal sums
+//
+// The reference's thermal energy and pressure take every velocity relative to
+// the centre-of-mass velocity, v - v_com, before squaring (energy.rs:25-37,
+// pressure.rs:11-14).  A one-pass sum of m |v|^2 minus M |v_com|^2 would
+// cancel catastrophically when the centre of mass drifts fast against the
+// thermal speed.  The per-block partials therefore sum m |v - v_ref|^2 about a
+// reference velocity that is already close to v_com: the centre-of-mass
+// velocity of the group sums stored by the previous reduction.  The finalize
+// then moves the reduced sum to the new centre of mass exactly,
+//   sum m |v - c|^2 = sum m |v - v_ref|^2 - M |c - v_ref|^2,
+// which loses nothing while v_ref ~ c, and stores it: stored group sums hold
+// the thermal sums (about their own centre of mass) in S_MV2 / S_KV2.
+
+// Component k of the reference velocity that partials of group sums S
+// (stored by the last reduction; M the group's mass) are taken about.
+__host__ __device__ __forceinline__ double sums_vref(const double* S, double M, int k) {
+    const double v = M > 0.0 ? S[S_MVX + k] / M : 0.0;
+    return isfinite(v) ? v : 0.0;
+}
+
+// sum m |v - v_ref|^2 moved to the centre of mass c = mv / M.
+__host__ __device__ __forceinline__ double sums_about_com(double q, const double* mv, double M, const double* vref) {
+    if (!(M > 0.0)) return q;
+    const double dx = mv[0] / M - vref[0], dy = mv[1] / M - vref[1], dz = mv[2] / M - vref[2];
+    return q - M * ((dx * dx + dy * dy) + dz * dz);
+}
+
 // ---------------------------------------------------------------- pair terms
 
 // potential.rs:201-211 with LJ of potential.rs:57-69, FP64, reference order.

@@ -81,11 +81,11 @@ enum : uint32_t {
 // per-block partial / per-group sum columns
 enum {
     S_MVX, S_MVY, S_MVZ,  // sum m v (state after the step)
-    S_MV2,                // sum m |v|^2
+    S_MV2,                // sum m |v - v_ref|^2 in partials; stored group sums: about v_com
     S_POT,                // sum potential
     S_VIR,                // sum virial (Particle::temp)
     S_KVX, S_KVY, S_KVZ,  // sum m v after the first half kick (Nose-Hoover)
-    S_KV2,                // sum m |v|^2 after the first half kick
+    S_KV2,                // as S_MV2, after the first half kick
     S_WV,                 // max |v|^2 of the state after the step
     S_WA,                 // max |F|^2 / m^2 of the state after the step
     S_VMAX,               // max |v|^2 of this step's drift

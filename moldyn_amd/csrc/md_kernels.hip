@@ -596,7 +596,8 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
 template <typename FT, int BS, bool MULTI, bool DENSE>
 __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     const int b = xcd_block(blockIdx.x, gridDim.x);
-    const GroupInfo G = d.groups[d.blk_group[b]];
+    const int gi = d.blk_group[b];
+    const GroupInfo G = d.groups[gi];
     const int r = G.rep;
     const RepCtrl& C = ctrl_in(d, r);
     const uint32_t flags = C.flags;
@@ -630,6 +631,11 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     const int ti = G.type;
     const double c = wave_uniform(P.cm[ti]);
     const double inv_m2 = wave_uniform(P.inv_m2[ti]);
+    // the thermal sums are taken about the last stored centre-of-mass velocity
+    const double* gs = d.sums + (int64_t)gi * NS;
+    const double vr0 = wave_uniform(sums_vref(gs, G.sum_m, 0));
+    const double vr1 = wave_uniform(sums_vref(gs, G.sum_m, 1));
+    const double vr2 = wave_uniform(sums_vref(gs, G.sum_m, 2));
     const uint32_t kick_cols = P.th_kind == 2 ? kColsKick : 0u;
     double part[NS];
 #pragma unroll
@@ -646,7 +652,8 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
             part[S_KVX] += vkx * m;
             part[S_KVY] += vky * m;
             part[S_KVZ] += vkz * m;
-            part[S_KV2] += m * ((vkx * vkx + vky * vky) + vkz * vkz);
+            const double wx = vkx - vr0, wy = vky - vr1, wz = vkz - vr2;
+            part[S_KV2] += m * ((wx * wx + wy * wy) + wz * wz);
             a.ux = vkx * lam;
             a.uy = vky * lam;
             a.uz = vkz * lam;
@@ -736,7 +743,8 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
                 part[S_KVX] += vkx * m;
                 part[S_KVY] += vky * m;
                 part[S_KVZ] += vkz * m;
-                part[S_KV2] += m * ((vkx * vkx + vky * vky) + vkz * vkz);
+                const double wx = vkx - vr0, wy = vky - vr1, wz = vkz - vr2;
+                part[S_KV2] += m * ((wx * wx + wy * wy) + wz * wz);
                 // thermostat.rs:55-57 scale, integrator.rs:40-45 drift + wrap
                 v1x = vkx * lam;
                 v1y = vky * lam;
@@ -780,7 +788,10 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
             part[S_MVX] += vx * m;
             part[S_MVY] += vy * m;
             part[S_MVZ] += vz * m;
-            part[S_MV2] += m * ((vx * vx + vy * vy) + vz * vz);
+            {
+                const double wx = vx - vr0, wy = vy - vr1, wz = vz - vr2;
+                part[S_MV2] += m * ((wx * wx + wy * wy) + wz * wz);
+            }
             part[S_POT] += (double)acc.pot;
             part[S_VIR] += (double)acc.vir;
             part[S_WV] = fmax(part[S_WV], (vx * vx + vy * vy) + vz * vz);
@@ -873,6 +884,8 @@ __global__ __launch_bounds__(BS) void k_force_eval(DevState d) {
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double m = G.mass;
     const int ti = G.type;
+    const double* gs = d.sums + (int64_t)d.blk_group[b] * NS;
+    const double vr[3] = {sums_vref(gs, G.sum_m, 0), sums_vref(gs, G.sum_m, 1), sums_vref(gs, G.sum_m, 2)};
     double part[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
@@ -902,7 +915,10 @@ __global__ __launch_bounds__(BS) void k_force_eval(DevState d) {
         part[S_MVX] += a.vx * m;
         part[S_MVY] += a.vy * m;
         part[S_MVZ] += a.vz * m;
-        part[S_MV2] += m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
+        {
+            const double wx = a.vx - vr[0], wy = a.vy - vr[1], wz = a.vz - vr[2];
+            part[S_MV2] += m * ((wx * wx + wy * wy) + wz * wz);
+        }
         part[S_POT] += (double)acc.pot;
         part[S_VIR] += (double)acc.vir;
         part[S_WV] = fmax(part[S_WV], (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
@@ -934,6 +950,8 @@ __global__ __launch_bounds__(BS) void k_reduce_state(DevState d) {
     const GroupInfo G = d.groups[d.blk_group[b]];
     const RepCtrl& C = ctrl_in(d, G.rep);
     const TileRange tr = block_tiles(d, G, b, BS);
+    const double* gs = d.sums + (int64_t)d.blk_group[b] * NS;
+    const double vr[3] = {sums_vref(gs, G.sum_m, 0), sums_vref(gs, G.sum_m, 1), sums_vref(gs, G.sum_m, 2)};
     double part[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
@@ -945,7 +963,8 @@ __global__ __launch_bounds__(BS) void k_reduce_state(DevState d) {
         part[S_MVX] += a.vx * m;
         part[S_MVY] += a.vy * m;
         part[S_MVZ] += a.vz * m;
-        part[S_MV2] += m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
+        const double wx = a.vx - vr[0], wy = a.vy - vr[1], wz = a.vz - vr[2];
+        part[S_MV2] += m * ((wx * wx + wy * wy) + wz * wz);
         part[S_POT] += d.pot[s];
         part[S_VIR] += d.vir[s];
         const double fx = (double)a.fx, fy = (double)a.fy, fz = (double)a.fz;
@@ -973,6 +992,8 @@ __global__ __launch_bounds__(BS) void k_drift(DevState d) {
     const int cur = C.cur, nx = cur ^ 1;
     const double dt = P.dt, lam = C.lambda, m = G.mass;
     const double c = P.cm[G.type];
+    const double* gs = d.sums + (int64_t)d.blk_group[b] * NS;
+    const double vr[3] = {sums_vref(gs, G.sum_m, 0), sums_vref(gs, G.sum_m, 1), sums_vref(gs, G.sum_m, 2)};
     double part[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) part[k] = 0.0;
@@ -986,7 +1007,8 @@ __global__ __launch_bounds__(BS) void k_drift(DevState d) {
         part[S_KVX] += vkx * m;
         part[S_KVY] += vky * m;
         part[S_KVZ] += vkz * m;
-        part[S_KV2] += m * ((vkx * vkx + vky * vky) + vkz * vkz);
+        const double wx = vkx - vr[0], wy = vky - vr[1], wz = vkz - vr[2];
+        part[S_KV2] += m * ((wx * wx + wy * wy) + wz * wz);
         a.vx = vkx * lam;
         a.vy = vky * lam;
         a.vz = vkz * lam;
@@ -1029,6 +1051,8 @@ __global__ __launch_bounds__(BS) void k_allpairs(DevState d) {
     const int T = d.n_types;
     const double m = G.mass;
     const double c = P.cm[ti];
+    const double* gs = d.sums + (int64_t)d.blk_group[b] * NS;
+    const double vr[3] = {sums_vref(gs, G.sum_m, 0), sums_vref(gs, G.sum_m, 1), sums_vref(gs, G.sum_m, 2)};
     __shared__ double tx[BS], ty[BS], tz[BS];
     __shared__ uint8_t tt[BS];
     const PairParam p0 = d.pairs[ti * T + ti];
@@ -1084,7 +1108,8 @@ __global__ __launch_bounds__(BS) void k_allpairs(DevState d) {
             part[S_MVX] += a.vx * m;
             part[S_MVY] += a.vy * m;
             part[S_MVZ] += a.vz * m;
-            part[S_MV2] += m * ((a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
+            const double wx = a.vx - vr[0], wy = a.vy - vr[1], wz = a.vz - vr[2];
+            part[S_MV2] += m * ((wx * wx + wy * wy) + wz * wz);
             part[S_POT] += pot;
             part[S_VIR] += vir;
             part[S_WV] = fmax(part[S_WV], (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
@@ -1099,12 +1124,9 @@ __global__ __launch_bounds__(BS) void k_allpairs(DevState d) {
 
 __device__ __forceinline__ void thermo(const double* S, const GroupInfo& G, const double* box,
                                        bool kicked, double& temperature, double& pressure) {
-    const int ox = kicked ? S_KVX : S_MVX;
-    const int o2 = kicked ? S_KV2 : S_MV2;
-    const double M = G.sum_m;
-    const double cx = S[ox] / M, cy = S[ox + 1] / M, cz = S[ox + 2] / M;
-    // sum m |v - v_com|^2 = sum m |v|^2 - M |v_com|^2  (energy.rs:25-37, pressure.rs:11-14)
-    const double th2 = S[o2] - M * ((cx * cx + cy * cy) + cz * cz);
+    // sum m |v - v_com|^2 (energy.rs:25-37, pressure.rs:11-14): the reduced
+    // sums hold it already, taken about the centre of mass
+    const double th2 = S[kicked ? S_KV2 : S_MV2];
     const double thermal = th2 / 2.0;
     // temperature.rs:4-7
     temperature = (2.0 * thermal) / (3.0 * (double)G.count * kKB) * 100.0;
@@ -1340,6 +1362,7 @@ __device__ __forceinline__ int64_t finalize_replica(const DevState& d, int r, in
     __shared__ RepParams s_par;
     __shared__ GroupInfo s_grp[kMaxTypes];
     __shared__ double s_sums[kMaxTypes * NS];
+    __shared__ double s_vref[kMaxTypes * 3];
     const RepInfo R = d.reps[r];
     // one system of one type: its partial rows are [0, n_blocks), so their
     // loads go out together with the staging loads (one memory round trip)
@@ -1366,6 +1389,12 @@ __device__ __forceinline__ int64_t finalize_replica(const DevState& d, int r, in
     copy_words<BS>(&s_ctrl, &ctrl_in(d, r), (int)sizeof(RepCtrl));
     copy_words<BS>(&s_par, d.params + r, (int)sizeof(RepParams));
     copy_words<BS>(s_grp, d.groups + g0, (int)sizeof(GroupInfo) * T);
+    // the reference velocities this launch's thermal partials were taken
+    // about: the centre of mass of the sums stored so far
+    if ((int)threadIdx.x < 3 * T) {
+        const int t = threadIdx.x / 3;
+        s_vref[threadIdx.x] = sums_vref(d.sums + (g0 + t) * NS, d.groups[g0 + t].sum_m, threadIdx.x % 3);
+    }
     __syncthreads();
     const uint32_t f0 = s_ctrl.flags;
     const int64_t launch = s_ctrl.launches;
@@ -1386,6 +1415,15 @@ __device__ __forceinline__ int64_t finalize_replica(const DevState& d, int r, in
             for (int t = 0; t < T; ++t) reduce_rows<BS>(d, s_grp[t], 0, s_grp[t].nblocks, s_sums + t * NS);
     } else {
         for (int i = threadIdx.x; i < T * NS; i += BS) s_sums[i] = d.sums[g0 * NS + i];
+    }
+    __syncthreads();
+    if (reduced && (int)threadIdx.x < T) {
+        // thermal sums moved from the reference velocity to the centre of mass
+        double* S = s_sums + threadIdx.x * NS;
+        const double M = s_grp[threadIdx.x].sum_m;
+        const double* vr = s_vref + 3 * threadIdx.x;
+        S[S_MV2] = sums_about_com(S[S_MV2], S + S_MVX, M, vr);
+        S[S_KV2] = sums_about_com(S[S_KV2], S + S_KVX, M, vr);
     }
     __syncthreads();
     if (d.mlp_stat) {

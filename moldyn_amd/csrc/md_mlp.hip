@@ -118,6 +118,8 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double dt = P.dt, lam = C.lambda;
     const double m = G.mass;
+    const double* gs = d.sums + (int64_t)d.blk_group[b] * NS;
+    const double vr[3] = {sums_vref(gs, G.sum_m, 0), sums_vref(gs, G.sum_m, 1), sums_vref(gs, G.sum_m, 2)};
     const double c = P.cm[G.type];
     double part[NS];
 #pragma unroll
@@ -142,7 +144,8 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
             part[S_KVX] += vkx * m;
             part[S_KVY] += vky * m;
             part[S_KVZ] += vkz * m;
-            part[S_KV2] += m * ((vkx * vkx + vky * vky) + vkz * vkz);
+            const double wx = vkx - vr[0], wy = vky - vr[1], wz = vkz - vr[2];
+            part[S_KV2] += m * ((wx * wx + wy * wy) + wz * wz);
             const double v1x = vkx * lam, v1y = vky * lam, v1z = vkz * lam;
             x1 = wrap1(a.x + v1x * dt, bx);
             y1 = wrap1(a.y + v1y * dt, by);
@@ -326,6 +329,8 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double mu = FORCE_ONLY ? 1.0 : C.myu;
     const double m = G.mass;
+    const double* gs = d.sums + (int64_t)d.blk_group[b] * NS;
+    const double vr[3] = {sums_vref(gs, G.sum_m, 0), sums_vref(gs, G.sum_m, 1), sums_vref(gs, G.sum_m, 2)};
     const double c = P.cm[G.type];
     const PairParam p0 = d.pairs[0];
     const double rc2 = (double)mc.rc * (double)mc.rc;
@@ -419,7 +424,10 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
         part[S_MVX] += out.vx * m;
         part[S_MVY] += out.vy * m;
         part[S_MVZ] += out.vz * m;
-        part[S_MV2] += m * ((out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz);
+        {
+            const double wx = out.vx - vr[0], wy = out.vy - vr[1], wz = out.vz - vr[2];
+            part[S_MV2] += m * ((wx * wx + wy * wy) + wz * wz);
+        }
         part[S_POT] += (double)pot;
         part[S_VIR] += (double)vir;
         part[S_WV] = fmax(part[S_WV], (out.vx * out.vx + out.vy * out.vy) + out.vz * out.vz);

@@ -172,6 +172,13 @@ int orc_update_force_cells(orc_state* s, const orc_lj* table, int nthreads) {
         m[d] = q < 1.0 ? 1 : (int64_t)q;
         if (m[d] > 1024) m[d] = 1024;
     }
+    /* at most about two cells per particle (dilute gases): coarser cells only
+     * add candidates, which the criterion rejects, so the result is unchanged */
+    while (m[0] * m[1] * m[2] > 2 * n + 27) {
+        int d = m[1] > m[0] ? (m[2] > m[1] ? 2 : 1) : (m[2] > m[0] ? 2 : 0);
+        m[d] = m[d] - (m[d] / 8 > 1 ? m[d] / 8 : 1);
+        if (m[d] < 1) m[d] = 1;
+    }
     int64_t ncell = m[0] * m[1] * m[2];
     int64_t* head = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ncell + 1));
     int64_t* order = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
@@ -470,9 +477,12 @@ void orc_barostat_update(orc_barostat* ba, orc_state* s, int t) {
     }
 }
 
-/* solver/integrator.rs:14-64 */
-void orc_verlet_step(orc_state* s, const orc_lj* table, double dt, orc_barostat* ba,
-                     double target_p, orc_thermostat* th, double target_t, int nthreads) {
+/* solver/integrator.rs:14-64; cells selects orc_update_force_cells for the
+ * force evaluation (bitwise the same forces).  Returns 0, or -1 when the cell
+ * grid could not be allocated (the state is then mid-step). */
+static int verlet_step_impl(orc_state* s, const orc_lj* table, double dt, orc_barostat* ba,
+                            double target_p, orc_thermostat* th, double target_t, int nthreads,
+                            int cells) {
     const int T = s->n_types;
     if (ba)
         for (int t = 0; t < T; ++t) orc_barostat_calculate_myu(ba, s, dt, t, target_p);
@@ -495,7 +505,11 @@ void orc_verlet_step(orc_state* s, const orc_lj* table, double dt, orc_barostat*
         for (int64_t i = 0; i < 3 * n; ++i) s->pos[i] += s->vel[i] * dt;
     }
     orc_apply_boundary_conditions(s);
-    orc_update_force(s, table, nthreads);
+    if (cells) {
+        if (orc_update_force_cells(s, table, nthreads) != 0) return -1;
+    } else {
+        orc_update_force(s, table, nthreads);
+    }
     for (int t = 0; t < T; ++t) {
         int64_t o = type_offset(s, t), n = s->counts[t];
         if (n == 0) continue;
@@ -508,6 +522,17 @@ void orc_verlet_step(orc_state* s, const orc_lj* table, double dt, orc_barostat*
     }
     if (ba)
         for (int t = 0; t < T; ++t) orc_barostat_update(ba, s, t);
+    return 0;
+}
+
+void orc_verlet_step(orc_state* s, const orc_lj* table, double dt, orc_barostat* ba,
+                     double target_p, orc_thermostat* th, double target_t, int nthreads) {
+    (void)verlet_step_impl(s, table, dt, ba, target_p, th, target_t, nthreads, 0);
+}
+
+int orc_verlet_step_cells(orc_state* s, const orc_lj* table, double dt, orc_barostat* ba,
+                          double target_p, orc_thermostat* th, double target_t, int nthreads) {
+    return verlet_step_impl(s, table, dt, ba, target_p, th, target_t, nthreads, 1);
 }
 
 /* initializer/position.rs:34-51: index x*ny*nz + y*nz + z. */

@@ -20,6 +20,9 @@
  *   orc_verlet_step (NVE)
  *       golden digits solver/src/lib.rs:109-332, cli/src/tests.rs:35-98;
  *       momentum invariant solver/src/lib.rs:49-75.
+ *   orc_verlet_step_cells
+ *       bitwise equal to orc_verlet_step over NVE and NPT trajectories of a
+ *       liquid, a gas and a two-type system.
  *   macro parameters
  *       golden digits solver/src/lib.rs:264-427; ideal-gas closed form.
  *   Berendsen thermostat / barostat, Nose-Hoover
@@ -122,6 +125,10 @@ void orc_barostat_update(orc_barostat* ba, orc_state* s, int t);
 /* solver/integrator.rs:14-64.  ba/th may be NULL (Option::None). */
 void orc_verlet_step(orc_state* s, const orc_lj* table, double dt, orc_barostat* ba,
                      double target_p, orc_thermostat* th, double target_t, int nthreads);
+/* The same step with the forces of orc_update_force_cells, for states too
+ * large for O(N^2) forces.  Returns 0, or -1 on allocation failure. */
+int orc_verlet_step_cells(orc_state* s, const orc_lj* table, double dt, orc_barostat* ba,
+                          double target_p, orc_thermostat* th, double target_t, int nthreads);
 
 /* initializer/position.rs:34-51 (UnitCell::U) and :52-99 (UnitCell::FCC) */
 void orc_lattice_u(double* pos, const double start[3], const int64_t grid[3], double a);

@@ -100,6 +100,8 @@ def lib():
         L.orc_barostat_calculate_myu.argtypes = [bp, sp, C.c_double, C.c_int, C.c_double]
         L.orc_barostat_update.argtypes = [bp, sp, C.c_int]
         L.orc_verlet_step.argtypes = [sp, lp, C.c_double, bp, C.c_double, tp, C.c_double, C.c_int]
+        L.orc_verlet_step_cells.argtypes = [sp, lp, C.c_double, bp, C.c_double, tp, C.c_double, C.c_int]
+        L.orc_verlet_step_cells.restype = C.c_int
         i64p = C.POINTER(C.c_int64)
         L.orc_lattice_u.argtypes = [dp, dp, i64p, C.c_double]
         L.orc_lattice_fcc.argtypes = [dp, dp, i64p, C.c_double]
@@ -272,13 +274,17 @@ def barostat_update(ba, state, t):
 
 
 def verlet_step(state, dt, table=None, barostat=None, target_p=0.0, thermostat=None,
-                target_t=0.0, nthreads=None):
-    """integrator.rs:14-64 (one step, in place)."""
+                target_t=0.0, nthreads=None, cells=False):
+    """integrator.rs:14-64 (one step, in place).  cells: forces through the
+    cell grid (bitwise the same), for states too large for all pairs."""
     table = table if table is not None else potential_table(state.n_types)
-    lib().orc_verlet_step(state._sync_in(), table, dt,
-                          C.byref(barostat) if barostat is not None else None, target_p,
-                          C.byref(thermostat) if thermostat is not None else None, target_t,
-                          nthreads or default_threads())
+    args = (state._sync_in(), table, dt, C.byref(barostat) if barostat is not None else None, target_p,
+            C.byref(thermostat) if thermostat is not None else None, target_t, nthreads or default_threads())
+    if cells:
+        if lib().orc_verlet_step_cells(*args) != 0:
+            raise MemoryError("orc_verlet_step_cells failed")
+    else:
+        lib().orc_verlet_step(*args)
     state._sync_out()
 
 

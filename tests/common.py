@@ -51,3 +51,44 @@ def engine_for(state, force_mode, masses=None, **kw):
 
 def rms(a):
     return float(np.sqrt(np.mean(np.square(a))))
+
+
+def grid_dims(box, rlist, budget):
+    """Cells per axis of a grid fill (md_device.h grid_dims, restated)."""
+    g = []
+    for b in box:
+        q = np.floor(b / (rlist * (1.0 + 1e-9)))
+        g.append(1 if q < 1.0 else (1024 if q > 1024.0 else int(q)))
+
+    def coarsen(x):
+        dec = x // 8
+        n = x - (1 if dec < 1 else dec)
+        return 1 if n < 1 else n
+    while g[0] * g[1] * g[2] > budget:
+        if g[1] > g[0] and g[1] >= g[2]:
+            g[1] = coarsen(g[1])
+        elif g[2] > g[0] and g[2] > g[1]:
+            g[2] = coarsen(g[2])
+        else:
+            g[0] = coarsen(g[0])
+    return g
+
+
+def brick_dims(gd, n_atoms, edge_max=14, cells_max=4096, target=900, own_max=440):
+    """Brick edge per axis and the mask of whole axes (md_device.h brick_dims,
+    restated): edges (e, e, e), (e, e, e + 1) or (e, e + 1, e + 1) with the
+    most centre particles within the staging limits, ties to the smaller halo."""
+    per_cell = n_atoms / float(gd[0] * gd[1] * gd[2])
+    best = (1, 1, 1)
+    best_own, best_halo = -1.0, 0.0
+    for e in range(1, edge_max + 1):
+        for x in range(3):
+            es = (e, e + (1 if x >= 2 else 0), e + (1 if x >= 1 else 0))
+            halo = np.prod([float(g if g <= b + 2 else b + 2) for g, b in zip(gd, es)])
+            own = np.prod([float(g if g <= b + 2 else b) for g, b in zip(gd, es)])
+            if (halo <= cells_max and halo * per_cell <= target and own * per_cell <= own_max
+                    and (own > best_own or (own == best_own and halo < best_halo))):
+                best_own, best_halo, best = own, halo, es
+    edge = [g if g <= b + 2 else b for g, b in zip(gd, best)]
+    whole = [g <= b + 2 for g, b in zip(gd, best)]
+    return edge, whole

@@ -409,8 +409,307 @@ def test_replicas_match_single(mode):
         assert np.array_equal(d["v"][sl], d1["v"]), k
         assert np.array_equal(d["box"][k], d1["box"]), k
         assert th[k].lambda_ == th1.lambda_ and ba[k].myu == ba1.myu
+        # each replica alone through the oracle under its own targets
+        # (integrator.rs:18-27, thermostat.rs:24-44, barostat.rs:21-36)
+        ref = st.copy()
+        oth, oba = O.Thermostat.berendsen(10.0), O.Barostat.berendsen(1.0, 5.0)
+        run_oracle(ref, 150, 0.002, oth, 100.0 + 50 * k, oba, 0.5 + k)
+        tol = 1e-10 if mode != "fp32_list" else 1e-6
+        assert np.allclose(d["box"][k], ref.box, rtol=1e-10 if mode != "fp32_list" else 1e-8, atol=0), k
+        dx = d["x"][sl] - ref.pos
+        dx -= ref.box * np.round(dx / ref.box)
+        assert np.abs(dx).max() < tol, k
+        assert np.allclose(d["v"][sl], ref.vel, rtol=tol, atol=tol), k
+        assert abs(th[k].lambda_ - oth.lambda_) < (1e-12 if mode != "fp32_list" else 1e-8), k
+        assert abs(ba[k].myu - oba.myu) < (1e-12 if mode != "fp32_list" else 1e-8), k
         off += st.n
         one.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64_list", "fp32_list"])
+def test_replica_buffers_after_split_steps(mode):
+    """A hot replica whose every step is split (drift, rebuild, forces: the
+    step ends in the buffer it started from) beside a cold one whose steps
+    alternate buffers: downloads, force uploads and device initialisers of
+    the batch address each replica's own current buffer."""
+    import moldyn_amd as M
+    hot = liquid_state(3, temperature=2000.0, seed=3)
+    cold = liquid_state(3, temperature=20.0, seed=4)
+    states = [hot, cold]
+    counts = [[st.n] for st in states]
+    boxes = [st.box for st in states]
+    e = M.Engine(counts, [AR_MASS], boxes, force_mode=mode, skin=0.005)
+    e.upload(np.concatenate([st.pos for st in states]), np.concatenate([st.vel for st in states]), boxes)
+    e.update_force()
+    singles = []
+    for st in states:
+        one = engine_for(st, mode, skin=0.005)
+        one.update_force()
+        singles.append(one)
+    splits = 0
+    for call in range(7):
+        e.step(0.002, 1)
+        for one in singles:
+            one.step(0.002, 1)
+        splits += e.stats()["split_steps"]
+        d = e.download()
+        off = 0
+        for k, one in enumerate(singles):
+            d1 = one.download()
+            sl = slice(off, off + states[k].n)
+            for key in ("x", "v", "force"):
+                assert np.array_equal(d[key][sl], d1[key]), (call, k, key)
+            off += states[k].n
+    assert splits >= 5
+    # forces uploaded into the batch land in each replica's current buffer:
+    # the next step's first half kick reads them (integrator.rs:28-34)
+    d = e.download()
+    f_new = np.random.default_rng(1).normal(0.0, 5.0, d["force"].shape)
+    e.upload_forces(f_new, d["pot"], d["virial"])
+    got = e.download()
+    assert np.array_equal(got["x"], d["x"]) and np.array_equal(got["v"], d["v"])
+    if mode == "fp64_list":
+        assert np.array_equal(got["force"], f_new)
+    else:
+        assert np.array_equal(got["force"], f_new.astype(np.float32).astype(np.float64))
+    # a device initialiser edits one replica and leaves the other as it is
+    e.init_velocities(50.0, seed=9, replica=1)
+    after = e.download()
+    n0 = hot.n
+    assert np.array_equal(after["x"], d["x"]) and np.array_equal(after["v"][:n0], d["v"][:n0])
+    ref_v = M.mb_velocities(cold.n, 50.0, AR_MASS, 9)
+    assert np.allclose(after["v"][n0:], ref_v, rtol=1e-12, atol=1e-13)
+    for one in singles:
+        one.close()
+    e.close()
+
+
+@pytest.mark.parametrize("mode", ["fp64_list", "fp32_list"])
+def test_thermal_sums_with_fast_centre_of_mass(mode):
+    """A liquid whose centre of mass moves a thousand times faster than its
+    thermal speed: temperature and pressure take v - v_com before squaring
+    (energy.rs:25-37, pressure.rs:11-14), so a one-pass sum of m |v|^2
+    minus M |v_com|^2 would lose about 1e-10 of them.  Macro values equal the
+    oracle's folds to 1e-12, and an NPT run tracks the oracle."""
+    import moldyn_amd as M
+    s = liquid_state(4, temperature=100.0, seed=6)
+    v_th = np.sqrt(O.K_B * 100.0 * 0.01 / AR_MASS)
+    s.vel = s.vel + np.array([1000.0 * v_th, -400.0 * v_th, 700.0 * v_th])
+    e = engine_for(s, mode)
+    e.update_force()
+    m = e.macro()
+    ref = s.copy()
+    O.update_force(ref)
+    mv = O.center_of_mass_velocity(ref, 0)
+    te = O.temperature(O.thermal_energy(ref, 0, mv), ref.n)
+    pr = O.pressure(ref, 0, mv)
+    tol = 1e-12 if mode != "fp32_list" else 1e-5
+    assert abs(m["temperature"] - te) <= 1e-12 * te
+    assert abs(m["pressure"] - pr) <= tol * abs(pr)
+    assert abs(m["thermal"] - O.thermal_energy(ref, 0, mv)) <= 1e-12 * m["thermal"]
+    assert abs(m["kinetic"] - O.kinetic_energy(ref, 0)) <= 1e-13 * m["kinetic"]
+    th = M.thermostat("berendsen", tau=0.5, target_t=150.0)
+    ba = M.barostat("berendsen", beta=1.0, tau=5.0, target_p=50.0)
+    e.step(0.002, 100, th, ba)
+    d = e.download()
+    oth, oba = O.Thermostat.berendsen(0.5), O.Barostat.berendsen(1.0, 5.0)
+    run_oracle(s, 100, 0.002, oth, 150.0, oba, 50.0)
+    ttol = 1e-10 if mode != "fp32_list" else 1e-4
+    assert abs(th.lambda_ - oth.lambda_) < ttol and abs(ba.myu - oba.myu) < ttol
+    assert np.allclose(d["box"], s.box, rtol=ttol, atol=0)
+    assert np.allclose(d["v"], s.vel, rtol=ttol, atol=ttol)
+
+
+def big_box_adversarial(seed=12):
+    """A 333.8339 nm box (the 1M-atom workload's), where an FP32 coordinate
+    has an ulp of 3e-5 nm: a background gas plus planted pairs at
+    |dx| = r_cut exactly and one ulp beyond, straddling the periodic faces
+    with coordinates next to L, along the diagonal, and at |dx| = L / 2."""
+    sigma, eps, rc, uc = O.new_lennard_jones(0.3418, 1.712)
+    L = 100 * 3.338339
+    rng = np.random.default_rng(seed)
+    pos = [rng.uniform(0.0, L, (20000, 3))]
+    top = np.nextafter(L, 0.0)
+
+    def pair_at(xi, along, inside):
+        # x_j = x_i + delta along one axis, moved ulp by ulp to the last
+        # separation the reference criterion keeps (r_abs <= r_cut) or the
+        # first it drops; r_abs is |dx| when the other components are 0
+        xi = np.array(xi, dtype=np.float64)
+        xj = xi.copy()
+        xj[along] = xi[along] + rc
+        if xj[along] >= L:
+            xj[along] -= L
+
+        def r_of(xj):
+            d = xj[along] - xi[along]
+            if d < -L / 2.0:
+                d += L
+            elif d > L / 2.0:
+                d -= L
+            return np.sqrt(d * d)
+        while r_of(xj) > rc:
+            xj[along] = np.nextafter(xj[along], -np.inf)
+        while r_of(np.where(np.arange(3) == along, np.nextafter(xj[along], np.inf), xj)) <= rc:
+            xj[along] = np.nextafter(xj[along], np.inf)
+        if not inside:
+            xj[along] = np.nextafter(xj[along], np.inf)
+        return [xi, xj]
+
+    planted = []
+    for k, (xi, ax) in enumerate([([10.0, 20.0, 30.0], 0), ([top, 50.0, 60.0], 0), ([70.0, top, 80.0], 1),
+                                  ([90.0, 100.0, L - 0.5], 2), ([L - 0.3, 120.0, 130.0], 0),
+                                  ([140.0, 150.0, top - 0.6], 2)]):
+        planted += pair_at(xi, ax, inside=True)
+        q = list(xi)
+        q[(ax + 1) % 3] += 7.0
+        planted += pair_at(q, ax, inside=False)
+    # diagonal pairs straddling the corner at r_cut (FP32 rounding on every axis)
+    h = rc / np.sqrt(3.0)
+    planted += [[top, top, top], [h - 1e-6 - (L - top), h - 1e-6 - (L - top), h - 1e-6 - (L - top)]]
+    planted += [[top - 1.0, top - 2.0, 3.0], [h - 1.0, h - 2.0, 3.0 + h]]
+    # |dx| = L / 2 exactly (no minimum-image correction at equality)
+    planted += [[0.0, 200.0, 200.0], [L / 2, 200.0, 200.0]]
+    pos.append(np.array(planted))
+    pos = np.concatenate(pos)
+    s = O.State([len(pos)], [AR_MASS], [L] * 3, pos=pos, vel=np.zeros_like(pos))
+    O.apply_boundary_conditions(s)
+    return s, len(planted)
+
+
+def test_adversarial_pairs_in_the_1m_atom_box():
+    """FP64 list forces equal the oracle's bit for bit at the planted pairs
+    of the 333.8 nm box, and the FP32 path (FP32-prefiltered list build)
+    keeps exactly the reference's pair set (potential.rs:186-204)."""
+    s, n_planted = big_box_adversarial()
+    ref = s.copy()
+    O.update_force_cells(ref)
+    kept = ref.force[-n_planted:].any(axis=1)
+    assert kept.sum() >= 12 and (~kept).sum() >= 12
+    e = engine_for(s, "fp64_list")
+    e.update_force()
+    d = e.download(("force", "virial"))
+    e.close()
+    assert np.array_equal(d["force"], ref.force) and np.array_equal(d["virial"], ref.temp)
+    f = engine_for(s, "fp32_list")
+    f.update_force()
+    g = f.download(("force", "virial"))
+    f.close()
+    assert np.array_equal(g["force"].any(axis=1), ref.force.any(axis=1))
+    assert np.array_equal(g["virial"] != 0, ref.temp != 0)
+    assert rms(g["force"] - ref.force) <= 1e-5 * rms(ref.force)
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_apply_boundary_conditions_entry(mode):
+    """md_apply_boundary_conditions: one conditional wrap per axis
+    (core/src/particle.rs:120-141), including coordinates more than a box
+    away, which one wrap leaves outside."""
+    s = liquid_state(3)
+    rng = np.random.default_rng(2)
+    s.pos = s.pos + rng.choice([-1.0, 0.0, 1.0, 1.7, -2.2], size=s.pos.shape) * s.box
+    e = engine_for(s, mode)
+    e.apply_boundary_conditions()
+    d = e.download(("x",))
+    O.apply_boundary_conditions(s)
+    assert np.array_equal(d["x"], s.pos)
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_upload_forces_and_momentum(mode):
+    """md_upload_forces hands the first half kick its forces
+    (integrator.rs:28-34), and md_macro's momentum is the reference's
+    sum of m v (macro_parameters/mod.rs:28-34)."""
+    s = liquid_state(3, temperature=130.0, seed=8)
+    e = engine_for(s, mode)
+    ref = s.copy()
+    O.update_force(ref)
+    f = ref.force * (1.0 + np.random.default_rng(3).uniform(-0.1, 0.1, ref.force.shape))
+    if mode == "fp32_list":
+        f = f.astype(np.float32).astype(np.float64)
+    e.upload_forces(f, ref.pot, ref.temp)
+    ref.force[:] = f
+    m = e.macro()
+    assert np.allclose(m["momentum"], O.momentum(ref, 0), rtol=0, atol=1e-12 * np.abs(ref.vel).sum() * AR_MASS)
+    e.step(0.002, 1)
+    d = e.download()
+    O.verlet_step(ref, 0.002)
+    if mode == "fp32_list":
+        assert np.allclose(d["v"], ref.vel, rtol=1e-6, atol=1e-9)
+    else:
+        assert np.array_equal(d["x"], ref.pos) and np.array_equal(d["v"], ref.vel)
+    m = e.macro()
+    # the device's sum of m v of its own state; the FP64 paths' state is the oracle's
+    scale = np.abs(d["v"]).sum() * AR_MASS
+    assert np.allclose(m["momentum"], (d["v"] * AR_MASS).sum(0), rtol=0, atol=1e-13 * scale)
+    if mode != "fp32_list":
+        assert np.allclose(m["momentum"], O.momentum(ref, 0), rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_momentum_invariant(mode):
+    """solver/src/lib.rs:59-75: 8 argon atoms on a 2x2x2 grid of edge
+    3.338339 in a box of twice that, MB at 273.15 K: every component of the
+    summed velocity stays below 1e-12 over 100,000 steps (checked after each
+    of the first 200 steps, then every 1000).  The FP32 path sums each
+    atom's pair forces in FP32, so where three or more atoms interact at
+    once the per-atom sums are antisymmetric only to FP32 rounding: its
+    bound is 1e-8."""
+    n = 8
+    pos = O.lattice_u([2, 2, 2], 3.338339)
+    vel = O.mb_velocities(n, 273.15, AR_MASS, 21)
+    s = O.State([n], [AR_MASS], [2 * 3.338339] * 3, pos=pos, vel=vel)
+    e = engine_for(s, mode)
+    e.update_force()
+    assert np.abs(e.download(("v",))["v"].sum(0)).max() < 1e-12
+    done = 0
+    while done < 100000:
+        k = 1 if done < 200 else 1000
+        e.step(0.002, k)
+        done += k
+        bound = 1e-12 if mode != "fp32_list" else 1e-8
+        assert np.abs(e.download(("v",))["v"].sum(0)).max() < bound, done
+
+
+@pytest.mark.parametrize("shape", ["liquid", "gas"])
+def test_bricks_with_unequal_tiled_edges(shape):
+    """Grids whose bricks are a cell longer along one or two axes that are
+    tiled, not whole (brick_dims in md_device.h, restated in tests/common.py
+    and asserted here): FP64 list forces equal the oracle's; a thin-skin run
+    of 200 steps equals the all-pairs path bit for bit."""
+    from tests.common import brick_dims, grid_dims
+    if shape == "liquid":
+        # SC 12^3 at a = 0.48: a 6^3 grid, bricks of 2 x 3 x 3 cells
+        s = sc_state(12, a=0.48, temperature=120.0, seed=3, jitter=0.05, jitter_seed=4)
+    else:
+        # 1,600 atoms in a 20 nm box: a 14 x 14 x 16 grid, bricks of 9 x 9 x 10;
+        # the first 1,600 sites of a jittered 12^3 lattice (no overlapping pairs)
+        a = 20.0 / 12
+        pos = O.lattice_u([12, 12, 12], a)[:1600]
+        pos = pos + np.random.default_rng(23).uniform(-0.4, 0.4, pos.shape) * a
+        s = O.State([1600], [AR_MASS], [20.0] * 3, pos=pos, vel=O.mb_velocities(1600, 273.15, AR_MASS, 23))
+        O.apply_boundary_conditions(s)
+    rlist = 2.5 * 0.3418 + 0.3 * 0.3418
+    gd = grid_dims(s.box, rlist, int(np.ceil(2.0 * s.n)))
+    edge, whole = brick_dims(gd, s.n)
+    print("grid", gd, "brick", edge, "whole", whole)
+    assert len(set(edge)) > 1 and not any(whole)
+    e = engine_for(s, "fp64_list")
+    e.update_force()
+    f = e.download(("force",))["force"]
+    ref = s.copy()
+    O.update_force_cells(ref)
+    assert np.count_nonzero(ref.force.any(axis=1)) > 100
+    assert np.array_equal(f, ref.force)
+    e.close()
+    thin = engine_for(s, "fp64_list", skin=0.02)
+    allp = engine_for(s, "fp64_allpairs")
+    for eng in (thin, allp):
+        eng.update_force()
+        eng.step(0.002, 200)
+    da, db = allp.download(), thin.download()
+    assert thin.stats()["rebuilds"] >= 5
+    assert np.array_equal(da["x"], db["x"]) and np.array_equal(da["v"], db["v"])
 
 
 @pytest.mark.parametrize("mode", ["fp64_list", "fp64_allpairs"])
@@ -545,9 +844,9 @@ def test_large_gas_interior_bricks(mode):
 @pytest.mark.parametrize("shape", ["liquid", "gas"])
 def test_elongated_boxes(shape):
     """Boxes much longer along one axis: grids whose short axes are whole
-    bricks while the long one is tiled, and brick edges that differ by a
-    cell.  FP64 list forces equal the oracle's; with a thin skin (frequent
-    rebuilds) 200 steps equal the all-pairs path bitwise."""
+    bricks while the long one is tiled.  FP64 list forces equal the
+    oracle's; with a thin skin (frequent rebuilds) 200 steps equal the
+    all-pairs path bitwise."""
     rng = np.random.default_rng(17)
     if shape == "liquid":
         a = 0.5256

@@ -114,6 +114,51 @@ def test_cell_variant_bitwise(case):
     assert np.array_equal(a.temp, b.temp)
 
 
+@pytest.mark.parametrize("case", ["liquid_nve", "gas_npt", "dilute_npt", "two_types_npt"])
+def test_cell_step_bitwise(case):
+    """orc_verlet_step_cells (forces through the cell grid) follows
+    orc_verlet_step bit for bit; the dilute gas has more cells than the
+    grid's budget of about two per particle, so the grid is coarsened."""
+    th = ba = None
+    t0 = p0 = 0.0
+    if case == "liquid_nve":
+        pos = O.lattice_fcc([4, 4, 4], 0.5256)
+        s = O.State([len(pos)], [AR_M], [4 * 0.5256] * 3, pos=pos,
+                    vel=O.mb_velocities(len(pos), 150.0, AR_M, 3))
+        s.pos += np.random.default_rng(1).uniform(-0.03, 0.03, s.pos.shape)
+        O.apply_boundary_conditions(s)
+    elif case in ("gas_npt", "dilute_npt"):
+        # jittered simple-cubic lattices (no overlapping pairs)
+        side, a, jit = (11, 0.5, 0.2) if case == "gas_npt" else (10, 1.25, 0.25)
+        pos = O.lattice_u([side] * 3, a)
+        pos += np.random.default_rng(5).uniform(-jit, jit, pos.shape) * a
+        s = O.State([len(pos)], [AR_M], [side * a] * 3, pos=pos,
+                    vel=O.mb_velocities(len(pos), 300.0, AR_M, 4))
+        O.apply_boundary_conditions(s)
+    else:
+        pos = O.lattice_u([8, 8, 8], 0.5)
+        pos += np.random.default_rng(7).uniform(-0.1, 0.1, pos.shape)
+        s = O.State([300, 212], [AR_M, 39.948], [4.0] * 3, pos=pos,
+                    vel=np.random.default_rng(8).normal(0, 0.3, pos.shape))
+        O.apply_boundary_conditions(s)
+    if case.endswith("npt"):
+        th, t0, ba, p0 = "b", 200.0, "b", 5.0
+    tab = O.potential_table(s.n_types, {(0, 1): O.new_lennard_jones(0.33, 1.2)})
+    a, b = s.copy(), s.copy()
+    for st in (a, b):
+        O.update_force(st, tab)
+    tha = thb = bea = beb = None
+    if th:
+        tha, thb = O.Thermostat.berendsen(1.0), O.Thermostat.berendsen(1.0)
+        bea, beb = O.Barostat.berendsen(1.0, 5.0), O.Barostat.berendsen(1.0, 5.0)
+    for _ in range(40):
+        O.verlet_step(a, 0.002, tab, bea, p0, tha, t0)
+        O.verlet_step(b, 0.002, tab, beb, p0, thb, t0, cells=True)
+    for k in ("pos", "vel", "force", "pot", "temp", "box"):
+        assert np.array_equal(getattr(a, k), getattr(b, k)), k
+    assert np.count_nonzero(a.force) > 0
+
+
 def np_temperature(vel, mass):
     m = np.longdouble(mass)
     v = vel.astype(np.longdouble)
