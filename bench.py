@@ -387,7 +387,7 @@ def main():
     achieved = bytes_per_launch / avg_step_s / 1e9 if step_n else 0.0
     log(f"[bench] k_step avg {avg_step_s * 1e6:.2f} us over {step_n} launches, "
         f"k_fin avg {fin_ms / max(fin_n, 1) * 1e3:.2f} us over {fin_n}; <n_l> {n_l:.4f}")
-    from moldyn_amd.rocprof_summary import read_traffic
+    from rocprof_summary import read_traffic
     traffic = read_traffic(args.traffic)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

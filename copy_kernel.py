@@ -3,7 +3,7 @@ counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE): the HIP runtime performs a
 hipMemcpy between two device buffers with its blit kernel, which reads and
 writes exactly the buffer size.
 
-python -m moldyn_amd.copy_kernel [--mib 1024] [--reps 5]
+python copy_kernel.py [--mib 1024] [--reps 5]
 """
 from __future__ import annotations
 

@@ -2,11 +2,11 @@
 
 Two commands:
 
-  python -m moldyn_amd.rocprof_summary stats DIR --command "..." --out FILE
+  python rocprof_summary.py stats DIR --command "..." --out FILE
       per-kernel duration statistics of a ``rocprofv3 --kernel-trace`` run
       (SQLite ``*_results.db`` or ``*kernel_trace.csv`` under DIR).
 
-  python -m moldyn_amd.rocprof_summary traffic --kernel REGEX \
+  python rocprof_summary.py traffic --kernel REGEX \
       --fetch DIR --write DIR --copy-fetch DIR --copy-write DIR \
       --copy-kernel REGEX --copy-bytes B --command "..." --out FILE
       HBM bytes per launch of the kernels matching REGEX from two ``--pmc``
@@ -149,7 +149,7 @@ def _emit(text, out):
 
 
 def main(argv=None):
-    ap = argparse.ArgumentParser(prog="python -m moldyn_amd.rocprof_summary")
+    ap = argparse.ArgumentParser(prog="python rocprof_summary.py")
     sub = ap.add_subparsers(dest="cmd", required=True)
     s = sub.add_parser("stats")
     s.add_argument("dir")
