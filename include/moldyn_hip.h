@@ -170,6 +170,21 @@ int md_barostat_update(md_ctx* ctx, md_barostat* ba, uint16_t type_id);
  * out[8..10] centre-of-mass velocity (mod.rs:12-25) */
 int md_macro(md_ctx* ctx, int replica, uint16_t type_id, double out[11]);
 
+/* No counterpart in the reference: md_macro for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops).  The state
+ * of one particle type is the caller's device memory: v (n x 3), pot and
+ * virial (n, Particle::potential / Particle::temp; either may be NULL for
+ * zeros); mass is the type's mass and box[3] (host) the boundary box.  The
+ * eleven values of md_macro are written to `out`, 11 doubles of device
+ * memory, by kernels launched on `stream` (a hipStream_t; NULL the null
+ * stream); the call neither synchronises nor copies to the host, and
+ * allocates nothing: its partial sums live in a fixed 106 KB array of
+ * device data per GPU, held until the process ends.  n must lie in
+ * [1, 2^40] (MD_ERR_ARG otherwise, message in md_global_error).  Calls
+ * that overlap on one GPU share that array and race. */
+int md_macro_noalloc(const double* v, const double* pot, const double* virial, int64_t n, double mass,
+                     const double box[3], double* out, void* stream);
+
 /* Lattice initialisers (host-side), initializer/position.rs:34-99:
  * kind 0 UnitCell::U (grid[0]*grid[1]*grid[2] sites), 1 UnitCell::FCC (4 per cell). */
 int md_init_lattice(int kind, const double start[3], const int64_t grid[3], double a, double* pos);

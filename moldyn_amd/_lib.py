@@ -40,7 +40,7 @@ EXPORTS = (
     "md_trainer_predict", "md_frame_writer_create", "md_frame_writer_save", "md_frame_writer_flush",
     "md_frame_writer_stats", "md_frame_writer_last_error", "md_frame_writer_destroy", "md_frame_load",
     "md_text_create", "md_text_destroy", "md_text_last_error", "md_text_table", "md_csv_write_table",
-    "md_init_positions", "md_init_velocities",
+    "md_init_positions", "md_init_velocities", "md_macro_noalloc",
 )
 
 
@@ -120,6 +120,7 @@ def load():
     L.md_barostat_calculate_myu.argtypes = [vp, bp, C.c_double, C.c_uint16]
     L.md_barostat_update.argtypes = [vp, bp, C.c_uint16]
     L.md_macro.argtypes = [vp, C.c_int, C.c_uint16, dp]
+    L.md_macro_noalloc.argtypes = [dp, dp, dp, C.c_int64, C.c_double, dp, dp, vp]
     L.md_init_lattice.argtypes = [C.c_int, dp, i64p, C.c_double, dp]
     L.md_init_mb_velocities.argtypes = [dp, C.c_int64, C.c_double, C.c_double, C.c_uint64]
     L.md_get_stats.argtypes = [vp, C.POINTER(MdStats)]

@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmoldyn_hip.so")
-SOURCES = ["md_kernels.hip", "md_mlp.hip", "md_mlp_train.hip", "md_api.hip", "md_train_api.hip", "md_frames.hip", "md_init.hip"]
+SOURCES = ["md_kernels.hip", "md_mlp.hip", "md_mlp_train.hip", "md_api.hip", "md_train_api.hip", "md_frames.hip", "md_init.hip",
+           "md_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else "hipcc")
 FLAGS = [
