@@ -68,22 +68,17 @@ def nve_64k(M, steps=10000):
     m1 = e.macro()
     d = e.download(("x", "v", "force", "box"))
     e.close()
-    ref = M.Engine([n], [MASS], d["box"], force_mode="fp64_list")
-    ref.upload(d["x"], d["v"], d["box"])
-    ref.update_force()
-    f64 = ref.download(("force",))["force"]
-    ref.close()
+    f32 = gpu_forces(M, d["x"], d["box"], "fp32_list")
+    f64 = gpu_forces(M, d["x"], d["box"], "fp64_list")
     e0 = m0["kinetic"] + m0["potential"]
     e1 = m1["kinetic"] + m1["potential"]
     ns = steps * DT * 1e-3
-    err = d["force"] - f64
-    rms_f = float(np.sqrt(np.mean(f64 ** 2)))
-    return {"workload": "argon SC 40^3 (64000 atoms), +-0.02a jitter, 120 K, NVE, skin 0.3 sigma, FP32 forces",
-            "steps": steps, "atom_steps_per_s": n * steps / wall,
-            "energy_drift_per_ns": (e1 - e0) / ns, "energy_drift_rel_per_ns": (e1 - e0) / abs(e0) / ns,
-            "total_energy_start": e0,
-            "force_rmse_rel_final": float(np.sqrt(np.mean(err ** 2))) / rms_f if rms_f > 0 else 0.0,
-            "force_max_abs_err_final": float(np.abs(err).max())}
+    res = {"workload": "argon SC 40^3 (64000 atoms), +-0.02a jitter, 120 K, NVE, skin 0.3 sigma, FP32 forces",
+           "steps": steps, "atom_steps_per_s": n * steps / wall,
+           "energy_drift_per_ns": (e1 - e0) / ns, "energy_drift_rel_per_ns": (e1 - e0) / abs(e0) / ns,
+           "total_energy_start": e0,
+           "force_error_final_vs_fp64_list": force_error(f32, f64)}
+    return res, {"x": d["x"], "v": d["v"], "box": d["box"], "f32": f32}
 
 
 def replicas_256(M, steps=2000, seq_sample=48):
@@ -112,9 +107,10 @@ def replicas_256(M, steps=2000, seq_sample=48):
         ba = (L.MdBarostat * R)(*[M.barostat("berendsen", 1.0, 5.0, float(p0[r])) for r in range(R)])
         return th, ba
 
-    # the skin for this batch by the autotuner, on the state half-way through
-    # the run (the barostats compress most replicas, which lengthens the
-    # lists; dilute gases favour a wide skin, denser ones a thinner)
+    # skin, block size, tiles per block and cell budget for this batch by the
+    # autotuner, on the state half-way through the run (the barostats
+    # compress most replicas, which lengthens the lists; dilute gases favour
+    # a wide skin, denser ones a thinner)
     from moldyn_amd.autotune import autotune
     xs, vs = np.concatenate(pos), np.concatenate(vel)
     e0 = M.Engine(counts[:, None], [MASS], boxes, force_mode="fp32_list")
@@ -127,9 +123,8 @@ def replicas_256(M, steps=2000, seq_sample=48):
     # 1000-step trials: about ten replicas rebuild per launch, so shorter
     # windows rank the skins by which replicas happened to rebuild in them
     best, _ = autotune(counts[:, None], [MASS], mid["box"], mid["x"], mid["v"], DT, controls, steps=1000,
-                       warmup=20, blocks=(256,), tiles=(0,), cells=(2.0,))
-    skin = best["skin"]
-    e = M.Engine(counts[:, None], [MASS], boxes, force_mode="fp32_list", skin=skin)
+                       warmup=20, blocks=(128, 256, 512), tiles=(0, 1, 2, 4), cells=(1.0, 2.0, 4.0), log=log)
+    e = M.Engine(counts[:, None], [MASS], boxes, force_mode="fp32_list", **best)
     e.upload(xs, vs, boxes)
     e.update_force()
     th, ba = controls()
@@ -143,7 +138,7 @@ def replicas_256(M, steps=2000, seq_sample=48):
     idx = rng.choice(R, seq_sample, replace=False)
     wall_s, atoms_s = 0.0, 0
     for r in idx:
-        one = M.Engine([int(counts[r])], [MASS], boxes[r], force_mode="fp32_list", skin=skin)
+        one = M.Engine([int(counts[r])], [MASS], boxes[r], force_mode="fp32_list", **best)
         one.upload(pos[r], vel[r], boxes[r])
         one.update_force()
         th1 = M.thermostat("berendsen", 10.0, float(t0[r]))
@@ -157,7 +152,7 @@ def replicas_256(M, steps=2000, seq_sample=48):
     seq_rate = atoms_s * steps / wall_s
     return {"workload": "256 argon replicas, N log-uniform in [500, 32768] (seed 4), a in [3.0, 4.2], "
                         "Berendsen T0 in [80, 400] K, P0 in [0.5, 5.0], FP32 forces",
-            "skin_nm": skin,
+            "autotuned": best,
             "atoms": total, "steps": steps, "batched_atom_steps_per_s": total * steps / wall_b,
             "batched_ms_per_step": 1e3 * wall_b / steps,
             "sequential_atom_steps_per_s": seq_rate,
@@ -294,15 +289,206 @@ def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log):
             "speed_vs_lj": res["mlp"]["atom_steps_per_s"] / res["fp32_list"]["atom_steps_per_s"]}
 
 
+def kernel_roofline(e, n, steps, block_size, traffic):
+    """Roofline of k_step from a profiled pass: stream time per launch on the
+    device clock, algorithmic bytes A = n (96 + 28 <n_l>) per launch."""
+    step_ms, step_n = e.kernel_time("step")
+    fin_ms, fin_n = e.kernel_time("finalize")
+    span_ms, span_n = e.kernel_time("step", span=True)
+    st = e.stats()
+    avg_s = step_ms / max(step_n, 1) * 1e-3
+    # mean list length over the launches (entries read / atoms / launches)
+    n_l = st["list_visits"] / (n * max(span_n, 1)) if span_n else st["mean_list_len"]
+    # step records in and out (48 B each), 4 B index + 24 B position per list entry
+    bytes_per_launch = n * (96.0 + 28.0 * n_l)
+    achieved = bytes_per_launch / avg_s / 1e9 if step_n else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": f"k_step<float,{block_size}>", "avg_us": round(avg_s * 1e6, 3),
+            "avg_us_basis": "stream time: previous kernel end to kernel end on the GPU clock, back-to-back "
+                            "launches (the interval rocprofv3 reports)",
+            "span_avg_us": round(span_ms / max(span_n, 1) * 1e3, 3), "launches_timed": step_n,
+            "steps": steps, "mean_list_len": round(n_l, 5),
+            "k_fin_avg_us": round(fin_ms / max(fin_n, 1) * 1e3, 3),
+            "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+
+
+def force_error(f, ref):
+    """Force error of f against ref (n x 3); relative figures are None when
+    no atom of ref has a force (the comparison would be vacuous)."""
+    err = f - ref
+    with_force = int(np.count_nonzero(np.abs(ref).sum(1)))
+    rms_f = float(np.sqrt(np.mean(ref ** 2)))
+    rmse = float(np.sqrt(np.mean(err ** 2)))
+    out = {"rmse": rmse, "rms_force": rms_f, "max_abs": float(np.abs(err).max()),
+           "atoms_with_force": with_force, "atoms": len(ref)}
+    if with_force == 0:
+        out.update({"rmse_rel": None, "max_rel": None, "degenerate": True})
+    else:
+        out.update({"rmse_rel": rmse / rms_f, "max_rel": float(np.abs(err).max()) / rms_f, "degenerate": False})
+    return out
+
+
+def gpu_forces(M, x, box, mode):
+    n = len(x)
+    e = M.Engine([n], [MASS], box, force_mode=mode)
+    e.upload(x, np.zeros_like(x), box)
+    e.update_force()
+    f = e.download(("force",))["force"]
+    e.close()
+    return f
+
+
+def cpu_leg(M, args, n, pos, vel, box, final, nve_final):
+    """The CPU side, on the host cores of the GPU's share: the FP64 oracle
+    (a restatement of the reference solver) timed as the baseline, its
+    cell-grid variant beside it, and the oracle as the checker of the GPU
+    results (forces on states with pairs, config-1 mean T and P, 100-step
+    trajectory deviation)."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or len(os.sched_getaffinity(0))
+    host = {"host_cpus_visible": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "threads": threads}
+    tab = O.potential_table(1)
+    # the reference algorithm: update_force rows (potential.rs:158-216), a
+    # bounded sample of the O(N^2) step, scaled to atom-steps/s
+    s = O.State([n], [MASS], box, pos=pos, vel=vel)
+    rows = 16 * threads
+    t0 = time.perf_counter()
+    O.update_force_rows(s, 0, rows, tab, threads)
+    dt_probe = time.perf_counter() - t0
+    rows = int(min(n, max(rows, rows * args.cpu_seconds / max(dt_probe, 1e-6))))
+    t0 = time.perf_counter()
+    O.update_force_rows(s, 0, rows, tab, threads)
+    dt_cpu = time.perf_counter() - t0
+    cpu_value = rows / dt_cpu  # one step needs all n rows: n / (dt * n / rows)
+    log(f"[bench] cpu baseline: {cpu_value:.4e} atom-steps/s on {threads} threads")
+    # the same FP64 arithmetic through a cell grid (bitwise the same forces),
+    # whole integrator steps under the thermostat and barostat
+    cs = O.State([n], [MASS], box, pos=pos, vel=vel)
+    O.update_force_cells(cs, tab, threads)
+    oth, oba = O.Thermostat.berendsen(10.0), O.Barostat.berendsen(1.0, 5.0)
+    O.verlet_step(cs, DT, tab, oba, 1.01325, oth, 300.0, threads, cells=True)
+    t0 = time.perf_counter()
+    for _ in range(args.cpu_cell_steps):
+        O.verlet_step(cs, DT, tab, oba, 1.01325, oth, 300.0, threads, cells=True)
+    dt_cells = time.perf_counter() - t0
+    cell_value = n * args.cpu_cell_steps / dt_cells
+    log(f"[bench] cpu cell-list variant: {cell_value:.4e} atom-steps/s")
+    cpu = {"value": cpu_value, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+           "sample": f"reference update_force (potential.rs:158-216) rows 0..{rows} of the {n}-atom initial "
+                     f"state, each row against all {n} atoms, FP64, OpenMP {threads} threads, {dt_cpu:.1f} s; "
+                     f"the O(N) integrator passes (1e-6 of this work at this size) omitted",
+           **host,
+           "cell_list_variant": {"value": cell_value, "unit": "atom-steps/s", "steps": args.cpu_cell_steps,
+                                 "sample": f"{args.cpu_cell_steps} whole Berendsen NPT steps of the {n}-atom "
+                                           f"state (integrator.rs:14-64) with the forces through a cell grid, "
+                                           f"FP64, OpenMP {threads} threads, {dt_cells:.2f} s"}}
+
+    acc = {}
+    # forces of the bench's own final state, FP32 GPU against the oracle
+    fs = O.State([n], [MASS], final["box"], pos=final["x"], vel=final["v"])
+    O.update_force_cells(fs, tab, threads)
+    acc["force_1M_final_state"] = {"state": f"final state of the {args.full_steps}-step run, box "
+                                            f"{final['box'][0]:.4f} nm",
+                                   **force_error(final["f32"], fs.force)}
+    acc["force_1M_final_state"]["fp64_list_bitwise"] = bool(np.array_equal(final["f64"], fs.force))
+    # dense fixtures: FCC liquid argon (rho sigma^3 ~ 0.8) and a gas at ten
+    # times the workload's density
+    rng = np.random.default_rng(5)
+    liq = O.lattice_fcc([8, 8, 8], 0.5256)
+    lbox = np.full(3, 8 * 0.5256)
+    liq = np.mod(liq + rng.uniform(-0.02, 0.02, liq.shape), lbox)
+    ls = O.State([len(liq)], [MASS], lbox, pos=liq)
+    O.update_force(ls, tab, threads)
+    acc["force_liquid_fcc"] = {"state": "FCC argon 8^3 cells (2048 atoms), a = 0.5256 nm, +-0.02 nm jitter",
+                               **force_error(gpu_forces(M, liq, lbox, "fp32_list"), ls.force)}
+    a10 = A_LAT / 10.0 ** (1.0 / 3.0)
+    gas = O.lattice_u([30, 30, 30], a10)
+    gbox = np.full(3, 30 * a10)
+    gas = np.mod(gas + rng.uniform(-0.25, 0.25, gas.shape) * a10, gbox)
+    gs = O.State([len(gas)], [MASS], gbox, pos=gas)
+    O.update_force_cells(gs, tab, threads)
+    acc["force_gas_10x"] = {"state": f"27000 argon atoms at ten times the workload's density (SC a = {a10:.4f} nm, "
+                                     f"+-0.25 a jitter)",
+                            **force_error(gpu_forces(M, gas, gbox, "fp32_list"), gs.force)}
+    if nve_final is not None:
+        ns = O.State([len(nve_final["x"])], [MASS], nve_final["box"], pos=nve_final["x"], vel=nve_final["v"])
+        O.update_force_cells(ns, tab, threads)
+        acc["force_64k_nve_final_state"] = {"state": "final state of the 10,000-step 64k NVE run",
+                                            **force_error(nve_final["f32"], ns.force)}
+    # config 1: 1,000 atoms, Berendsen T and P, 1,000 steps; the FP32 path's
+    # temperature and pressure every step against the oracle's
+    c1 = O.lattice_u([10, 10, 10], A_LAT)
+    c1v = M.mb_velocities(1000, 273.15, MASS, 1)
+    c1box = np.full(3, 10 * A_LAT)
+    e = M.Engine([1000], [MASS], c1box, force_mode="fp32_list")
+    e.upload(c1, c1v, c1box)
+    e.update_force()
+    ref = O.State([1000], [MASS], c1box, pos=c1, vel=c1v)
+    O.apply_boundary_conditions(ref)
+    O.update_force(ref, tab, threads)
+    th, ba = controls(M)
+    oth, oba = O.Thermostat.berendsen(10.0), O.Barostat.berendsen(1.0, 5.0)
+    tg, pg, to, po, dx100 = [], [], [], [], None
+    for k in range(1000):
+        e.step(DT, 1, th, ba)
+        O.verlet_step(ref, DT, tab, oba, 1.01325, oth, 300.0, threads)
+        m = e.macro()
+        mv = O.center_of_mass_velocity(ref, 0)
+        tg.append(m["temperature"])
+        pg.append(m["pressure"])
+        to.append(O.temperature(O.thermal_energy(ref, 0, mv), 1000))
+        po.append(O.pressure(ref, 0, mv))
+        if k == 99:
+            d = e.download(("x",))["x"] - ref.pos
+            d -= ref.box * np.round(d / ref.box)
+            dx100 = float(np.abs(d).max())
+    fin = e.download(("x", "box"))
+    e.close()
+    d = fin["x"] - ref.pos
+    d -= ref.box * np.round(d / ref.box)
+    tg, pg, to, po = map(np.array, (tg, pg, to, po))
+    acc["config1_npt_1000_steps"] = {
+        "path": "fp32_list", "mean_T": float(tg.mean()), "mean_T_oracle": float(to.mean()),
+        "mean_P": float(pg.mean()), "mean_P_oracle": float(po.mean()),
+        "max_abs_dT": float(np.abs(tg - to).max()), "max_rel_dP": float(np.abs(pg - po).max() / np.abs(po).max()),
+        "max_dx_100_steps": dx100, "max_dx_1000_steps": float(np.abs(d).max()),
+        "box_rel_diff": float(np.abs(fin["box"] - ref.box).max() / ref.box[0])}
+    # 100-step trajectory deviation where forces act: the liquid, NVE
+    lv = M.mb_velocities(len(liq), 120.0, MASS, 5)
+    e = M.Engine([len(liq)], [MASS], lbox, force_mode="fp32_list")
+    e.upload(liq, lv, lbox)
+    e.update_force()
+    e.step(DT, 100)
+    got = e.download(("x", "v"))
+    e.close()
+    ref = O.State([len(liq)], [MASS], lbox, pos=liq, vel=lv)
+    O.update_force(ref, tab, threads)
+    for _ in range(100):
+        O.verlet_step(ref, DT, tab, nthreads=threads)
+    d = got["x"] - ref.pos
+    d -= lbox * np.round(d / lbox)
+    acc["trajectory_100_steps_liquid"] = {
+        "state": "FCC argon liquid (2048 atoms), 120 K, NVE, FP32 path against the FP64 oracle",
+        "max_dx_nm": float(np.abs(d).max()), "rms_dx_nm": float(np.sqrt(np.mean(d ** 2))),
+        "max_dv_rel": float(np.abs(got["v"] - ref.vel).max() / np.sqrt(np.mean(ref.vel ** 2)))}
+    log(f"[bench] accuracy vs oracle: {acc}")
+    return cpu, acc
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--full-steps", type=int, default=5000,
+                    help="length of the workload's full run, reported beside the timed steps")
     ap.add_argument("--n-side", type=int, default=N_SIDE)
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-cell-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines and the oracle checks")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--skin", type=float, default=0.5, help="nm (the autotuner's choice for this workload)")
     ap.add_argument("--block-size", type=int, default=256)
@@ -310,7 +496,7 @@ def main():
     ap.add_argument("--tiles-per-block", type=int, default=0)
     ap.add_argument("--graph-chunk", type=int, default=32, help="launches per captured graph (1: none)")
     ap.add_argument("--no-profile", action="store_true",
-                    help="no per-launch device timestamps in the timed region (no roofline)")
+                    help="no per-launch device timestamps (no roofline)")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--tune-steps", type=int, default=300)
     ap.add_argument("--no-extras", action="store_true",
@@ -326,18 +512,21 @@ def main():
         sys.exit(2)
 
     import moldyn_amd as M
+    from rocprof_summary import read_traffic
+    traffic = read_traffic(args.traffic)
 
     n, pos, vel, box = initial_state(M, args.n_side)
     log(f"[bench] state ready: {n} atoms, box {box[0]:.4f}")
     opts = {"skin": args.skin,
             "block_size": args.block_size, "cells_per_atom": args.cells_per_atom,
             "tiles_per_block": args.tiles_per_block}
+    full_steps = max(args.full_steps, args.steps)
     if not args.no_tune:
         from moldyn_amd.autotune import autotune
-        # tune on the state half-way through the timed run: the barostat
-        # compresses the gas several-fold over the run, which lengthens the
-        # lists and shortens the rebuild interval
-        pre = args.warmup + args.steps // 2
+        # tune on the state half-way through the full run: the barostat
+        # compresses the gas several-fold over it, which lengthens the lists
+        # and shortens the rebuild interval
+        pre = args.warmup + full_steps // 2
         e0 = M.Engine([n], [MASS], box, force_mode="fp32_list", skin=0.5, cells_per_atom=4.0)
         e0.upload(pos, vel, box)
         e0.update_force()
@@ -351,7 +540,7 @@ def main():
         log(f"[bench] autotuned options: {opts}")
     e = M.Engine([n], [MASS], box, force_mode="fp32_list", graph_chunk=args.graph_chunk, **opts)
 
-    def run(profile):
+    def run(steps, profile):
         # warmup then the timed steps, from the initial state
         e.upload(pos, vel, box)
         e.update_force()
@@ -359,69 +548,60 @@ def main():
         e.step(DT, args.warmup, th, ba)
         e.set_profiling(profile)
         t0 = time.perf_counter()
-        e.step(DT, args.steps, th, ba)  # returns after the device finished
+        e.step(DT, steps, th, ba)  # returns after the device finished
         t1 = time.perf_counter()
-        return t1 - t0, e.stats()
+        st = e.stats()
+        if profile:
+            roof = kernel_roofline(e, n, steps, opts["block_size"], traffic)
+            e.set_profiling(False)
+            return t1 - t0, st, roof
+        return t1 - t0, st, None
 
-    wall, st = run(False)
+    # the timed region: exactly --steps steps
+    wall, st, _ = run(args.steps, False)
     value = n * args.steps / wall
     ms_per_step = 1e3 * wall / args.steps
     log(f"[bench] timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value:.4e} atom-steps/s; stats {st}")
-    # The same trajectory again with per-launch device timestamps on every
-    # kernel (the graphs keep running; each launch is timed from its first
-    # block's start to its last block's end on the GPU wall clock).
-    step_ms = fin_ms = span_ms = 0.0
-    step_n = fin_n = span_n = 0
+    timed_roof = None
     if not args.no_profile:
-        wall_p, st = run(True)
-        step_ms, step_n = e.kernel_time("step")
-        fin_ms, fin_n = e.kernel_time("finalize")
-        span_ms, span_n = e.kernel_time("step", span=True)
-        e.set_profiling(False)
-        log(f"[bench] profiled pass: {1e3 * wall_p / args.steps:.4f} ms/step")
-    avg_step_s = step_ms / max(step_n, 1) * 1e-3
-    # mean list length over the timed launches (entries read / atoms / launches)
-    n_l = st["list_visits"] / (n * max(span_n, 1)) if span_n else st["mean_list_len"]
-    # step records in and out (48 B each), 4 B index + 24 B position per list entry
-    bytes_per_launch = n * (96.0 + 28.0 * n_l)
-    achieved = bytes_per_launch / avg_step_s / 1e9 if step_n else 0.0
-    log(f"[bench] k_step avg {avg_step_s * 1e6:.2f} us over {step_n} launches, "
-        f"k_fin avg {fin_ms / max(fin_n, 1) * 1e3:.2f} us over {fin_n}; <n_l> {n_l:.4f}")
-    from rocprof_summary import read_traffic
-    traffic = read_traffic(args.traffic)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": f"k_step<float,{opts['block_size']}>", "avg_us": round(avg_step_s * 1e6, 3),
-                "avg_us_basis": "stream time: previous kernel end to kernel end on the GPU clock, "
-                                "back-to-back launches (the interval rocprofv3 reports)",
-                "span_avg_us": round(span_ms / max(span_n, 1) * 1e3, 3),
-                "launches_timed": step_n, "mean_list_len_timed": round(n_l, 5),
-                "k_fin_avg_us": round(fin_ms / max(fin_n, 1) * 1e3, 3),
-                "algorithmic_bytes_per_launch": round(bytes_per_launch)}
+        # the same steps again with per-launch device timestamps (the graphs
+        # keep running; the results are bitwise those of the timed pass)
+        _, _, timed_roof = run(args.steps, True)
+        log(f"[bench] timed region k_step {timed_roof['avg_us']} us, <n_l> {timed_roof['mean_list_len']}")
+    # the workload's full run (5,000 steps), whatever --steps was
+    if full_steps == args.steps:
+        wall_f, st_f = wall, st
+    else:
+        wall_f, st_f, _ = run(full_steps, False)
+    final = e.download(("x", "v", "box"))
+    full_roof = None
+    if not args.no_profile:
+        if full_steps == args.steps:
+            full_roof = timed_roof
+        else:
+            _, _, full_roof = run(full_steps, True)
+    full_run = {"steps": full_steps, "atom_steps_per_s": n * full_steps / wall_f,
+                "ms_per_step": 1e3 * wall_f / full_steps, "rebuilds": st_f["rebuilds"],
+                "box_final": float(final["box"][0]), "roofline": full_roof}
+    log(f"[bench] full run: {full_run}")
+    # the representative roofline is the full run's; the timed region's beside it
+    roofline = dict(full_roof) if full_roof else {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS,
+                                                  "unit": "GB/s", "frac": None, "traffic": traffic}
+    roofline["basis"] = f"k_step over the full {full_steps}-step run (the workload), device clock"
+    roofline["timed_region"] = ({k: timed_roof[k] for k in ("achieved", "frac", "avg_us", "mean_list_len",
+                                                             "launches_timed", "k_fin_avg_us")}
+                                if timed_roof else None)
 
-    # FP32 force error on the final state, against the FP64 path (bitwise the
-    # reference update_force, see tests/test_gpu_parity.py), both evaluated at
-    # the same positions (a step's stored forces precede its barostat scaling)
-    d = e.download(("x", "v", "box"))
-    e.upload(d["x"], d["v"], d["box"])
-    e.update_force()
-    f32 = e.download(("force",))["force"]
+    # forces of the final state at identical positions: the FP32 path and the
+    # FP64 list path (bitwise the reference update_force)
+    final["f32"] = gpu_forces(M, final["x"], final["box"], "fp32_list")
+    final["f64"] = gpu_forces(M, final["x"], final["box"], "fp64_list")
     e.close()
-    ref = M.Engine([n], [MASS], d["box"], force_mode="fp64_list")
-    ref.upload(d["x"], d["v"], d["box"])
-    ref.update_force()
-    f64 = ref.download(("force",))["force"]
-    ref.close()
-    err = f32 - f64
-    rms_f = float(np.sqrt(np.mean(f64 ** 2)))
-    force_rmse = float(np.sqrt(np.mean(err ** 2)))
-    force_err = {"rmse": force_rmse, "rms_force": rms_f,
-                 "rmse_rel": force_rmse / rms_f if rms_f > 0 else 0.0,
-                 "max_abs": float(np.abs(err).max()),
-                 "atoms_with_force": int(np.count_nonzero(np.abs(f64).sum(1)))}
-    log(f"[bench] force error vs fp64: {force_err}")
+    force_err = {"state": f"final state of the {full_steps}-step run", **force_error(final["f32"], final["f64"]),
+                 "reference": "FP64 list path on the GPU (bitwise the oracle's update_force)"}
+    log(f"[bench] force error vs fp64 list: {force_err}")
 
-    plain = None
+    comparators = {}
     if not args.no_plain:
         p = M.Engine([n], [MASS], box, force_mode="fp64_allpairs")
         p.upload(pos, vel, box)
@@ -433,15 +613,34 @@ def main():
         t1 = time.perf_counter()
         p.close()
         pv = n * args.plain_steps / (t1 - t0)
-        plain = {"value": pv, "unit": "atom-steps/s", "steps": args.plain_steps,
-                 "ms_per_step": 1e3 * (t1 - t0) / args.plain_steps,
-                 "speedup": value / pv}
-        log(f"[bench] fp64 plain all-pairs: {pv:.4e} atom-steps/s ({plain['ms_per_step']:.1f} ms/step)")
+        comparators["fp64_plain"] = {"value": pv, "unit": "atom-steps/s", "steps": args.plain_steps,
+                                     "ms_per_step": 1e3 * (t1 - t0) / args.plain_steps, "speedup": value / pv,
+                                     "what": "FP64 all-pairs restatement of update_force (potential.rs:158-216), "
+                                             "no list, same NPT controls"}
+        log(f"[bench] fp64 plain all-pairs: {pv:.4e} atom-steps/s ({comparators['fp64_plain']['ms_per_step']:.1f} ms/step)")
+    # the same list machinery in FP64 throughout (reference precision)
+    q = M.Engine([n], [MASS], box, force_mode="fp64_list", graph_chunk=args.graph_chunk, **opts)
+    q.upload(pos, vel, box)
+    q.update_force()
+    th3, ba3 = controls(M)
+    q.step(DT, args.warmup, th3, ba3)
+    t0 = time.perf_counter()
+    q.step(DT, full_steps, th3, ba3)
+    t1 = time.perf_counter()
+    q.close()
+    qv = n * full_steps / (t1 - t0)
+    comparators["fp64_list"] = {"value": qv, "unit": "atom-steps/s", "steps": full_steps,
+                                "ms_per_step": 1e3 * (t1 - t0) / full_steps,
+                                "speedup_of_fp32": full_run["atom_steps_per_s"] / qv,
+                                "what": "the cell-list / Verlet-skin path with FP64 pair forces (bitwise the "
+                                        "reference), full run"}
+    log(f"[bench] fp64 list: {qv:.4e} atom-steps/s")
 
     extras = {}
+    nve_final = None
     if not args.no_extras:
         log("[bench] 64k NVE")
-        extras["nve_64k"] = nve_64k(M)
+        extras["nve_64k"], nve_final = nve_64k(M)
         log(f"[bench] nve_64k: {extras['nve_64k']}")
         log("[bench] 256 replicas")
         extras["replicas_256"] = replicas_256(M)
@@ -453,26 +652,7 @@ def main():
         extras["mlp_surrogate"] = mlp_surrogate(M, n_frames=args.mlp_frames, epochs=args.mlp_epochs)
         log(f"[bench] mlp_surrogate: {extras['mlp_surrogate']}")
 
-    cpu = None
-    if not args.no_cpu:
-        from oracle import oracle as O
-        threads = max(1, min(os.cpu_count() or 1, 16))
-        s = O.State([n], [MASS], box, pos=pos, vel=vel)
-        tab = O.potential_table(1)
-        rows = 16 * threads
-        t0 = time.perf_counter()
-        O.update_force_rows(s, 0, rows, tab, threads)
-        dt_probe = time.perf_counter() - t0
-        rows = int(min(n, max(rows, rows * args.cpu_seconds / max(dt_probe, 1e-6))))
-        t0 = time.perf_counter()
-        O.update_force_rows(s, 0, rows, tab, threads)
-        dt_cpu = time.perf_counter() - t0
-        cpu_value = rows / dt_cpu  # one step needs all n rows: n / (dt * n / rows)
-        cpu = {"value": cpu_value, "unit": "atom-steps/s", "cores": threads, "kind": "port",
-               "sample": f"reference update_force (potential.rs:158-216) rows 0..{rows} of the "
-                         f"{n}-atom initial state, each row against all {n} atoms, FP64, "
-                         f"OpenMP {threads} threads, {dt_cpu:.1f} s; O(N) integrator passes omitted"}
-        log(f"[bench] cpu baseline: {cpu_value:.4e} atom-steps/s on {threads} threads")
+    cpu, acc = (None, None) if args.no_cpu else cpu_leg(M, args, n, pos, vel, box, final, nve_final)
 
     out = {
         "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1,
@@ -483,15 +663,18 @@ def main():
                                f"tau 10, Berendsen P 1.01325 (1, 5), dt {DT}, Verlet list",
                    "n_atoms": n, "force_mode": "fp32_list",
                    "skin_nm": opts["skin"], "block_size": opts["block_size"],
+                   "tiles_per_block": opts["tiles_per_block"],
                    "cells_per_atom": opts["cells_per_atom"], "autotuned": not args.no_tune,
-                   "rebuilds": st["rebuilds"], "mean_list_len": n_l},
+                   "rebuilds": st["rebuilds"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "fp64_plain": plain,
+        "full_run": full_run,
+        "comparators": comparators,
         "force_error_vs_fp64": force_err,
+        "accuracy_vs_oracle": acc,
         "other_workloads": extras or None,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out, default=float), flush=True)
 
 
 if __name__ == "__main__":
