@@ -470,6 +470,10 @@ static int alloc_base(md_ctx* c) {
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * (size_t)std::max(c->n_blocks, 1))))
         return rc;
     d.partial = (double*)p;
+    d.acc_mode = (c->R == 1 && c->T == 1) ? 1 : 0;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(unsigned long long) * kAccWords))) return rc;
+    d.acc = (unsigned long long*)p;
+    HIPCHK(c, hipMemset(p, 0, sizeof(unsigned long long) * kAccWords));
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double) * NS * c->groups.size()))) return rc;
     d.sums = (double*)p;
     HIPCHK(c, hipMemset(p, 0, sizeof(double) * NS * c->groups.size()));
@@ -532,6 +536,7 @@ static RepParams base_params(md_ctx* c) {
     P.margin = 1e-9 * c->rlist;
     P.insert_growth = 1.0;
     P.target_step = 0;
+    P.full_dst = -1;
     return P;
 }
 
@@ -671,12 +676,17 @@ static int ensure_graph(md_ctx* c, int len) {
 
 // ------------------------------------------------------------------ snapshot
 
+// The step-record path leaves each replica's starting buffer untouched
+// during a chunk (its last step writes the other one), so its snapshot
+// needs no records.
+static bool records_in_snapshot(const md_ctx* c) { return !c->d.compact; }
+
 static size_t state_bytes(md_ctx* c) {
     // both record buffers (replicas after a split step sit at different
     // parities), pot, vir, the group sums the first step's controls derive
     // from, and the surrogate's counters
-    return (2 * rec_size(c) + 2 * sizeof(double)) * (size_t)c->n_slots + sizeof(double) * NS * c->groups.size() +
-           (c->d.mlp_stat ? sizeof(MlpStat) * c->R : 0);
+    const size_t recs = records_in_snapshot(c) ? (2 * rec_size(c) + 2 * sizeof(double)) * (size_t)c->n_slots : 0;
+    return recs + sizeof(double) * NS * c->groups.size() + (c->d.mlp_stat ? sizeof(MlpStat) * c->R : 0);
 }
 
 static int snapshot(md_ctx* c, bool save) {
@@ -687,9 +697,11 @@ static int snapshot(md_ctx* c, bool save) {
         HIPCHK(c, hipMalloc((void**)&c->snap_ctrl, sizeof(RepCtrl) * c->R));
     }
     char* s = (char*)c->snap;
+    const bool recs = records_in_snapshot(c);
     void* arrs[6] = {c->d.rec[0], c->d.rec[1], c->d.pot, c->d.vir, c->d.sums, c->d.mlp_stat};
-    const size_t sz[6] = {rec_size(c) * (size_t)c->n_slots, rec_size(c) * (size_t)c->n_slots, n, n,
-                          sizeof(double) * NS * c->groups.size(), c->d.mlp_stat ? sizeof(MlpStat) * c->R : 0};
+    const size_t rs = recs ? rec_size(c) * (size_t)c->n_slots : 0, ps = recs ? n : 0;
+    const size_t sz[6] = {rs, rs, ps, ps, sizeof(double) * NS * c->groups.size(),
+                          c->d.mlp_stat ? sizeof(MlpStat) * c->R : 0};
     size_t off = 0;
     for (int k = 0; k < 6; ++k) {
         if (sz[k] == 0) continue;
@@ -864,6 +876,7 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
         for (int r = 0; r < c->R; ++r) {
             P[r].target_step = step0 + done + cn;
             P[r].use_psi_in = first ? 1 : 0;
+            P[r].full_dst = c->d.compact ? ((C[r].cur & 1) ^ 1) : -1;
         }
         if ((rc = write_params(c, P))) return rc;
         if ((rc = reset_stats(c))) return rc;

@@ -103,11 +103,11 @@ __device__ __forceinline__ double wave_total(double v) {
 }
 
 // Block reduction of NS columns in a fixed order (wave scan tree, then waves
-// in index order); the columns set in mask are stored to out[k * stride]
-// after the closing barrier, so no thread waits for the stores.
+// in index order); thread k < NS receives column k's total when mask has it
+// (returns false otherwise).  It returns after the closing barrier, so no
+// thread waits for the stores a caller makes of the totals.
 template <int BS>
-__device__ __forceinline__ void block_store(const double (&v)[NS], double* out, int64_t stride,
-                                            uint32_t mask) {
+__device__ __forceinline__ bool block_reduce(const double (&v)[NS], uint32_t mask, double& total) {
     constexpr int NW = BS / 64;
     __shared__ double red[NW * NS];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -127,7 +127,122 @@ __device__ __forceinline__ void block_store(const double (&v)[NS], double* out, 
             acc = k >= kFirstMax ? fmax(acc, red[i * NS + k]) : acc + red[i * NS + k];
     }
     lds_barrier();  // red may be reused by the next call
-    if (mine) out[k * stride] = acc;
+    total = acc;
+    return mine;
+}
+
+// block_reduce, column k stored to out[k * stride].
+template <int BS>
+__device__ __forceinline__ void block_store(const double (&v)[NS], double* out, int64_t stride,
+                                            uint32_t mask) {
+    double t;
+    if (block_reduce<BS>(v, mask, t)) out[threadIdx.x * stride] = t;
+}
+
+// ---------------------------------------------------------------- exact accumulators
+
+// v in units of 2^-60, rounded to nearest (ties to even), as three signed
+// 40-bit digits d0 + d1 2^40 + d2 2^80.  Returns false when v is not finite
+// or |v| >= 2^60.
+__device__ __forceinline__ bool acc_digits(double v, long long (&dg)[kAccDigits]) {
+    dg[0] = dg[1] = dg[2] = 0;
+    if (v == 0.0) return true;
+    const double a = fabs(v);
+    if (!(a < 0x1p60)) return false;  // also NaN
+    int e;
+    const double m = frexp(a, &e);                          // a = m 2^e, m in [0.5, 1)
+    unsigned long long M = (unsigned long long)(m * 0x1p53);  // exact: a = M 2^(e - 53)
+    int sh = e - 53 + 60;                                    // a = M 2^sh units
+    if (sh < 0) {
+        const int r = -sh;
+        if (r > 54) return true;  // below half a unit
+        const unsigned long long q = M >> r, rem = M - (q << r), half = 1ull << (r - 1);
+        M = q + ((rem > half || (rem == half && (q & 1ull))) ? 1ull : 0ull);
+        sh = 0;
+    }
+    // N = M << sh, sh <= 67: 128 bits as hi:lo
+    unsigned long long lo, hi;
+    if (sh == 0) {
+        lo = M;
+        hi = 0;
+    } else if (sh < 64) {
+        lo = M << sh;
+        hi = M >> (64 - sh);
+    } else {
+        lo = 0;
+        hi = M << (sh - 64);
+    }
+    constexpr unsigned long long mask = (1ull << 40) - 1ull;
+    const long long s = v < 0.0 ? -1 : 1;
+    dg[0] = s * (long long)(lo & mask);
+    dg[1] = s * (long long)(((lo >> 40) | (hi << 24)) & mask);
+    dg[2] = s * (long long)((hi >> 16) & mask);
+    return true;
+}
+
+// Column k's partial of logical block b into the accumulators.
+__device__ __forceinline__ void acc_add(const DevState& d, int b, int k, double v) {
+    const int seg = (int)(((int64_t)b * kAccSeg) / d.n_blocks);
+    unsigned long long* w = d.acc + ((int64_t)seg * NS + k) * kAccDigits;
+    if (k >= kFirstMax) {
+        // non-negative maxima order like their bit patterns; NaN is skipped as fmax skips it
+        if (!isnan(v)) __hip_atomic_fetch_max(w, __builtin_bit_cast(unsigned long long, v), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    long long dg[kAccDigits];
+    if (!acc_digits(v, dg)) {
+        __hip_atomic_fetch_or(d.acc + (int64_t)kAccSeg * NS * kAccDigits + k, 1ull, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < kAccDigits; ++q)
+        if (dg[q] != 0)
+            __hip_atomic_fetch_add(w + q, (unsigned long long)dg[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Column k's total from the accumulators (the mark makes it NaN).
+__device__ __forceinline__ double acc_total(const DevState& d, int k) {
+    const unsigned long long* base = d.acc;
+    if (k >= kFirstMax) {
+        unsigned long long mx = 0;
+#pragma unroll
+        for (int seg = 0; seg < kAccSeg; ++seg) {
+            const unsigned long long x = base[((int64_t)seg * NS + k) * kAccDigits];
+            mx = x > mx ? x : mx;
+        }
+        return __builtin_bit_cast(double, mx);
+    }
+    long long s[kAccDigits] = {0, 0, 0};
+#pragma unroll
+    for (int seg = 0; seg < kAccSeg; ++seg) {
+        const unsigned long long* w = base + ((int64_t)seg * NS + k) * kAccDigits;
+#pragma unroll
+        for (int q = 0; q < kAccDigits; ++q) s[q] += (long long)w[q];
+    }
+    if (base[(int64_t)kAccSeg * NS * kAccDigits + k] != 0ull) return __builtin_nan("");
+    return ((double)s[2] * 0x1p20 + (double)s[1] * 0x1p-20) + (double)s[0] * 0x1p-60;
+}
+
+// Column k's accumulator words reset for the next launch.
+__device__ __forceinline__ void acc_clear(const DevState& d, int k) {
+#pragma unroll
+    for (int seg = 0; seg < kAccSeg; ++seg)
+#pragma unroll
+        for (int q = 0; q < kAccDigits; ++q) d.acc[((int64_t)seg * NS + k) * kAccDigits + q] = 0ull;
+    d.acc[(int64_t)kAccSeg * NS * kAccDigits + k] = 0ull;
+}
+
+// The partials of logical block b: a row of `partial`, or the accumulators.
+template <int BS>
+__device__ __forceinline__ void partial_store(const DevState& d, const double (&v)[NS], int b, uint32_t mask) {
+    double t;
+    if (!block_reduce<BS>(v, mask, t)) return;
+    if (d.acc_mode)
+        acc_add(d, b, threadIdx.x, t);
+    else
+        d.partial[(int64_t)threadIdx.x * d.n_blocks + b] = t;
 }
 
 // In-place exclusive scan of a[0..n) (n <= kBrickCells); a[n] receives the total.

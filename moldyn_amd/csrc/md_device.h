@@ -128,6 +128,11 @@ struct RepParams {
     double psi_in;        // Nose-Hoover psi handed in by the caller
     int32_t use_psi_in;   // M_PRIME starts from psi_in instead of psi_used
     int32_t in_place;     // a step ends in the buffer it started from (surrogate path)
+    // step-record path: the buffer the chunk's last step writes the full
+    // records to, the one not holding the chunk's starting state (which a
+    // replay after an overflow restarts from); -1 on the other paths
+    int32_t full_dst;
+    int32_t pad_p;
     double cm[kMaxTypes];     // dt / (2 m_t), the half-kick factor (integrator.rs:29)
     double inv_m2[kMaxTypes]; // 1 / m_t^2 (drift-bound bookkeeping only)
 };
@@ -220,6 +225,11 @@ struct DevState {
     RepStats* stats;
     const PairParam* pairs;  // n_types x n_types
     double* partial;         // NS x n_blocks (column-major)
+    // one system of one type: the per-block partials go instead to exact
+    // fixed-point accumulators (kAccSeg segments x NS columns x 3 digits),
+    // which the finalize reads and clears
+    int32_t acc_mode;
+    unsigned long long* acc;
     double* sums;            // n_groups x NS
     unsigned long long* prof;  // per-launch kernel timestamps (null: off)
     unsigned long long* prof_blk;  // per-block {start, end, list entries} of the per-slot kernels
@@ -363,6 +373,19 @@ __host__ __device__ inline void brick_dims(const int* gd, int64_t n_atoms, int* 
     w |= g2 <= b2 + 2 ? 4 : 0;
     *whole = w;
 }
+
+// Exact accumulation of block partials (acc_mode): a sum column's value v is
+// taken in units of 2^-60 (rounded to nearest there; |v| < 2^60) as a signed
+// 120-bit integer split into three 40-bit digits, and each digit is added
+// with an integer atomic into the accumulator of the block's segment.
+// Integer sums do not depend on the order of the additions, so the result is
+// the same whatever order the blocks finish in and however the particles are
+// grouped into blocks.  Max columns take an atomic max of the bit pattern
+// (non-negative doubles order like their bits).  A non-finite or too large
+// value marks its column, whose sum is then NaN.
+constexpr int kAccSeg = 16;
+constexpr int kAccDigits = 3;
+constexpr int kAccWords = kAccSeg * NS * kAccDigits + NS;  // + one mark word per column
 
 // column masks for block partial stores
 constexpr uint32_t kColsKick = (1u << S_KVX) | (1u << S_KVY) | (1u << S_KVZ) | (1u << S_KV2);

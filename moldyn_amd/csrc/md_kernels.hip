@@ -614,8 +614,6 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     const int cur = C.cur, nx = cur ^ 1;
     const CRec* __restrict__ rin = d.crec[cur];
     CRec* __restrict__ rnx = d.crec[nx];
-    double* const part_out = d.partial + b;
-    const int64_t pstride = d.n_blocks;
     if (flags & F_NULL) {
         for (int64_t t = tr.t0; t < tr.t1; ++t) {
             const int64_t s = base + t * BS;
@@ -664,7 +662,7 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
             part[S_VMAX] = fmax(part[S_VMAX], (a.ux * a.ux + a.uy * a.uy) + a.uz * a.uz);
             grid_insert(d, C, R, s, a.x, a.y, a.z);
         }
-        block_store<BS>(part, part_out, pstride, kick_cols | kColsDrift);
+        partial_store<BS>(d, part, b, kick_cols | kColsDrift);
         prof_block(d, P_STEP, t_start, true, 0);
         return;
     }
@@ -672,7 +670,9 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     const bool split = (flags & F_SPLIT_B) != 0;
     const CRec* __restrict__ src = split ? rnx : rin;
     CRec* __restrict__ dst = split ? d.crec[cur] : rnx;
-    Rec<FT>* __restrict__ full = (Rec<FT>*)(split ? d.rec[cur] : d.rec[nx]);
+    // the full records of the chunk's last step go to the buffer not holding
+    // the chunk's starting state
+    Rec<FT>* __restrict__ full = (Rec<FT>*)d.rec[P.full_dst & 1];
     const PairParam p0 = wave_uniform(d.pairs[0]);
     int visits = 0;  // list entries read by this thread (profiling)
     // A tile's records move between HBM and LDS as whole 16-byte words with
@@ -824,7 +824,7 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
         if ((lane & 63) == 0) s_vis[lane >> 6] = visits;
     }
     // a split step stored its kick / drift columns in the first half
-    block_store<BS>(part, part_out, pstride, split ? kColsState : (kColsState | kick_cols | kColsDrift));
+    partial_store<BS>(d, part, b, split ? kColsState : (kColsState | kick_cols | kColsDrift));
     if (d.prof && lane == 0) {
         uint64_t tot = 0;
         for (int w = 0; w < BS / 64; ++w) tot += (uint64_t)s_vis[w];
@@ -925,7 +925,7 @@ __global__ __launch_bounds__(BS) void k_force_eval(DevState d) {
         part[S_WA] = fmax(part[S_WA], (((double)acc.fx * (double)acc.fx + (double)acc.fy * (double)acc.fy) +
                                        (double)acc.fz * (double)acc.fz) / (m * m));
     }
-    block_store<BS>(part, d.partial + b, d.n_blocks, kColsAll);
+    partial_store<BS>(d, part, b, kColsAll);
 }
 
 template <typename FT, int BS>
@@ -971,7 +971,7 @@ __global__ __launch_bounds__(BS) void k_reduce_state(DevState d) {
         part[S_WV] = fmax(part[S_WV], (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
         part[S_WA] = fmax(part[S_WA], ((fx * fx + fy * fy) + fz * fz) / (m * m));
     }
-    block_store<BS>(part, d.partial + b, d.n_blocks, kColsAll);
+    partial_store<BS>(d, part, b, kColsAll);
 }
 
 // ---------------------------------------------------------------- O(N^2) FP64 path
@@ -1018,8 +1018,7 @@ __global__ __launch_bounds__(BS) void k_drift(DevState d) {
         ((Rec<double>*)d.rec[nx])[s] = a;
         part[S_VMAX] = fmax(part[S_VMAX], (a.vx * a.vx + a.vy * a.vy) + a.vz * a.vz);
     }
-    block_store<BS>(part, d.partial + b, d.n_blocks,
-                    (P.th_kind == 2 ? kColsKick : 0u) | kColsDrift);
+    partial_store<BS>(d, part, b, (P.th_kind == 2 ? kColsKick : 0u) | kColsDrift);
     prof_block(d, P_DRIFT, t_start, true, 0);
 }
 
@@ -1116,7 +1115,7 @@ __global__ __launch_bounds__(BS) void k_allpairs(DevState d) {
             part[S_WA] = fmax(part[S_WA], ((fx * fx + fy * fy) + fz * fz) / (m * m));
         }
     }
-    block_store<BS>(part, d.partial + b, d.n_blocks, FORCE_ONLY ? kColsAll : kColsState);
+    partial_store<BS>(d, part, b, FORCE_ONLY ? kColsAll : kColsState);
     if (!FORCE_ONLY) prof_block(d, P_STEP, t_start, true, 0);
 }
 
@@ -1233,6 +1232,9 @@ __device__ __forceinline__ void advance(RepCtrl& C, const RepParams& P, const Re
             // k_step wrote buffer cur ^ 1 (a split step, and every surrogate
             // step, ends in cur)
             if (P.list_mode && !P.in_place && !(f & F_SPLIT_B)) C.cur ^= 1;
+            // the chunk's last step wrote the full records to buffer full_dst:
+            // the next chunk's step records are packed from there
+            if (P.full_dst >= 0 && C.step == P.target_step) C.cur = P.full_dst;
             C.flags = nf;
         }
     } else {
@@ -1364,28 +1366,12 @@ __device__ __forceinline__ int64_t finalize_replica(const DevState& d, int r, in
     __shared__ double s_sums[kMaxTypes * NS];
     __shared__ double s_vref[kMaxTypes * 3];
     const RepInfo R = d.reps[r];
-    // one system of one type: its partial rows are [0, n_blocks), so their
-    // loads go out together with the staging loads (one memory round trip)
-    const bool single = d.n_reps == 1 && T == 1;
-    double acc[NS];
-#pragma unroll
-    for (int k = 0; k < NS; ++k) acc[k] = 0.0;
-    if (single && mode != M_PRIME) {
-        const int64_t nb = d.n_blocks;
-        for (int b0 = threadIdx.x; b0 < nb; b0 += 2 * BS) {
-            double v[2][NS];
-#pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const int64_t bb = b0 + u * BS;
-#pragma unroll
-                for (int k = 0; k < NS; ++k) v[u][k] = bb < nb ? d.partial[k * nb + bb] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-#pragma unroll
-                for (int k = 0; k < NS; ++k) acc[k] = k >= kFirstMax ? fmax(acc[k], v[u][k]) : acc[k] + v[u][k];
-        }
-    }
+    // one system of one type: the column totals come from the exact
+    // accumulators; their loads go out together with the staging loads
+    // (one memory round trip)
+    const bool single = d.acc_mode != 0;
+    double acc_k = 0.0;
+    if (single && mode != M_PRIME && (int)threadIdx.x < NS) acc_k = acc_total(d, threadIdx.x);
     copy_words<BS>(&s_ctrl, &ctrl_in(d, r), (int)sizeof(RepCtrl));
     copy_words<BS>(&s_par, d.params + r, (int)sizeof(RepParams));
     copy_words<BS>(s_grp, d.groups + g0, (int)sizeof(GroupInfo) * T);
@@ -1409,10 +1395,14 @@ __device__ __forceinline__ int64_t finalize_replica(const DevState& d, int r, in
     // make every thread wait for its completion
     const bool reduced = finalize_reduces(mode, f0);
     if (reduced) {
-        if (single)
-            block_store<BS>(acc, s_sums, 1, kColsAll);
-        else
+        if (single) {
+            if ((int)threadIdx.x < NS) {
+                s_sums[threadIdx.x] = acc_k;
+                acc_clear(d, threadIdx.x);  // for the next launch's partials
+            }
+        } else {
             for (int t = 0; t < T; ++t) reduce_rows<BS>(d, s_grp[t], 0, s_grp[t].nblocks, s_sums + t * NS);
+        }
     } else {
         for (int i = threadIdx.x; i < T * NS; i += BS) s_sums[i] = d.sums[g0 * NS + i];
     }

@@ -172,7 +172,7 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
         oor |= mlp_store_features(d, mc, s, g, true);
     }
     if (oor) atomicOr(&d.mlp_stat[r].oor, 1u);
-    if (drift) block_store<BS>(part, d.partial + b, d.n_blocks, (P.th_kind == 2 ? kColsKick : 0u) | kColsDrift);
+    if (drift) partial_store<BS>(d, part, b, (P.th_kind == 2 ? kColsKick : 0u) | kColsDrift);
     prof_block(d, P_DRIFT, t_start, true, 0);
 }
 
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
         part[S_WA] = fmax(part[S_WA], (((double)fx * fx + (double)fy * fy) + (double)fz * fz) * P.inv_m2[G.type]);
         if (flags & F_INSERT) grid_insert(d, C, R, s, out.x, out.y, out.z);
     }
-    block_store<BS>(part, d.partial + b, d.n_blocks, FORCE_ONLY ? kColsAll : kColsState);
+    partial_store<BS>(d, part, b, FORCE_ONLY ? kColsAll : kColsState);
     if (check) {
         double jv[NS];
 #pragma unroll
