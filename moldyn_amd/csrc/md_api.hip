@@ -90,6 +90,13 @@ struct md_ctx {
     md_stats stats{};
     int64_t last_rebuilds = 0, last_refresh = 0, last_splits = 0;
     bool ctrl_valid = false;  // ctrl holds epochs of fills into the current grid buffers
+    // the step records still hold every replica's state as (x, v + F dt / 2m)
+    // for step length crec_dt: the next call needs no pack
+    bool crec_valid = false;
+    double crec_dt = 0.0;
+    // pinned staging of the control blocks and statistics read after a chunk
+    RepCtrl* h_ctrl = nullptr;
+    RepStats* h_stats = nullptr;
     std::string err;
 };
 
@@ -220,6 +227,10 @@ static void free_all(md_ctx* c) {
     c->prof_blk = nullptr;
     if (c->prof_host) (void)hipHostFree(c->prof_host);
     c->prof_host = nullptr;
+    if (c->h_ctrl) (void)hipHostFree(c->h_ctrl);
+    c->h_ctrl = nullptr;
+    if (c->h_stats) (void)hipHostFree(c->h_stats);
+    c->h_stats = nullptr;
     c->d.prof = nullptr;
     c->d.prof_blk = nullptr;
     c->prof = false;  // a new layout starts unprofiled
@@ -404,6 +415,7 @@ static int alloc_mlp(md_ctx* c) {
 // Allocate everything for the current layout; state buffers are NaN / zero.
 static int alloc_base(md_ctx* c) {
     free_all(c);
+    c->crec_valid = false;
     make_layout(c);
     c->fp32 = fp32_records(c->opt.force_mode);
     c->list_mode = c->opt.force_mode != MD_FORCE_FP64_ALLPAIRS;
@@ -732,6 +744,23 @@ static int read_stats(md_ctx* c, RepStats& s) {
     return MD_OK;
 }
 
+// Control blocks and statistics after a chunk, with one wait.
+static int read_ctrl_stats(md_ctx* c, std::vector<RepCtrl>& C, RepStats& s) {
+    if (!c->h_ctrl) {
+        void* p = nullptr;
+        HIPCHK(c, hipHostMalloc(&p, sizeof(RepCtrl) * c->R, hipHostMallocDefault));
+        c->h_ctrl = (RepCtrl*)p;
+        HIPCHK(c, hipHostMalloc(&p, sizeof(RepStats), hipHostMallocDefault));
+        c->h_stats = (RepStats*)p;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->h_ctrl, ctrl_cur(c), sizeof(RepCtrl) * c->R, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(c->h_stats, c->d.stats, sizeof(RepStats), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    C.assign(c->h_ctrl, c->h_ctrl + c->R);
+    s = *c->h_stats;
+    return MD_OK;
+}
+
 // Grow list / cell capacity after an overflow; returns 1 when grown.
 static int grow_capacity(md_ctx* c, const RepStats& s, int* grown) {
     *grown = 0;
@@ -880,8 +909,9 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
         }
         if ((rc = write_params(c, P))) return rc;
         if ((rc = reset_stats(c))) return rc;
-        if (c->d.compact) {
-            // the chunk steps on step records; its last step writes the full ones back
+        if (c->d.compact && !(c->crec_valid && c->crec_dt == dt)) {
+            // the chunk steps on step records; its last step writes the full
+            // ones back (and leaves the step records valid for the next call)
             c->d.par = c->par;
             HIPCHK(c, launch_pack(c->d, c->fp32, c->bs, c->st));
         }
@@ -893,8 +923,9 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
         }
         for (; k < cn; ++k) HIPCHK(c, launch_once(c));
         int64_t launched = cn;
+        RepStats s;
         for (int iter = 0;; ++iter) {
-            if ((rc = read_ctrl(c, C))) return rc;
+            if ((rc = read_ctrl_stats(c, C, s))) return rc;
             int64_t behind = 0;
             for (int r = 0; r < c->R; ++r) behind = std::max(behind, P[r].target_step - C[r].step);
             if (behind <= 0) break;
@@ -902,8 +933,7 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
             for (int64_t q = 0; q < behind; ++q) HIPCHK(c, launch_once(c));
             launched += behind;
         }
-        RepStats s;
-        if ((rc = read_stats(c, s))) return rc;
+        c->crec_valid = false;  // set again once the chunk stands
         if (s.list_overflow || s.cell_overflow) {
             int grown = 0;
             if ((rc = grow_capacity(c, s, &grown))) return rc;
@@ -933,6 +963,8 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
         }
         for (int r = 0; r < c->R; ++r)
             if (C[r].flags & F_BAD) bad = true;
+        c->crec_valid = c->d.compact != 0;
+        c->crec_dt = dt;
         st.launches += launched;
         done += cn;
         first = false;
@@ -1285,6 +1317,7 @@ static int reset_controls(md_ctx* c, const double* box, uint32_t epoch0) {
     if (rc) return rc;
     c->ctrl_valid = true;
     c->sums_valid = false;
+    c->crec_valid = false;
     c->stats.rebuilds = 0;
     c->stats.refresh_steps = 0;
     c->stats.split_steps = 0;
@@ -1336,6 +1369,7 @@ int md_upload_forces(md_ctx* c, const double* force, const double* pot, const do
     if (rc) return rc;
     if ((rc = put_pv(c, pot, vir))) return rc;
     c->sums_valid = false;
+    c->crec_valid = false;
     return MD_OK;
 }
 
@@ -1407,6 +1441,7 @@ int md_set_options(md_ctx* c, const md_options* o) {
     if (n.tiles_per_block < 0 || n.tiles_per_block > 64)
         return fail(c, MD_ERR_ARG, "tiles_per_block must be 0 (automatic) or 1..64");
     c->skin_set = n.skin > 0;
+    c->crec_valid = false;
     const bool relay = n.block_size != c->opt.block_size || n.force_mode != c->opt.force_mode ||
                        n.cells_per_atom != c->opt.cells_per_atom ||
                        n.tiles_per_block != c->opt.tiles_per_block;
@@ -1434,6 +1469,7 @@ int md_set_options(md_ctx* c, const md_options* o) {
 static int set_pot(md_ctx* c, uint16_t t0, uint16_t t1) {
     (void)t0;
     (void)t1;
+    c->crec_valid = false;
     rebuild_pairs(c);
     int rc = upload_pairs(c);
     if (rc) return rc;
@@ -1464,6 +1500,7 @@ int md_set_pair_potential_lj(md_ctx* c, uint16_t t0, uint16_t t1, double sigma, 
 
 int md_update_force(md_ctx* c) {
     if (!c) return MD_ERR_ARG;
+    c->crec_valid = false;
     if (c->mlp && !c->mlp_ready) return fail(c, MD_ERR_ARG, "surrogate parameters not set (md_mlp_set_params)");
     if (c->list_mode) return list_force_eval(c);
     std::vector<RepParams> P((size_t)c->R, base_params(c));
@@ -1480,6 +1517,7 @@ int md_update_force(md_ctx* c) {
 
 int md_apply_boundary_conditions(md_ctx* c) {
     if (!c) return MD_ERR_ARG;
+    c->crec_valid = false;
     c->d.par = c->par;
     if (c->n_blocks > 0) HIPCHK(c, launch_wrap(c->d, c->fp32, c->bs, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1565,6 +1603,7 @@ int md_thermostat_update(md_ctx* c, md_thermostat* th, double dt, uint16_t type_
     int rc = type_thermo(c, 0, type_id, &t, nullptr, nullptr);
     if (rc) return rc;
     if (th->kind == 1 || th->kind == 2) {
+        c->crec_valid = false;
         const GroupInfo& G = c->groups[c->reps[0].first_group + type_id];
         c->d.par = c->par;
         HIPCHK(c, launch_scale(c->d, c->fp32, G.start, G.count, th->lambda, 0, c->st));
@@ -1592,6 +1631,7 @@ int md_barostat_update(md_ctx* c, md_barostat* ba, uint16_t type_id) {
     if (!c || !ba || type_id >= c->T) return fail(c, MD_ERR_ARG, "bad argument");
     if (ba->kind == 2) return fail(c, MD_ERR_UNIMPLEMENTED, "Barostat::Custom is todo!() in the reference");
     if (ba->kind != 1) return MD_OK;
+    c->crec_valid = false;
     const GroupInfo& G = c->groups[c->reps[0].first_group + type_id];
     c->d.par = c->par;
     HIPCHK(c, launch_scale(c->d, c->fp32, G.start, G.count, ba->myu, 1, c->st));

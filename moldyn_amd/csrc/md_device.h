@@ -150,8 +150,8 @@ struct RepCtrl {
     uint32_t ins_epoch;        // epoch this launch inserts with
     uint32_t fill_epoch;       // epoch of the last completed fill
     int32_t cur;               // state buffer holding the input
-    int32_t build_src;         // state buffer a BUILD launch reads
-    int32_t pad1;
+    int32_t build_src;         // state buffer a BUILD launch reads (step records: their index)
+    int32_t cflip;             // step records of the input: buffer cur ^ cflip
     int64_t step;              // completed real steps
     int64_t rebuilds, nulls, launches, splits;
 };

@@ -611,7 +611,8 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     const TileRange tr = block_tiles(d, G, b, BS);
     const int lane = threadIdx.x;
     const int64_t base = G.start + lane;
-    const int cur = C.cur, nx = cur ^ 1;
+    // step-record buffers (the full records' buffers are cur and full_dst)
+    const int cur = (C.cur ^ C.cflip) & 1, nx = cur ^ 1;
     const CRec* __restrict__ rin = d.crec[cur];
     CRec* __restrict__ rnx = d.crec[nx];
     if (flags & F_NULL) {
@@ -840,10 +841,10 @@ __global__ __launch_bounds__(BS) void k_pack(DevState d) {
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
     const double c = d.params[G.rep].cm[G.type];
-    const int buf = ctrl_in(d, G.rep).cur;
+    const RepCtrl& C = ctrl_in(d, G.rep);
     const TileRange tr = block_tiles(d, G, b, BS);
-    const Rec<FT>* __restrict__ in = (const Rec<FT>*)d.rec[buf];
-    CRec* __restrict__ out = d.crec[buf];
+    const Rec<FT>* __restrict__ in = (const Rec<FT>*)d.rec[C.cur];
+    CRec* __restrict__ out = d.crec[(C.cur ^ C.cflip) & 1];
     for (int64_t t = tr.t0; t < tr.t1; ++t) {
         const int64_t s = G.start + t * BS + threadIdx.x;
         const Rec<FT> a = in[s];
@@ -1175,7 +1176,7 @@ __device__ __forceinline__ void advance(RepCtrl& C, const RepParams& P, const Re
         } else if (f & F_SPLIT_A) {
             // drifted state and its grid are ready: build lists there, then the second half
             C.fill_epoch = C.ins_epoch;
-            C.build_src = C.cur ^ 1;
+            C.build_src = (C.cur ^ C.cflip) ^ 1;
             C.flags = F_BUILD | F_SPLIT_PENDING;
             C.splits++;
             controls = false;
@@ -1234,7 +1235,10 @@ __device__ __forceinline__ void advance(RepCtrl& C, const RepParams& P, const Re
             if (P.list_mode && !P.in_place && !(f & F_SPLIT_B)) C.cur ^= 1;
             // the chunk's last step wrote the full records to buffer full_dst:
             // the next chunk's step records are packed from there
-            if (P.full_dst >= 0 && C.step == P.target_step) C.cur = P.full_dst;
+            if (P.full_dst >= 0 && C.step == P.target_step) {
+                C.cflip = (C.cur ^ C.cflip) ^ P.full_dst;  // the step records stay where they are
+                C.cur = P.full_dst;
+            }
             C.flags = nf;
         }
     } else {
@@ -1287,7 +1291,7 @@ __device__ __forceinline__ void advance(RepCtrl& C, const RepParams& P, const Re
                 set_fill(C, P, R, C.box);
             } else if (C.flags & F_GRID_FRESH) {
                 C.flags |= F_BUILD;
-                C.build_src = C.cur;
+                C.build_src = C.cur ^ C.cflip;
             } else {
                 C.flags |= F_NULL;
                 set_fill(C, P, R, C.box);
