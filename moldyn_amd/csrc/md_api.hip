@@ -97,6 +97,10 @@ struct md_ctx {
     // pinned staging of the control blocks and statistics read after a chunk
     RepCtrl* h_ctrl = nullptr;
     RepStats* h_stats = nullptr;
+    // host copy of the control blocks the next launch reads, valid while no
+    // finalize, graph replay, snapshot restore or layout change has run since
+    std::vector<RepCtrl> ctrl_cache;
+    bool ctrl_cache_valid = false;
     std::string err;
 };
 
@@ -416,6 +420,7 @@ static int alloc_mlp(md_ctx* c) {
 static int alloc_base(md_ctx* c) {
     free_all(c);
     c->crec_valid = false;
+    c->ctrl_cache_valid = false;
     make_layout(c);
     c->fp32 = fp32_records(c->opt.force_mode);
     c->list_mode = c->opt.force_mode != MD_FORCE_FP64_ALLPAIRS;
@@ -510,20 +515,29 @@ static int alloc_base(md_ctx* c) {
 static RepCtrl* ctrl_cur(md_ctx* c) { return c->d.ctrl + (size_t)c->par * c->R; }
 
 static int read_ctrl(md_ctx* c, std::vector<RepCtrl>& out) {
+    if (c->ctrl_cache_valid && (int)c->ctrl_cache.size() == c->R) {
+        out = c->ctrl_cache;
+        return MD_OK;
+    }
     out.resize(c->R);
     HIPCHK(c, hipMemcpyAsync(out.data(), ctrl_cur(c), sizeof(RepCtrl) * c->R, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    c->ctrl_cache = out;
+    c->ctrl_cache_valid = true;
     return MD_OK;
 }
 
 static int write_ctrl(md_ctx* c, const std::vector<RepCtrl>& in) {
     HIPCHK(c, hipMemcpyAsync(ctrl_cur(c), in.data(), sizeof(RepCtrl) * c->R, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    c->ctrl_cache = in;
+    c->ctrl_cache_valid = true;
     return MD_OK;
 }
 
 // Finalize launch; M_STEP and M_PRIME write the other control buffer.
 static hipError_t finalize(md_ctx* c, int mode) {
+    if (mode != M_REDUCE) c->ctrl_cache_valid = false;
     c->d.par = c->par;
     hipError_t e = launch_finalize(c->d, mode, c->list_mode, c->fp32, c->st);
     if (e == hipSuccess && mode != M_REDUCE) c->par ^= 1;
@@ -728,6 +742,7 @@ static int snapshot(md_ctx* c, bool save) {
         HIPCHK(c, hipMemcpyAsync(c->snap_ctrl, ctrl_cur(c), sizeof(RepCtrl) * c->R, hipMemcpyDeviceToDevice, c->st));
     } else {
         c->par = c->snap_par;
+        c->ctrl_cache_valid = false;
         HIPCHK(c, hipMemcpyAsync(ctrl_cur(c), c->snap_ctrl, sizeof(RepCtrl) * c->R, hipMemcpyDeviceToDevice, c->st));
     }
     return MD_OK;
@@ -758,6 +773,8 @@ static int read_ctrl_stats(md_ctx* c, std::vector<RepCtrl>& C, RepStats& s) {
     HIPCHK(c, hipStreamSynchronize(c->st));
     C.assign(c->h_ctrl, c->h_ctrl + c->R);
     s = *c->h_stats;
+    c->ctrl_cache = C;
+    c->ctrl_cache_valid = true;
     return MD_OK;
 }
 
@@ -919,6 +936,7 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
         int64_t k = 0;
         if (K > 1 && cn >= K) {
             if ((rc = ensure_graph(c, K))) return rc;
+            c->ctrl_cache_valid = false;
             for (; k + K <= cn; k += K) HIPCHK(c, hipGraphLaunch(c->gexec[c->par], c->st));
         }
         for (; k < cn; ++k) HIPCHK(c, launch_once(c));
