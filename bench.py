@@ -210,6 +210,61 @@ def frame_io(M, pos, vel, box, frames=6, steps_between=20):
             f"steps_{steps_between}_plain_s": plain, f"steps_{steps_between}_with_save_s": with_io}
 
 
+def liquid_262k_state(M):
+    """64^3 SC argon at a = 0.47 nm (262,144 atoms) equilibrated 500 steps at
+    150 K by the FP32 LJ path (Berendsen T, tau 0.1 ps): (n, a, state, mean
+    pressure over the last 100 steps)."""
+    n_side, a = 64, 0.47
+    n = n_side ** 3
+    box = np.full(3, n_side * a)
+    rng = np.random.default_rng(6)
+    pos = np.mod(M.lattice("u", [n_side] * 3, a) + rng.uniform(-0.02, 0.02, (n, 3)) * a, box)
+    vel = M.mb_velocities(n, 150.0, MASS, 6)
+    probe = M.Engine([n], [MASS], box, force_mode="fp32_list")
+    probe.upload(pos, vel, box)
+    probe.update_force()
+    th_eq = M.thermostat("berendsen", tau=0.1, target_t=150.0)
+    probe.step(DT, 400, th_eq)
+    p_hist = []
+    for _ in range(5):
+        probe.step(DT, 20, th_eq)
+        p_hist.append(probe.macro()["pressure"])
+    st = probe.download(("x", "v", "box"))
+    probe.close()
+    return n, a, st, float(np.mean(p_hist))
+
+
+def dense_liquid(M, steps):
+    """The FP32 LJ path alone on the equilibrated 262,144-atom liquid under
+    Berendsen T and P (the surrogate workload's LJ leg), with per-launch
+    device timing: the dense-list force case for hardware counters."""
+    n, a, st, p_state = liquid_262k_state(M)
+    e = M.Engine([n], [MASS], st["box"], force_mode="fp32_list")
+    e.upload(st["x"], st["v"], st["box"])
+    e.update_force()
+    th = M.thermostat("berendsen", tau=1.0, target_t=150.0)
+    ba = M.barostat("berendsen", beta=1.0, tau=50.0, target_p=p_state)
+    e.step(DT, 20, th, ba)
+    e.set_profiling(True)
+    ts = time.perf_counter()
+    e.step(DT, steps, th, ba)
+    mac = e.macro()
+    wall = time.perf_counter() - ts
+    step_ms, step_n = e.kernel_time("step")
+    fin_ms, fin_n = e.kernel_time("finalize")
+    span_ms, span_n = e.kernel_time("step", span=True)
+    stt = e.stats()
+    e.close()
+    return {"workload": f"64^3 SC argon ({n} atoms) at a = {a} nm, FP32 LJ list, Berendsen T and P, dt {DT}, "
+                        f"{steps} steps",
+            "atom_steps_per_s": n * steps / wall, "ms_per_step": 1e3 * wall / steps,
+            "k_step_avg_us": 1e3 * step_ms / max(step_n, 1), "k_fin_avg_us": 1e3 * fin_ms / max(fin_n, 1),
+            "k_step_span_avg_us": 1e3 * span_ms / max(span_n, 1),
+            "list_entries_per_launch": stt["list_visits"] / max(span_n, 1),
+            "mean_list_len": stt["list_visits"] / max(span_n, 1) / n, "rebuilds": stt["rebuilds"],
+            "temperature_final": mac["temperature"]}
+
+
 def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log):
     """Surrogate: 4,096-atom LJ argon frames from thermostatted SC runs at
     lattice edges 0.38-0.7 nm and 80-400 K, labelled by the FP64 LJ path,
@@ -236,28 +291,7 @@ def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log):
         f = e.download(("force",))["force"]
         e.close()
         rel.append(float(np.sqrt(np.mean((f - fr.f_ref[k]) ** 2)) / np.sqrt(np.mean(fr.f_ref[k] ** 2))))
-    # inference run: 64^3 SC at a = 0.47, equilibrated 500 steps at 150 K by
-    # the FP32 LJ path (Berendsen T, tau 0.1 ps); both paths then continue from
-    # that state under Berendsen T (tau 1 ps) and P (beta 1, tau 50 ps, P0 =
-    # the equilibrated state's pressure)
-    n_side, a = 64, 0.47
-    n = n_side ** 3
-    box = np.full(3, n_side * a)
-    rng = np.random.default_rng(6)
-    pos = np.mod(M.lattice("u", [n_side] * 3, a) + rng.uniform(-0.02, 0.02, (n, 3)) * a, box)
-    vel = M.mb_velocities(n, 150.0, MASS, 6)
-    probe = M.Engine([n], [MASS], box, force_mode="fp32_list")
-    probe.upload(pos, vel, box)
-    probe.update_force()
-    th_eq = M.thermostat("berendsen", tau=0.1, target_t=150.0)
-    probe.step(DT, 400, th_eq)
-    p_hist = []
-    for _ in range(5):
-        probe.step(DT, 20, th_eq)
-        p_hist.append(probe.macro()["pressure"])
-    p_state = float(np.mean(p_hist))
-    st = probe.download(("x", "v", "box"))
-    probe.close()
+    n, a, st, p_state = liquid_262k_state(M)
     res = {}
     for mode in ("mlp", "fp32_list"):
         e = M.Engine([n], [MASS], st["box"], force_mode=mode)
@@ -503,6 +537,11 @@ def main():
                     help="skip the 64k NVE, 256-replica and surrogate measurements")
     ap.add_argument("--mlp-frames", type=int, default=2000)
     ap.add_argument("--mlp-epochs", type=int, default=40)
+    ap.add_argument("--only", choices=("mlp", "liquid"), default=None,
+                    help="run one secondary workload alone and print its JSON (for profiling): 'mlp', the "
+                         "learned surrogate's training and 262,144-atom liquid run beside the FP32 LJ path; "
+                         "'liquid', that FP32 LJ liquid run alone")
+    ap.add_argument("--mlp-steps", type=int, default=2000)
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "k_step_1M_npt_traffic.txt"),
                     help="rocprof_summary traffic file holding bytes_per_launch of the step kernel")
     args = ap.parse_args()
@@ -514,6 +553,14 @@ def main():
     import moldyn_amd as M
     from rocprof_summary import read_traffic
     traffic = read_traffic(args.traffic)
+    if args.only == "mlp":
+        res = mlp_surrogate(M, n_frames=args.mlp_frames, epochs=args.mlp_epochs, steps=args.mlp_steps)
+        print(json.dumps({"workload": "mlp_surrogate", **res}, default=float), flush=True)
+        return
+    if args.only == "liquid":
+        res = dense_liquid(M, args.mlp_steps)
+        print(json.dumps({"workload": "dense_liquid", **res}, default=float), flush=True)
+        return
 
     n, pos, vel, box = initial_state(M, args.n_side)
     log(f"[bench] state ready: {n} atoms, box {box[0]:.4f}")

@@ -57,5 +57,40 @@ def build(verbose=False):
     return LIB
 
 
+def build_host_checks(out_dir):
+    """Host-only build of every source under AddressSanitizer and
+    UndefinedBehaviorSanitizer, linked with csrc/host_checks.cpp into
+    out_dir/host_checks (no device code is compiled, so the binary runs
+    without a GPU).  Returns the binary's path."""
+    os.makedirs(out_dir, exist_ok=True)
+    san = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+           "-Xarch_host", "-fno-sanitize-recover=all"]
+    # the device code is built unoptimised: only the host code runs
+    base = [a for a in FLAGS if a != "-O3"] + ["-O1", "-g", "-fno-omit-frame-pointer", "-Xarch_device", "-O0"]
+
+    def one(src):
+        out = os.path.join(out_dir, os.path.splitext(src)[0] + ".o")
+        r = subprocess.run([HIPCC] + base + san + ["-c", os.path.join(CSRC, src), "-o", out],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host-only build of {src} failed:\n{r.stdout}{r.stderr}")
+        return out
+
+    with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
+        objs = list(ex.map(one, SOURCES))
+    drv = os.path.join(out_dir, "host_checks.o")
+    r = subprocess.run([HIPCC, "-O1", "-g", "-fno-omit-frame-pointer", "-std=c++17", "-fPIC"] + san +
+                       ["-I" + os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, "host_checks.cpp"),
+                        "-o", drv], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host_checks.cpp failed:\n" + r.stdout + r.stderr)
+    exe = os.path.join(out_dir, "host_checks")
+    r = subprocess.run([HIPCC, "--offload-arch=gfx950"] + san + ["-o", exe, drv] + objs + ["-lpthread"],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host_checks link failed:\n" + r.stdout + r.stderr)
+    return exe
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

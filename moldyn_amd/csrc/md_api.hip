@@ -1100,9 +1100,11 @@ int md_create_replicas(md_ctx** out, int n_replicas, int n_types, const int64_t*
     int64_t total = 0;
     for (int64_t i = 0; i < (int64_t)n_replicas * n_types; ++i) {
         if (counts[i] < 0) return fail(nullptr, MD_ERR_ARG, "negative particle count");
+        // each term is checked before the sum so that it cannot overflow
+        if (counts[i] >= ((int64_t)1 << 30) - total)
+            return fail(nullptr, MD_ERR_ARG, "too many particles for 32-bit slot ids");
         total += counts[i];
     }
-    if (total >= (int64_t)1 << 30) return fail(nullptr, MD_ERR_ARG, "too many particles for 32-bit slot ids");
     int dev = 0;
     int rc = check_device(&dev);
     if (rc) return rc;
@@ -1873,6 +1875,12 @@ int md_mlp_set_params(md_ctx* c, const float* theta, const float* g_mean, const 
     }
     k.inv_2w2 = (float)(1.0 / (2.0 * w * w));
     k.inv_w2 = (float)(1.0 / (w * w));
+    {
+        const double a = 1.0 / (2.0 * w * w), dm = sigma * 1.7 / (kMlpK - 1);
+        k.bas_2ad = (float)(2.0 * a * dm);
+        k.bas_ad2 = (float)(a * dm * dm);
+        k.bas_c = (float)exp(-2.0 * a * dm * dm);
+    }
     k.rc = (float)(2.5 * sigma);
     k.e_scale = (float)e_scale;
     k.e_shift = (float)e_shift;

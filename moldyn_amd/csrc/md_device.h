@@ -184,8 +184,12 @@ struct MlpConst {
     float g_mean[kMlpK], g_inv_std[kMlpK];  // input normalisation
     float g_lo[kMlpK], g_hi[kMlpK];         // feature range seen in training
     float mu[kMlpK];                        // basis centres (nm)
-    float inv_2w2;                          // 1 / (2 w^2)
+    float inv_2w2;                          // a = 1 / (2 w^2)
     float inv_w2;                           // 1 / w^2
+    // the centres are evenly spaced (mu_{k+1} = mu_k + D), so along k each
+    // basis value is the previous one times q_k = exp(2 a D t_k - a D^2),
+    // t_k = r - mu_k, and q_{k+1} = q_k exp(-2 a D^2)
+    float bas_2ad, bas_ad2, bas_c;          // 2 a D, a D^2, exp(-2 a D^2)
     float rc;                               // basis cutoff (nm)
     float e_scale, e_shift;
     float b3;
@@ -333,10 +337,14 @@ __host__ __device__ inline void grid_dims(const double* gb, double rlist, int64_
     for (int k = 0; k < 3; ++k) gd[k] = g[k];
 }
 
-// Brick shape of a grid: the largest cubic edge E <= kBrickEdge cells whose
-// halo ((E + 2)^3 cells, or the whole axis where that is shorter) holds about
-// kBrickTarget particles and whose centre (E^3 cells) about kBrickOwn at the
-// grid's mean occupancy.
+// Brick shape of a grid.  Candidates for e = 1..kBrickEdge are three shape
+// families: (e, e, e), (e, e, e+1) and (e, e+1, e+1) cells.  An axis whose
+// grid is no longer than the candidate edge + 2 is taken whole (no halo on
+// it).  A candidate qualifies when its halo (edge + 2 cells per tiled axis)
+// has at most kBrickCells cells and, at the grid's mean occupancy, at most
+// kBrickTarget particles, and its centre at most kBrickOwn particles.  Of the
+// qualifying shapes the one with the most centre cells wins, ties going to
+// the smaller halo; with none, bricks are single cells.
 __host__ __device__ inline void brick_dims(const int* gd, int64_t n_atoms, int* edge, int* whole) {
     const double per_cell = (double)n_atoms / ((double)gd[0] * gd[1] * gd[2]);
     const int g0 = gd[0], g1 = gd[1], g2 = gd[2];

@@ -31,6 +31,29 @@ __device__ __forceinline__ float silu_f(float a, float& ds) {
     return a * s;
 }
 
+// The 32 Gaussians exp(-(r - mu_k)^2 / (2 w^2)) of one distance, handed
+// to f(k, value) in order of k.  The centres are evenly spaced, so within
+// each block of 16 the values follow from the block's first one by the
+// recurrence g_{k+1} = g_k q_k, q_{k+1} = q_k exp(-2 a D^2) (MlpConst): four
+// exponentials instead of 32, and no array of values.  A block's first value
+// is at least exp(-a (15 D)^2) ~ 1e-15 whenever the distance lies among its
+// centres, so nothing that matters underflows.
+template <typename F>
+__device__ __forceinline__ void gauss_basis(float r, const MlpConst& mc, F&& f) {
+#pragma unroll
+    for (int k0 = 0; k0 < kMlpK; k0 += 16) {
+        const float t = r - mc.mu[k0];
+        float g = __expf(-t * t * mc.inv_2w2);
+        float q = __expf(fmaf(mc.bas_2ad, t, -mc.bas_ad2));
+#pragma unroll
+        for (int k = k0; k < k0 + 16; ++k) {
+            f(k, g);
+            g *= q;
+            q *= mc.bas_c;
+        }
+    }
+}
+
 // Features of one particle from its neighbour list.  Positions come from
 // `pos` records; with DRIFT the neighbours' drifted positions are recomputed
 // from their input records with the operations their own threads perform.
@@ -42,9 +65,18 @@ __device__ __forceinline__ void mlp_features(const DevState& d, const MlpConst& 
     for (int k = 0; k < kMlpK; ++k) g[k] = 0.0f;
     const int nn = d.ncount[s];
     const double rc2 = (double)mc.rc * (double)mc.rc;
+    const int64_t ns = d.n_slots;
+    // software pipeline over the list: the index two entries ahead, the
+    // neighbour's record one entry ahead
+    int32_t jn = nn > 1 ? d.nbr[ns + s] : 0;
+    Rec<float> on;
+    if (nn > 0) on = pos[d.nbr[s]];
     for (int q = 0; q < nn; ++q) {
-        const int32_t j = d.nbr[(int64_t)q * d.n_slots + s];
-        const Rec<float> o = pos[j];
+        const Rec<float> o = on;
+        if (q + 1 < nn) {
+            on = pos[jn];
+            jn = q + 2 < nn ? d.nbr[(int64_t)(q + 2) * ns + s] : 0;
+        }
         double xj = o.x, yj = o.y, zj = o.z;
         if (DRIFT) {
             xj = wrap1(xj + ((o.vx + (double)o.fx * c) * lam) * dt, bx);
@@ -58,11 +90,7 @@ __device__ __forceinline__ void mlp_features(const DevState& d, const MlpConst& 
         if (!(r2 < rc2)) continue;
         const float r = (float)sqrt(r2);
         const float fc = 0.5f * (__cosf(kPi * r / mc.rc) + 1.0f);
-#pragma unroll
-        for (int k = 0; k < kMlpK; ++k) {
-            const float t = r - mc.mu[k];
-            g[k] = fmaf(__expf(-t * t * mc.inv_2w2), fc, g[k]);
-        }
+        gauss_basis(r, mc, [&](int k, float e) { g[k] = fmaf(e, fc, g[k]); });
     }
 }
 
@@ -357,9 +385,32 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
         float fx = 0.f, fy = 0.f, fz = 0.f, vir = 0.f;
         Acc<float> acc;  // LJ (fallback and check)
         const int nn = d.ncount[s];
+        const int64_t ns = d.n_slots;
+        // software pipeline over the list: the index two entries ahead, the
+        // record and dE/dG one entry ahead (both gathers go out together,
+        // not after the distance test)
+        int32_t jn = nn > 1 ? d.nbr[ns + s] : 0;  // index of the entry after the next
+        Rec<float> on;
+        f32x4 gn[kMlpK / 4];
+        if (nn > 0) {
+            const int32_t j0 = d.nbr[s];
+            on = src[j0];
+            const f32x4* gp = (const f32x4*)(d.mlp_gam + (int64_t)j0 * kMlpK);
+#pragma unroll
+            for (int q4 = 0; q4 < kMlpK / 4; ++q4) gn[q4] = gp[q4];
+        }
         for (int q = 0; q < nn; ++q) {
-            const int32_t j = d.nbr[(int64_t)q * d.n_slots + s];
-            const Rec<float> o = src[j];
+            const Rec<float> o = on;
+            f32x4 gj[kMlpK / 4];
+#pragma unroll
+            for (int q4 = 0; q4 < kMlpK / 4; ++q4) gj[q4] = gn[q4];
+            if (q + 1 < nn) {
+                on = src[jn];
+                const f32x4* gp = (const f32x4*)(d.mlp_gam + (int64_t)jn * kMlpK);
+#pragma unroll
+                for (int q4 = 0; q4 < kMlpK / 4; ++q4) gn[q4] = gp[q4];
+                jn = q + 2 < nn ? d.nbr[(int64_t)(q + 2) * ns + s] : 0;
+            }
             const double dx = min_image(o.x - a.x, bx);
             const double dy = min_image(o.y - a.y, by);
             const double dz = min_image(o.z - a.z, bz);
@@ -370,20 +421,12 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
             const float arg = kPi * rr / mc.rc;
             const float fc = 0.5f * (__cosf(arg) + 1.0f);
             const float dfc = -0.5f * kPi / mc.rc * __sinf(arg);
-            const f32x4* gp = (const f32x4*)(d.mlp_gam + (int64_t)j * kMlpK);
             float cc = 0.f;
-#pragma unroll
-            for (int q4 = 0; q4 < kMlpK / 4; ++q4) {
-                const f32x4 gj = gp[q4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int k = 4 * q4 + u;
-                    const float tt = rr - mc.mu[k];
-                    const float e = __expf(-tt * tt * mc.inv_2w2);
-                    const float dg = e * fmaf(-tt * mc.inv_w2, fc, dfc);
-                    cc = fmaf(gi[k] + gj[u], dg, cc);
-                }
-            }
+            gauss_basis(rr, mc, [&](int k, float e) {
+                const float tt = rr - mc.mu[k];
+                const float dg = e * fmaf(-tt * mc.inv_w2, fc, dfc);
+                cc = fmaf(gi[k] + gj[k / 4][k % 4], dg, cc);
+            });
             const float ir = 1.0f / rr;
             fx = fmaf(cc * ir, (float)dx, fx);
             fy = fmaf(cc * ir, (float)dy, fy);

@@ -21,6 +21,10 @@
 // 125-bit power of 5 (md_text_tables.h), then digits removed while the
 // interval still holds a shorter candidate.  The functions are usable on the
 // device and the host; the frame kernels (md_frames.hip) call them on the GPU.
+// The digit-removal loop and the log10/log2 approximations follow the
+// structure of Ryu's reference implementation in C (d2s.c of
+// github.com/ulfjack/ryu, by Ulf Adams, licensed Apache-2.0 or BSL-1.0),
+// written here for this code's own types and tables.
 #pragma once
 
 #include <hip/hip_runtime.h>

@@ -1,6 +1,6 @@
 """Text summaries of rocprofv3 output for the measurement records.
 
-Two commands:
+Three commands:
 
   python rocprof_summary.py stats DIR --command "..." --out FILE
       per-kernel duration statistics of a ``rocprofv3 --kernel-trace`` run
@@ -13,6 +13,13 @@ Two commands:
       passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by
       what the same counter reads for a plain copy kernel that moves B bytes in
       and B bytes out.  Writes ``bytes_per_launch: <int>`` among its lines.
+      ``--fetch-above K`` keeps only the dispatches whose FETCH_SIZE exceeds
+      K KiB (the same dispatch index in both passes).
+
+  python rocprof_summary.py counters DIR --kernel REGEX --command "..." \
+      [--trace DIR] --out FILE
+      median per dispatch of every counter of a ``--pmc`` run, with the
+      durations of a kernel-trace run of the same program beside them.
 
 Only the standard library is used; the module reads what rocprofv3 wrote and
 never runs anything.
@@ -100,9 +107,27 @@ def _per_dispatch(rows, regex):
     return vals
 
 
+def _dispatches(rows, regex):
+    """Values of the kernels matching regex in dispatch order (unsorted)."""
+    pat = re.compile(regex)
+    return [v for name, vs in rows.items() if pat.search(name) for v in vs]
+
+
 def traffic(args):
     fetch = _per_dispatch(_pmc_rows(args.fetch, "FETCH_SIZE"), args.kernel)
     write = _per_dispatch(_pmc_rows(args.write, "WRITE_SIZE"), args.kernel)
+    if args.fetch_above is not None:
+        # the dispatches whose FETCH_SIZE exceeds the threshold (the two
+        # passes run the same program, so dispatch k is the same launch)
+        f_all = _dispatches(_pmc_rows(args.fetch, "FETCH_SIZE"), args.kernel)
+        w_all = _dispatches(_pmc_rows(args.write, "WRITE_SIZE"), args.kernel)
+        if len(f_all) != len(w_all):
+            raise SystemExit("the FETCH_SIZE and WRITE_SIZE passes saw different dispatch counts")
+        keep = [k for k, v in enumerate(f_all) if v > args.fetch_above]
+        if not keep:
+            raise SystemExit("no dispatch above the FETCH_SIZE threshold")
+        fetch = sorted(f_all[k] for k in keep)
+        write = sorted(w_all[k] for k in keep)
     cf = _per_dispatch(_pmc_rows(args.copy_fetch, "FETCH_SIZE"), args.copy_kernel)
     cw = _per_dispatch(_pmc_rows(args.copy_write, "WRITE_SIZE"), args.copy_kernel)
     kib = 1024.0
@@ -114,7 +139,8 @@ def traffic(args):
     wr = wm * kib * w_scale
     lines = [f"# rocprofv3 --pmc HBM traffic: {args.title}", "",
              "commands:", ""] + [f"    {c}" for c in args.command] + ["",
-             f"kernel regex: {args.kernel}",
+             f"kernel regex: {args.kernel}" + (f", dispatches with FETCH_SIZE above {args.fetch_above} KiB"
+                                               if args.fetch_above is not None else ""),
              f"dispatches: FETCH_SIZE {len(fetch)}, WRITE_SIZE {len(write)}",
              f"FETCH_SIZE KiB per dispatch: median {fm:.1f} (p10 {_q(fetch, .1):.1f}, p90 {_q(fetch, .9):.1f})",
              f"WRITE_SIZE KiB per dispatch: median {wm:.1f} (p10 {_q(write, .1):.1f}, p90 {_q(write, .9):.1f})",
@@ -126,6 +152,39 @@ def traffic(args):
              f"read_bytes_per_launch: {rd:.0f}",
              f"write_bytes_per_launch: {wr:.0f}",
              f"bytes_per_launch: {rd + wr:.0f}"]
+    _emit("\n".join(lines) + "\n", args.out)
+
+
+def counters(args):
+    """Per-dispatch medians of every counter a --pmc run collected for the
+    kernels matching the regex, with the dispatch durations of a kernel-trace
+    run of the same program when --trace names one."""
+    pat = re.compile(args.kernel)
+    per = {}
+    files = sorted(glob.glob(os.path.join(args.dir, "**", "*counter_collection.csv"), recursive=True))
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {args.dir}")
+    for f in files:
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if not pat.search(r["Kernel_Name"]):
+                    continue
+                per.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    if not per:
+        raise SystemExit(f"no dispatch of a kernel matching {args.kernel!r}")
+    lines = [f"# rocprofv3 --pmc counters: {args.title}", "", "command:", "",
+             f"    {args.command}", "", f"kernel regex: {args.kernel}", "",
+             f"{'counter':34s} {'dispatches':>10s} {'median':>16s} {'p10':>16s} {'p90':>16s}"]
+    for name in sorted(per):
+        vs = sorted(per[name])
+        lines.append(f"{name:34s} {len(vs):10d} {statistics.median(vs):16.1f} "
+                     f"{_q(vs, .1):16.1f} {_q(vs, .9):16.1f}")
+    if args.trace:
+        ds = sorted((e - s) / 1000.0 for name, s, e in _kernel_rows(args.trace) if pat.search(name))
+        if ds:
+            lines += ["", f"kernel-trace durations ({args.trace_command or 'same program'}):",
+                      f"  dispatches {len(ds)}, median {statistics.median(ds):.2f} us, "
+                      f"mean {statistics.mean(ds):.2f} us"]
     _emit("\n".join(lines) + "\n", args.out)
 
 
@@ -167,9 +226,19 @@ def main(argv=None):
     t.add_argument("--copy-bytes", type=float, required=True)
     t.add_argument("--command", action="append", required=True)
     t.add_argument("--title", default="")
+    t.add_argument("--fetch-above", type=float, default=None,
+                   help="only dispatches whose FETCH_SIZE exceeds this many KiB")
     t.add_argument("--out")
+    c = sub.add_parser("counters", help="per-dispatch medians of SQ counters and the dispatch durations")
+    c.add_argument("dir")
+    c.add_argument("--kernel", required=True)
+    c.add_argument("--command", required=True)
+    c.add_argument("--title", default="")
+    c.add_argument("--trace", help="kernel-trace directory of the same program, for durations")
+    c.add_argument("--trace-command")
+    c.add_argument("--out")
     a = ap.parse_args(argv)
-    (stats if a.cmd == "stats" else traffic)(a)
+    {"stats": stats, "traffic": traffic, "counters": counters}[a.cmd](a)
 
 
 if __name__ == "__main__":
