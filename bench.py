@@ -265,7 +265,7 @@ def dense_liquid(M, steps):
             "temperature_final": mac["temperature"]}
 
 
-def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log):
+def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log, prof_steps=200):
     """Surrogate: 4,096-atom LJ argon frames from thermostatted SC runs at
     lattice edges 0.38-0.7 nm and 80-400 K, labelled by the FP64 LJ path,
     trained on the GPU; inference on a 64^3 (262,144-atom) SC grid at
@@ -311,6 +311,33 @@ def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log):
                      "box_final": float(e.download(("box",))["box"][0])}
         if mode == "mlp":
             res[mode].update(e.mlp_stats())
+        # per-kernel device time over a further profiled stretch
+        e.set_profiling(True)
+        e.step(DT, prof_steps, th, ba)
+        kern = {}
+        for name, key in (("k_mlp_desc" if mode == "mlp" else None, "drift"),
+                          ("k_mlp_net" if mode == "mlp" else None, "mlp_net"),
+                          ("k_mlp_force" if mode == "mlp" else "k_step", "step"), ("k_fin", "finalize")):
+            if name is None:
+                continue
+            ms, cnt = e.kernel_time(key)
+            sms, scnt = e.kernel_time(key, span=True)
+            kern[name] = {"stream_us": 1e3 * ms / max(cnt, 1), "span_us": 1e3 * sms / max(scnt, 1),
+                          "launches": cnt}
+        e.set_profiling(False)
+        res[mode]["kernels"] = kern
+        if mode == "mlp":
+            # 80 MFMA 16x16x32 (2*16*16*32 flops each) per 16 atoms: the two
+            # forward layers (8 + 32), the backward through W2 (32) and W1 (8)
+            flops = n * 80 * 2 * 16 * 16 * 32 / 16
+            t_net = kern["k_mlp_net"]["span_us"] * 1e-6
+            res[mode]["k_mlp_net_mfma"] = {
+                "flops_per_launch": flops, "span_us": kern["k_mlp_net"]["span_us"],
+                "achieved_tflops": flops / t_net / 1e12 if t_net > 0 else None,
+                "peak_tflops_bf16_dense": 2500.0,
+                "frac": flops / t_net / 2.5e15 if t_net > 0 else None,
+                "basis": "device timestamps (s_memrealtime) of every block of every launch, first start to "
+                         "last end"}
         e.close()
     return {"workload": f"64^3 SC argon ({n} atoms) at a = {a} nm equilibrated 500 steps at 150 K, then Berendsen T "
                         f"(150 K, tau 1) and P (beta 1, tau 50, P0 = equilibrated pressure {p_state:.4g}), "

@@ -282,8 +282,9 @@ typedef struct {
 int md_get_stats(md_ctx* ctx, md_stats* out);
 /* Per-kernel device time of the launches of md_step: the step kernel
  * (kernel 0; the pair-force kernel on the surrogate path), the finalize /
- * list-build kernel (1), the all-pairs force kernel (2) and the drift kernel
- * (3; the feature kernel on the surrogate path); kernel 4 reports nothing.
+ * list-build kernel (1), the all-pairs force kernel (2), the drift kernel
+ * (3; the feature kernel on the surrogate path) and the surrogate's network
+ * kernel (5); kernel 4 reports nothing.
  * Every block records its start and end on the GPU wall clock
  * (s_memrealtime), so the captured graphs keep running unchanged.
  *   total_ms / launches: stream time, from the end of the previous kernel of

@@ -29,7 +29,7 @@ static thread_local std::string g_error;
 
 namespace {
 
-enum { K_STEP = 0, K_FINALIZE = 1, K_ALLPAIRS = 2, K_DRIFT = 3, K_BUILD = 4, K_NUM = 5 };
+enum { K_STEP = 0, K_FINALIZE = 1, K_ALLPAIRS = 2, K_DRIFT = 3, K_BUILD = 4, K_MLP_NET = 5, K_NUM = 6 };
 
 }  // namespace
 
@@ -82,10 +82,10 @@ struct md_ctx {
     unsigned long long* prof_blk = nullptr;
     unsigned long long* prof_host = nullptr;  // pinned staging of the ring
     double wall_hz = 1e8;
-    double ktime[P_NUM] = {0, 0, 0};      // stream time (previous kernel end to end)
-    int64_t kcount[P_NUM] = {0, 0, 0};
-    double kspan_time[P_NUM] = {0, 0, 0};  // first block start to last block end
-    int64_t kspan_count[P_NUM] = {0, 0, 0};
+    double ktime[P_NUM] = {};      // stream time (previous kernel end to end)
+    int64_t kcount[P_NUM] = {};
+    double kspan_time[P_NUM] = {};  // first block start to last block end
+    int64_t kspan_count[P_NUM] = {};
     int64_t list_visits = 0;
     md_stats stats{};
     int64_t last_rebuilds = 0, last_refresh = 0, last_splits = 0;
@@ -627,7 +627,7 @@ static int prof_collect(md_ctx* c) {
         c->kspan_count[k] += 1;
     }
     // stream time: from the end of the previous recorded kernel on the stream
-    // (in launch order drift, step, finalize) to this kernel's end, counted
+    // (in launch order drift, net, step, finalize) to this kernel's end, counted
     // where the two ran back to back (the interval a dispatch trace reports)
     const unsigned long long gap = (unsigned long long)(20e-6 * c->wall_hz);
     auto rec = [&](int l, int k, unsigned long long* a, unsigned long long* b) {
@@ -641,7 +641,7 @@ static int prof_collect(md_ctx* c) {
         const int lp = (l + kProfRing - 1) % kProfRing;
         bool have_prev = rec(lp, P_FIN, &pa, &pb);
         unsigned long long prev_end = pb;
-        for (int k : {(int)P_DRIFT, (int)P_STEP, (int)P_FIN}) {
+        for (int k : {(int)P_DRIFT, (int)P_NET, (int)P_STEP, (int)P_FIN}) {
             if (!rec(l, k, &a, &b)) continue;
             if (have_prev && a >= prev_end && a - prev_end < gap) {
                 c->ktime[k] += (double)(b - prev_end) / c->wall_hz * 1e3;
@@ -1944,7 +1944,8 @@ int md_kernel_time(md_ctx* c, int kernel, double* total_ms, int64_t* launches, d
     if (kernel == K_STEP && c->list_mode) slot = P_STEP;
     if (kernel == K_ALLPAIRS && !c->list_mode) slot = P_STEP;
     if (kernel == K_FINALIZE) slot = P_FIN;
-    if (kernel == K_DRIFT && !c->list_mode) slot = P_DRIFT;
+    if (kernel == K_DRIFT && (!c->list_mode || c->mlp)) slot = P_DRIFT;
+    if (kernel == K_MLP_NET && c->mlp) slot = P_NET;
     if (total_ms) *total_ms = slot >= 0 ? c->ktime[slot] : 0.0;
     if (launches) *launches = slot >= 0 ? c->kcount[slot] : 0;
     if (span_ms) *span_ms = slot >= 0 ? c->kspan_time[slot] : 0.0;

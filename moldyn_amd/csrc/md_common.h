@@ -517,6 +517,19 @@ __device__ __forceinline__ void prof_mark(const DevState& d, int64_t launch, int
     }
 }
 
+// Start and end of a whole block of k_mlp_net, merged with atomics into the
+// first two words of its per-block area (every thread of the block calls it:
+// it holds a barrier, and d.prof is the same for all).
+__device__ __forceinline__ void prof_span_atomic(const DevState& d, int slot, uint64_t t0) {
+    if (!d.prof) return;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long* e = d.prof_blk + (int64_t)slot * d.n_blocks * 3;
+        atomicMin(e, (unsigned long long)t0);
+        atomicMax(e + 1, (unsigned long long)wall_clock64());
+    }
+}
+
 // Per-block record of a per-slot kernel (thread 0, after the block's last
 // barrier); blocks that did no work store an empty interval.
 __device__ __forceinline__ void prof_block(const DevState& d, int slot, uint64_t t0, bool worked,

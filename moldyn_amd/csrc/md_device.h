@@ -274,14 +274,17 @@ __host__ __device__ inline TileRange block_tiles(const DevState& d, const GroupI
 }
 
 // Per-launch timestamp ring (wall clock ticks): for launch l, entry
-// (l % kProfRing) holds {min start, max end} of kernels P_STEP, P_FIN, P_DRIFT;
-// one counter of list entries read follows the ring.  The per-slot kernels
-// store per-block {start, end, entries} with plain stores (slot P_STEP for
-// k_step / k_allpairs, P_DRIFT for k_drift) and one block of the finalize
-// kernel folds them into the ring; the finalize kernel's own blocks (one per
+// (l % kProfRing) holds {min start, max end} of kernels P_STEP, P_FIN,
+// P_DRIFT, P_NET; one counter of list entries read follows the ring.  The
+// per-slot kernels store per-block {start, end, entries} with plain stores
+// (slot P_STEP for k_step / k_allpairs / k_mlp_force, P_DRIFT for k_drift /
+// k_mlp_desc) and one block of the finalize kernel folds them into the ring;
+// k_mlp_net (grid not one block per slot range) keeps {min start, max end}
+// with atomics in the first two words of its per-block area, which the fold
+// moves to the ring and resets; the finalize kernel's own blocks (one per
 // replica plus the blocks that build lists) record with atomics.
 constexpr int kProfRing = 8192;
-enum { P_STEP = 0, P_FIN = 1, P_DRIFT = 2, P_NUM = 3 };
+enum { P_STEP = 0, P_FIN = 1, P_DRIFT = 2, P_NET = 3, P_NUM = 4 };
 constexpr int64_t kProfCount = (int64_t)kProfRing * P_NUM * 2;  // index of the entry counter
 
 enum FinalizeMode { M_STEP = 0, M_REDUCE = 1, M_PRIME = 2 };

@@ -1506,6 +1506,16 @@ __device__ __forceinline__ void prof_fold(const DevState& d, int64_t launch, boo
         }
         __syncthreads();
     }
+    if (d.mlp_w && threadIdx.x == 0) {
+        unsigned long long* e = d.prof_blk + (int64_t)P_NET * d.n_blocks * 3;
+        if (e[0] != ~0ull) {
+            unsigned long long* r = d.prof + ((launch % kProfRing) * P_NUM + P_NET) * 2;
+            r[0] = e[0];
+            r[1] = e[1];
+        }
+        e[0] = ~0ull;
+        e[1] = 0ull;
+    }
 }
 
 template <typename FT, bool MULTI, bool BUILD, bool REPS, bool DENSE>
@@ -1515,7 +1525,7 @@ __global__ __launch_bounds__(kFinBS) void k_fin(DevState d, int mode) {
     // the last block folds the per-block records of k_step (or k_drift +
     // k_allpairs) while the first blocks finalize
     if (d.prof && mode == M_STEP && b == (int)gridDim.x - 1)
-        prof_fold(d, ctrl_in(d, 0).launches, !BUILD);
+        prof_fold(d, ctrl_in(d, 0).launches, !BUILD || d.mlp_w != nullptr);
     int64_t launch = -1;
     if (b < d.n_reps) launch = finalize_replica(d, b, mode);
     if (BUILD && !REPS && b < d.n_bld) {
@@ -1627,6 +1637,9 @@ __global__ __launch_bounds__(256) void k_prof_init(DevState d) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < kProfCount) d.prof[i] = (i & 1) ? 0ull : ~0ull;
     if (i == kProfCount) d.prof[i] = 0ull;
+    // the atomic {min start, max end} pair of k_mlp_net
+    if (i == kProfCount + 1 && d.prof_blk) d.prof_blk[(int64_t)P_NET * d.n_blocks * 3] = ~0ull;
+    if (i == kProfCount + 2 && d.prof_blk) d.prof_blk[(int64_t)P_NET * d.n_blocks * 3 + 1] = 0ull;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1697,7 +1710,7 @@ hipError_t launch_build(const DevState& d, bool fp32, hipStream_t st) {
 }
 
 hipError_t launch_prof_init(const DevState& d, hipStream_t st) {
-    const int n = (int)kProfCount + 1;
+    const int n = (int)kProfCount + 3;
     hipLaunchKernelGGL(k_prof_init, dim3((n + 255) / 256), dim3(256), 0, st, d);
     return hipGetLastError();
 }

@@ -228,6 +228,7 @@ __global__ __launch_bounds__(kNetBS) void k_mlp_net(DevState d) {
     __shared__ __attribute__((aligned(16))) __bf16 sW2t[kMlpH * kStrH];  // W2^T [n][k]: B of t = d2 W2
     __shared__ __attribute__((aligned(16))) __bf16 sH[kNetWaves][16 * kStrH];
     __shared__ float sb1[kMlpH], sb2[kMlpH], sw3[kMlpH];
+    const uint64_t t_start = prof_now(d);
     const MlpConst& mc = *d.mlp_c;
     const __bf16* w = d.mlp_w;
     for (int i = threadIdx.x; i < kMlpH * kMlpK; i += kNetBS) {
@@ -323,6 +324,7 @@ __global__ __launch_bounds__(kNetBS) void k_mlp_net(DevState d) {
             if (lr == 0) d.mlp_e[r0 + lq * 4 + i] = mc.e_scale * (v + mc.b3) + mc.e_shift;
         }
     }
+    prof_span_atomic(d, P_NET, t_start);
 }
 
 // ---------------------------------------------------------------- forces

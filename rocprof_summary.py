@@ -107,6 +107,24 @@ def _per_dispatch(rows, regex):
     return vals
 
 
+def _durations(path, counter, regex, keep=None):
+    """Dispatch durations (us) of the matching kernels from a --pmc csv that
+    carries Start/End timestamps, in dispatch order (None without them)."""
+    pat = re.compile(regex)
+    out = []
+    for f in sorted(glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True)):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] != counter or not pat.search(r["Kernel_Name"]):
+                    continue
+                if "Start_Timestamp" not in r or "End_Timestamp" not in r:
+                    return None
+                out.append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0)
+    if keep is not None:
+        out = [out[k] for k in keep]
+    return out
+
+
 def _dispatches(rows, regex):
     """Values of the kernels matching regex in dispatch order (unsorted)."""
     pat = re.compile(regex)
@@ -116,6 +134,7 @@ def _dispatches(rows, regex):
 def traffic(args):
     fetch = _per_dispatch(_pmc_rows(args.fetch, "FETCH_SIZE"), args.kernel)
     write = _per_dispatch(_pmc_rows(args.write, "WRITE_SIZE"), args.kernel)
+    keep = None
     if args.fetch_above is not None:
         # the dispatches whose FETCH_SIZE exceeds the threshold (the two
         # passes run the same program, so dispatch k is the same launch)
@@ -152,6 +171,12 @@ def traffic(args):
              f"read_bytes_per_launch: {rd:.0f}",
              f"write_bytes_per_launch: {wr:.0f}",
              f"bytes_per_launch: {rd + wr:.0f}"]
+    durs = _durations(args.fetch, "FETCH_SIZE", args.kernel, keep)
+    if durs:
+        dm = statistics.median(durs)
+        lines += ["", "dispatch duration in the FETCH_SIZE pass (counter collection serialises dispatches):",
+                  f"  median {dm:.2f} us over {len(durs)} dispatches -> {(rd + wr) / dm / 1e3:.1f} GB/s of HBM "
+                  "traffic"]
     _emit("\n".join(lines) + "\n", args.out)
 
 

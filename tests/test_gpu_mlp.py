@@ -109,3 +109,28 @@ def test_mlp_periodic_lj_check():
     # an untrained network is far from LJ, and the check says so
     assert 0.1 < st["last_rmse_rel"] < 1e3 and st["max_rmse_rel"] >= st["last_rmse_rel"]
     assert st["last_rms_force"] > 1.0
+
+
+def test_mlp_kernel_timing_leaves_results_unchanged():
+    """Device timestamps of the three surrogate kernels (features, network,
+    forces) cover the launches of a graph-replayed run and change nothing."""
+    s = liquid_state(4, temperature=100.0)
+    p = oracle_params(s, seed=7)
+    outs = []
+    for prof in (False, True):
+        e = mlp_engine(s, p, judge_every=0)
+        e.update_force()
+        e.set_profiling(prof)
+        e.step(0.002, 600)
+        outs.append(e.download(("x", "v")))
+        if prof:
+            launches = e.stats()["launches"]
+            for key in ("drift", "mlp_net", "step"):
+                ms, n = e.kernel_time(key, span=True)
+                assert 0.95 * launches <= n <= launches, key
+                assert 0.0 < ms / n < 5.0, key
+            # the network kernel starts after the feature kernel ends
+            ms_net, n_net = e.kernel_time("mlp_net")
+            assert 0.8 * launches <= n_net <= launches
+        e.close()
+    assert np.array_equal(outs[0]["x"], outs[1]["x"]) and np.array_equal(outs[0]["v"], outs[1]["v"])
