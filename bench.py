@@ -305,8 +305,11 @@ def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log, prof_steps=2
         ts = time.perf_counter()
         e.step(DT, steps, th, ba)
         wall = time.perf_counter() - ts
+        st_run = e.stats()
         mac = e.macro()
         res[mode] = {"atom_steps_per_s": n * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                     "rebuilds": st_run["rebuilds"], "launches": st_run["launches"],
+                     "mean_list_len": st_run["mean_list_len"],
                      "temperature_final": mac["temperature"], "pressure_final": mac["pressure"],
                      "box_final": float(e.download(("box",))["box"][0])}
         if mode == "mlp":

@@ -94,9 +94,13 @@ struct md_ctx {
     // for step length crec_dt: the next call needs no pack
     bool crec_valid = false;
     double crec_dt = 0.0;
-    // pinned staging of the control blocks and statistics read after a chunk
+    // pinned staging of every control block, statistics and parameter copy
+    // of the stepping loop (no pageable transfer there)
     RepCtrl* h_ctrl = nullptr;
     RepStats* h_stats = nullptr;
+    RepParams* h_params = nullptr;
+    hipEvent_t params_ev = nullptr;  // the last parameter upload (h_params is reused after it)
+    bool params_pending = false;
     // host copy of the control blocks the next launch reads, valid while no
     // finalize, graph replay, snapshot restore or layout change has run since
     std::vector<RepCtrl> ctrl_cache;
@@ -235,6 +239,14 @@ static void free_all(md_ctx* c) {
     c->h_ctrl = nullptr;
     if (c->h_stats) (void)hipHostFree(c->h_stats);
     c->h_stats = nullptr;
+    if (c->params_ev) {
+        if (c->params_pending) (void)hipEventSynchronize(c->params_ev);
+        (void)hipEventDestroy(c->params_ev);
+    }
+    c->params_ev = nullptr;
+    c->params_pending = false;
+    if (c->h_params) (void)hipHostFree(c->h_params);
+    c->h_params = nullptr;
     c->d.prof = nullptr;
     c->d.prof_blk = nullptr;
     c->prof = false;  // a new layout starts unprofiled
@@ -514,21 +526,42 @@ static int alloc_base(md_ctx* c) {
 // Control blocks of the buffer the next launch reads.
 static RepCtrl* ctrl_cur(md_ctx* c) { return c->d.ctrl + (size_t)c->par * c->R; }
 
+// Pinned host staging of the stepping loop's transfers (allocated once per
+// layout).
+static int pinned_staging(md_ctx* c) {
+    if (c->h_ctrl) return MD_OK;
+    void* p = nullptr;
+    HIPCHK(c, hipHostMalloc(&p, sizeof(RepCtrl) * c->R, hipHostMallocDefault));
+    c->h_ctrl = (RepCtrl*)p;
+    HIPCHK(c, hipHostMalloc(&p, sizeof(RepStats), hipHostMallocDefault));
+    c->h_stats = (RepStats*)p;
+    HIPCHK(c, hipHostMalloc(&p, sizeof(RepParams) * c->R, hipHostMallocDefault));
+    c->h_params = (RepParams*)p;
+    HIPCHK(c, hipEventCreateWithFlags(&c->params_ev, hipEventDisableTiming));
+    return MD_OK;
+}
+
 static int read_ctrl(md_ctx* c, std::vector<RepCtrl>& out) {
     if (c->ctrl_cache_valid && (int)c->ctrl_cache.size() == c->R) {
         out = c->ctrl_cache;
         return MD_OK;
     }
-    out.resize(c->R);
-    HIPCHK(c, hipMemcpyAsync(out.data(), ctrl_cur(c), sizeof(RepCtrl) * c->R, hipMemcpyDeviceToHost, c->st));
+    int rc;
+    if ((rc = pinned_staging(c))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->h_ctrl, ctrl_cur(c), sizeof(RepCtrl) * c->R, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    out.assign(c->h_ctrl, c->h_ctrl + c->R);
     c->ctrl_cache = out;
     c->ctrl_cache_valid = true;
     return MD_OK;
 }
 
 static int write_ctrl(md_ctx* c, const std::vector<RepCtrl>& in) {
-    HIPCHK(c, hipMemcpyAsync(ctrl_cur(c), in.data(), sizeof(RepCtrl) * c->R, hipMemcpyHostToDevice, c->st));
+    int rc;
+    if ((rc = pinned_staging(c))) return rc;
+    // every use of h_ctrl ends in a stream synchronisation, so it is free here
+    memcpy(c->h_ctrl, in.data(), sizeof(RepCtrl) * c->R);
+    HIPCHK(c, hipMemcpyAsync(ctrl_cur(c), c->h_ctrl, sizeof(RepCtrl) * c->R, hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->ctrl_cache = in;
     c->ctrl_cache_valid = true;
@@ -567,8 +600,15 @@ static RepParams base_params(md_ctx* c) {
 }
 
 static int write_params(md_ctx* c, const std::vector<RepParams>& P) {
-    HIPCHK(c, hipMemcpyAsync((void*)c->d.params, P.data(), sizeof(RepParams) * c->R,
+    int rc;
+    if ((rc = pinned_staging(c))) return rc;
+    // the previous upload may still read the staging buffer
+    if (c->params_pending) HIPCHK(c, hipEventSynchronize(c->params_ev));
+    memcpy(c->h_params, P.data(), sizeof(RepParams) * c->R);
+    HIPCHK(c, hipMemcpyAsync((void*)c->d.params, c->h_params, sizeof(RepParams) * c->R,
                              hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipEventRecord(c->params_ev, c->st));
+    c->params_pending = true;
     return MD_OK;
 }
 
@@ -761,13 +801,8 @@ static int read_stats(md_ctx* c, RepStats& s) {
 
 // Control blocks and statistics after a chunk, with one wait.
 static int read_ctrl_stats(md_ctx* c, std::vector<RepCtrl>& C, RepStats& s) {
-    if (!c->h_ctrl) {
-        void* p = nullptr;
-        HIPCHK(c, hipHostMalloc(&p, sizeof(RepCtrl) * c->R, hipHostMallocDefault));
-        c->h_ctrl = (RepCtrl*)p;
-        HIPCHK(c, hipHostMalloc(&p, sizeof(RepStats), hipHostMallocDefault));
-        c->h_stats = (RepStats*)p;
-    }
+    int rc;
+    if ((rc = pinned_staging(c))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->h_ctrl, ctrl_cur(c), sizeof(RepCtrl) * c->R, hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipMemcpyAsync(c->h_stats, c->d.stats, sizeof(RepStats), hipMemcpyDeviceToHost, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));

@@ -262,14 +262,12 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
     const double rlm = P.rlist + 2.0 * 1.7320508075688772 * eps;
     const float rl2f = __double2float_ru(rlm * rlm * (1.0 + 0x1p-20));
     const int64_t N = d.n_slots;
-    int nbk[3], hd[3];
+    int nbk[3];
     for (int k = 0; k < 3; ++k) {
         const bool w = (whole >> k) & 1;
         nbk[k] = w ? 1 : (gd[k] + be[k] - 1) / be[k];
-        hd[k] = w ? gd[k] : be[k] + 2;
     }
     const int nbricks = nbk[0] * nbk[1] * nbk[2];
-    const int H = hd[0] * hd[1] * hd[2];
     BrickLds& L = brick_lds();
     int* const s_off = L.off;
     int* const s_cen = L.cen;
@@ -284,13 +282,45 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
     for (int64_t b64 = bi0; b64 < nbricks; b64 += stride) {  // 64-bit: a stride may be INT_MAX
         const int bi = (int)b64;
         const int bc[3] = {bi / (nbk[1] * nbk[2]), (bi / nbk[2]) % nbk[1], bi % nbk[2]};
-        int org[3], ncen[3];
+        // The regions built from LDS: the brick; when its halo holds more
+        // particles than LDS stages (a density fluctuation above the grid's
+        // mean), its centre split into sub-bricks along the tiled axes, and a
+        // sub-brick that still overflows cell by cell.  A single cell whose
+        // halo overflows builds its particles' lists straight from the grid,
+        // one particle per thread.  All region bookkeeping is uniform over
+        // the block (it follows the LDS totals).
+        int borg[3], bcen[3];  // the brick: halo origin (grid cells) and centre extent
         for (int k = 0; k < 3; ++k) {
             const bool w = (whole >> k) & 1;
-            org[k] = w ? 0 : bc[k] * be[k] - 1;
+            borg[k] = w ? 0 : bc[k] * be[k] - 1;
             const int rest = gd[k] - bc[k] * be[k];
-            ncen[k] = w ? gd[k] : (rest < be[k] ? rest : be[k]);
+            bcen[k] = w ? gd[k] : (rest < be[k] ? rest : be[k]);
         }
+        int level = 0;                             // 0 brick, 1 sub-brick, 2 cell
+        int sm[3] = {1, 1, 1}, sp[3] = {1, 1, 1};  // sub-brick centre extent, sub-bricks per axis
+        int j1 = 0, n1 = 1, j2 = 0, n2 = 1;        // sub-brick / cell index and count
+        int sorg[3] = {0, 0, 0}, scen[3] = {1, 1, 1};  // the current sub-brick
+        for (;;) {
+        int org[3], ncen[3], hd[3];
+        for (int k = 0; k < 3; ++k) {
+            const bool w = (whole >> k) & 1;
+            if (w || level == 0) {
+                org[k] = borg[k];
+                ncen[k] = bcen[k];
+            } else if (level == 1) {
+                const int q = k == 0 ? j1 / (sp[1] * sp[2]) : (k == 1 ? (j1 / sp[2]) % sp[1] : j1 % sp[2]);
+                org[k] = borg[k] + q * sm[k];
+                const int rest = bcen[k] - q * sm[k];
+                ncen[k] = rest < sm[k] ? rest : sm[k];
+            } else {
+                const int e1 = ((whole >> 1) & 1) ? 1 : scen[1], e2 = ((whole >> 2) & 1) ? 1 : scen[2];
+                const int q = k == 0 ? j2 / (e1 * e2) : (k == 1 ? (j2 / e2) % e1 : j2 % e2);
+                org[k] = sorg[k] + q - 1;
+                ncen[k] = 1;
+            }
+            hd[k] = w ? gd[k] : ncen[k] + 2;
+        }
+        const int H = hd[0] * hd[1] * hd[2];
         // halo cell h -> grid cell (halo coordinates wrap periodically)
         auto cell_of = [&](int h) -> int64_t {
             const int h0 = h / (hd[1] * hd[2]), h1 = (h / hd[2]) % hd[1], h2 = h % hd[2];
@@ -325,21 +355,65 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
         const int total = s_off[H];
         const int n_own = s_cen[H];
         if (total > kBrickAtoms) {
-            // too dense for LDS: lists of the brick's particles straight from the grid
-            if (threadIdx.x == 0) atomicOr(&d.stats->brick_fallback, 1u);
-            const int ncell = ncen[0] * ncen[1] * ncen[2];
-            for (int q = threadIdx.x; q < ncell; q += BS) {
-                const int q0 = q / (ncen[1] * ncen[2]), q1 = (q / ncen[2]) % ncen[1], q2 = q % ncen[2];
-                const int h = (((whole & 1) ? q0 : q0 + 1) * hd[1] + ((whole & 2) ? q1 : q1 + 1)) * hd[2] +
-                              ((whole & 4) ? q2 : q2 + 1);
-                const int64_t cell = cell_of(h);
-                const int cnt = s_off[h + 1] - s_off[h];
-                for (int k = 0; k < cnt; ++k)
-                    build_list_global<RT, MULTI>(d, C, P, R, recs, cs[cell * cc + k]);
+            // the centre cells along tiled axes that can still be split
+            const bool split = level < 2 && (level == 0 || ((~whole & 1) && ncen[0] > 1) ||
+                                             ((~whole & 2) && ncen[1] > 1) || ((~whole & 4) && ncen[2] > 1));
+            if (split && level == 0) {
+                // sub-brick centres: halve the longest tiled extent until the
+                // halo's expected count at this brick's density fits
+                // (fixed indices only: the extents stay in registers)
+                const double per = (double)total / (double)H;
+                const bool w0 = whole & 1, w1 = (whole >> 1) & 1, w2 = (whole >> 2) & 1;
+                int m0 = w0 ? gd[0] : bcen[0], m1 = w1 ? gd[1] : bcen[1], m2 = w2 ? gd[2] : bcen[2];
+                for (int it = 0; it < 32; ++it) {
+                    const double hv = (double)(w0 ? gd[0] : m0 + 2) * (double)(w1 ? gd[1] : m1 + 2) *
+                                      (double)(w2 ? gd[2] : m2 + 2);
+                    const int a0 = w0 ? 1 : m0, a1 = w1 ? 1 : m1, a2 = w2 ? 1 : m2;
+                    if ((a0 <= 1 && a1 <= 1 && a2 <= 1) || hv * per <= 0.75 * (double)kBrickAtoms) break;
+                    if (a0 >= a1 && a0 >= a2) m0 = (m0 + 1) / 2;
+                    else if (a1 >= a2) m1 = (m1 + 1) / 2;
+                    else m2 = (m2 + 1) / 2;
+                }
+                sm[0] = m0;
+                sm[1] = m1;
+                sm[2] = m2;
+                n1 = 1;
+                for (int k = 0; k < 3; ++k) {
+                    sp[k] = ((whole >> k) & 1) ? 1 : (bcen[k] + sm[k] - 1) / sm[k];
+                    n1 *= sp[k];
+                }
+                level = 1;
+                j1 = 0;
+                lds_barrier();
+                continue;
             }
-            lds_barrier();
-            continue;
-        }
+            if (split && level == 1) {
+                for (int k = 0; k < 3; ++k) {
+                    sorg[k] = org[k] + 1;  // the centre's first cell (tiled axes)
+                    scen[k] = ncen[k];
+                }
+                n2 = 1;
+                for (int k = 0; k < 3; ++k)
+                    if (!((whole >> k) & 1)) n2 *= scen[k];
+                level = 2;
+                j2 = 0;
+                lds_barrier();
+                continue;
+            }
+            // a region that cannot be split further: its centre particles'
+            // lists straight from the grid, one particle per thread (the
+            // centre prefix sums map a particle to its halo cell)
+            if (threadIdx.x == 0) atomicOr(&d.stats->brick_fallback, 1u);
+            for (int k = threadIdx.x; k < n_own; k += BS) {
+                int lo = 0, hi = H;  // the h with s_cen[h] <= k < s_cen[h + 1]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_cen[mid] <= k) lo = mid;
+                    else hi = mid;
+                }
+                build_list_global<RT, MULTI>(d, C, P, R, recs, cs[cell_of(lo) * cc + (k - s_cen[lo])]);
+            }
+        } else {
         // staged particle -> its halo cell, centre particle -> staging index
         // (LDS only), then one particle per thread: slot id and position
         // loads of all particles go out together
@@ -462,7 +536,17 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                 list_close(d, i, lst.n);
             }
         }
+        }  // the region fitted in LDS
         lds_barrier();
+        // the next region: the sub-brick's next cell, the brick's next
+        // sub-brick, else the next brick
+        if (level == 2) {
+            if (++j2 < n2) continue;
+            level = 1;
+        }
+        if (level == 1 && ++j1 < n1) continue;
+        break;
+        }  // regions
     }
 }
 

@@ -92,3 +92,63 @@ def brick_dims(gd, n_atoms, edge_max=14, cells_max=4096, target=900, own_max=440
     edge = [g if g <= b + 2 else b for g, b in zip(gd, best)]
     whole = [g <= b + 2 for g, b in zip(gd, best)]
     return edge, whole
+
+
+def cell_counts(pos, box, gd):
+    """Particles per cell of a grid fill (md_common.h cell_coord, restated)."""
+    q = []
+    for k in range(3):
+        u = pos[:, k] / box[k]
+        u = u - np.floor(u)
+        q.append(np.clip((u * gd[k]).astype(np.int64), 0, gd[k] - 1))
+    counts = np.zeros(gd, dtype=np.int64)
+    np.add.at(counts, (q[0], q[1], q[2]), 1)
+    return counts
+
+
+def halo_total(counts, org, hd):
+    """Particles staged for a region: its halo of hd cells from grid cell org
+    (periodic)."""
+    gd = counts.shape
+    idx = [np.arange(org[k], org[k] + hd[k]) % gd[k] for k in range(3)]
+    return int(counts[np.ix_(idx[0], idx[1], idx[2])].sum())
+
+
+def brick_regions(counts, edge, whole, capacity=1024):
+    """The regions md_kernels.hip brick_build stages, restated for tiled axes
+    (no whole axis): {level: number of regions whose halo overflowed}, level
+    0 bricks, 1 sub-bricks, 2 cells."""
+    gd = counts.shape
+    assert not any(whole)
+    over = {0: 0, 1: 0, 2: 0}
+    nb = [(gd[k] + edge[k] - 1) // edge[k] for k in range(3)]
+    for b0 in range(nb[0]):
+        for b1 in range(nb[1]):
+            for b2 in range(nb[2]):
+                bc = (b0, b1, b2)
+                borg = [bc[k] * edge[k] - 1 for k in range(3)]
+                bcen = [min(gd[k] - bc[k] * edge[k], edge[k]) for k in range(3)]
+                tot = halo_total(counts, borg, [c + 2 for c in bcen])
+                if tot <= capacity:
+                    continue
+                over[0] += 1
+                per = tot / float(np.prod([c + 2 for c in bcen]))
+                m = list(bcen)
+                for _ in range(32):
+                    hv = float(np.prod([x + 2 for x in m]))
+                    if max(m) <= 1 or hv * per <= 0.75 * capacity:
+                        break
+                    k = int(np.argmax(m))  # the first of the largest
+                    m[k] = (m[k] + 1) // 2
+                sp = [(bcen[k] + m[k] - 1) // m[k] for k in range(3)]
+                for q in np.ndindex(*sp):
+                    org = [borg[k] + q[k] * m[k] for k in range(3)]
+                    cen = [min(m[k], bcen[k] - q[k] * m[k]) for k in range(3)]
+                    if halo_total(counts, org, [c + 2 for c in cen]) <= capacity:
+                        continue
+                    over[1] += 1
+                    for c in np.ndindex(*cen):
+                        corg = [org[k] + c[k] for k in range(3)]
+                        if halo_total(counts, corg, [3, 3, 3]) > capacity:
+                            over[2] += 1
+    return over

@@ -875,6 +875,62 @@ def test_elongated_boxes(shape):
     assert np.array_equal(da["x"], db["x"]) and np.array_equal(da["v"], db["v"])
 
 
+def droplet_in_gas(seed=41):
+    """A dense FCC droplet (8^3 cells, a = 0.5256 nm, 2048 atoms) in a
+    dilute gas (a jittered SC lattice of 14 / 13 nm, 14 nm box): the grid's
+    mean occupancy sizes the bricks for the gas, so bricks over the droplet
+    hold more particles than LDS stages."""
+    rng = np.random.default_rng(seed)
+    box = 14.0
+    a = 0.5256
+    off = (box - 8 * a) / 2
+    drop = O.lattice_fcc([8, 8, 8], a) + off
+    g = O.lattice_u([13, 13, 13], box / 13) + rng.uniform(-0.25, 0.25, (13 ** 3, 3))
+    lo, hi = off - 0.6, off + 8 * a + 0.6
+    keep = ~np.all((g > lo) & (g < hi), axis=1)
+    pos = np.concatenate([drop + rng.uniform(-0.01, 0.01, drop.shape), g[keep]])
+    n = len(pos)
+    s = O.State([n], [AR_MASS], [box] * 3, pos=pos, vel=O.mb_velocities(n, 100.0, AR_MASS, seed))
+    O.apply_boundary_conditions(s)
+    return s
+
+
+def test_dense_droplet_splits_bricks():
+    """Bricks over the droplet overflow the LDS staging and are built as
+    sub-bricks, and sub-bricks that still overflow cell by cell (restated in
+    tests/common.py and asserted here): FP64 list forces equal the oracle's,
+    FP32 ones are within 1e-5, and a thin-skin run of 100 steps (rebuilds
+    through the split regions) equals the all-pairs path bit for bit."""
+    from tests.common import brick_dims, brick_regions, cell_counts, grid_dims
+    s = droplet_in_gas()
+    rlist = 2.5 * 0.3418 + 0.3 * 0.3418
+    gd = grid_dims(s.box, rlist, int(np.ceil(2.0 * s.n)))
+    edge, whole = brick_dims(gd, s.n)
+    over = brick_regions(cell_counts(s.pos, s.box, gd), edge, whole)
+    print("grid", gd, "brick", edge, "overflowing regions", over)
+    assert over[0] > 0 and over[1] > 0 and over[2] == 0
+    ref = s.copy()
+    O.update_force_cells(ref)
+    assert np.count_nonzero(ref.force.any(axis=1)) > 800
+    for mode in ("fp64_list", "fp32_list"):
+        e = engine_for(s, mode)
+        e.update_force()
+        f = e.download(("force",))["force"]
+        e.close()
+        if mode == "fp64_list":
+            assert np.array_equal(f, ref.force)
+        else:
+            assert rms(f - ref.force) <= 1e-5 * rms(ref.force)
+    thin = engine_for(s, "fp64_list", skin=0.02)
+    allp = engine_for(s, "fp64_allpairs")
+    for eng in (thin, allp):
+        eng.update_force()
+        eng.step(0.002, 100)
+    da, db = allp.download(), thin.download()
+    assert thin.stats()["rebuilds"] >= 3
+    assert np.array_equal(da["x"], db["x"]) and np.array_equal(da["v"], db["v"])
+
+
 def test_dense_bricks_fall_back_to_grid_scan():
     """Liquid argon with ~32 atoms per cell: brick halos exceed the LDS
     staging capacity and lists are built straight from the grid."""
