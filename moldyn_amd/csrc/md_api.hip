@@ -813,15 +813,21 @@ static int read_ctrl_stats(md_ctx* c, std::vector<RepCtrl>& C, RepStats& s) {
     return MD_OK;
 }
 
-// Grow list / cell capacity after an overflow; returns 1 when grown.
-static int grow_capacity(md_ctx* c, const RepStats& s, int* grown) {
+// Grow list / cell capacity; returns 1 when grown.  After an overflow (the
+// chunk is replayed) the capacity at least doubles; ahead of one (a list or
+// cell within a quarter of its capacity) it grows by a quarter: dense
+// builds keep each centre particle's list column in a fixed LDS pool, so a
+// capacity far above the longest list costs build passes.
+static int grow_capacity(md_ctx* c, const RepStats& s, int* grown, bool ahead = false) {
     *grown = 0;
     if (s.list_overflow) {
-        c->d.list_cap = std::max(c->d.list_cap * 2, (int)(s.max_list * 1.25) + 4);
+        const int cap = c->d.list_cap;
+        c->d.list_cap = std::max(ahead ? cap + (cap + 3) / 4 : cap * 2, (int)(s.max_list * 1.25) + 4);
         *grown = 1;
     }
     if (s.cell_overflow) {
-        c->d.cell_cap = std::max(c->d.cell_cap * 2, (int)(s.max_cell * 1.25) + 4);
+        const int cap = c->d.cell_cap;
+        c->d.cell_cap = std::max(ahead ? cap + (cap + 3) / 4 : cap * 2, (int)(s.max_cell * 1.25) + 4);
         *grown = 1;
     }
     if (*grown) return alloc_lists(c);
@@ -1009,7 +1015,7 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
             g.max_list = s.max_list;
             g.max_cell = s.max_cell;
             int grown = 0;
-            if ((rc = grow_capacity(c, g, &grown))) return rc;
+            if ((rc = grow_capacity(c, g, &grown, true))) return rc;
             for (int r = 0; r < c->R; ++r) C[r].flags &= ~(F_HAVE_LIST | F_GRID_FRESH);
             if ((rc = write_ctrl(c, C))) return rc;
             st.grows++;

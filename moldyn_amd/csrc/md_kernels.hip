@@ -438,8 +438,17 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             if (MULTI) s_ty[a] = (uint8_t)rec_type(d, recs, j);
         }
         lds_barrier();
-        const int seg_cap = n_own > 0 ? kBrickSeg / n_own : 0;
-        for (int k = threadIdx.x; k < n_own; k += BS) {
+        // LDS list columns (dense instances): every centre particle of a pass
+        // owns a column of list_cap entries in the pool, so a long capacity
+        // splits the centre particles into passes instead of sending their
+        // lists to global memory
+        const int dl = d.list_cap > 1 ? d.list_cap : 1;
+        const int per_pass = SEG ? (n_own < kBrickSeg / dl ? n_own : (kBrickSeg / dl > 1 ? kBrickSeg / dl : 1))
+                                 : n_own;
+        for (int p0 = 0; p0 < n_own; p0 += per_pass) {
+        const int p1 = n_own - p0 < per_pass ? n_own : p0 + per_pass;
+        const int seg_cap = kBrickSeg / (p1 - p0);
+        for (int k = p0 + threadIdx.x; k < p1; k += BS) {
             const int a = s_cidx[k];
             const int h = s_hc[a];
             const int hc[3] = {h / (hd[1] * hd[2]), (h / hd[2]) % hd[1], h % hd[2]};
@@ -479,24 +488,29 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             const float xf = pi.x, yf = pi.y, zf = pi.z;
             RegList<SEG> lst(d.nbr + i, N, d.list_cap);
             if (SEG && seg_cap >= d.list_cap) {
-                lst.seg = L.seg + k;
-                lst.seg_stride = n_own;
+                lst.seg = L.seg + (k - p0);
+                lst.seg_stride = p1 - p0;
             }
             auto scan = [&](int b0, int b1) {
+                // the next candidate's position is read while this one is
+                // tested (the index stays in the run)
+                float4 pn = s_pf[b0 < b1 ? b0 : 0];
                 for (int b2 = b0; b2 < b1; ++b2) {
                     // FP32 test first: it never rejects a pair the FP64 test keeps
-                    const float4 pj = s_pf[b2];
+                    const float4 pj = pn;
+                    pn = s_pf[b2 + 1 < b1 ? b2 + 1 : b2];
                     const float ex = min_image_f(pj.x - xf, bxf);
                     const float ey = min_image_f(pj.y - yf, byf);
                     const float ez = min_image_f(pj.z - zf, bzf);
                     if (!((ex * ex + ey * ey) + ez * ez <= rl2f)) continue;
-                    const int32_t j = s_id[b2];
-                    if (j == i) continue;
+                    // a particle is staged once per region: b2 == a is j == i
+                    if (b2 == a) continue;
                     if (MULTI && (int)s_ty[b2] < ti) continue;
                     if (SEG && lst.seg) {
                         lst.push_raw(b2);  // the FP64 test runs once the scan is done
                         continue;
                     }
+                    const int32_t j = s_id[b2];
                     const double dx = min_image_sel(s_x[b2] - xi, bx);
                     const double dy = min_image_sel(s_y[b2] - yi, by);
                     const double dz = min_image_sel(s_z[b2] - zi, bz);
@@ -519,8 +533,11 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                 // FP32-accepted candidates, in a loop every lane runs at once
                 int kept = 0;
                 const int m = lst.n < d.list_cap ? lst.n : d.list_cap;
+                // the next entry is read before this one is kept (kept <= q)
+                int bn = m > 0 ? lst.seg[0] : 0;
                 for (int q = 0; q < m; ++q) {
-                    const int b2 = lst.seg[q * lst.seg_stride];
+                    const int b2 = bn;
+                    if (q + 1 < m) bn = lst.seg[(q + 1) * lst.seg_stride];
                     const double dx = min_image_sel(s_x[b2] - xi, bx);
                     const double dy = min_image_sel(s_y[b2] - yi, by);
                     const double dz = min_image_sel(s_z[b2] - zi, bz);
@@ -536,6 +553,8 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                 list_close(d, i, lst.n);
             }
         }
+        if (p1 < n_own) lds_barrier();  // the next pass reuses the columns
+        }  // passes
         }  // the region fitted in LDS
         lds_barrier();
         // the next region: the sub-brick's next cell, the brick's next

@@ -237,8 +237,10 @@ def test_list_schedule_invariance(mode):
     on capacity-overflow replays or on graph capture."""
     s = liquid_state(4, temperature=150.0)
     outs = []
+    # list_capacity=600: more LDS list columns than the pool holds at once,
+    # so dense bricks build their centre particles in passes
     for kw in [dict(), dict(skin=0.05), dict(skin=0.4, cells_per_atom=0.2),
-               dict(list_capacity=2, cell_capacity=1), dict(graph_chunk=1)]:
+               dict(list_capacity=2, cell_capacity=1), dict(graph_chunk=1), dict(list_capacity=600)]:
         e = engine_for(s, mode, **kw)
         e.update_force()
         e.step(0.002, 200)
