@@ -377,6 +377,33 @@ def kernel_roofline(e, n, steps, block_size, traffic):
             "algorithmic_bytes_per_launch": round(bytes_per_launch)}
 
 
+def counter_evidence():
+    """Hardware-counter figures from the committed rocprofv3 summaries
+    (profiles/): HBM GB/s of the neighbour build and of integration at
+    1M atoms, and the VALU FLOP rates of the force loop on the dense liquid
+    (the FLOPS counters count per wave instruction: x 64 lanes)."""
+    from rocprof_summary import read_counters, read_rate
+    out = {}
+    prof = os.path.join(ROOT, "profiles")
+    for key, name in (("integration_k_step_1M", "k_step_1M_npt_traffic.txt"),
+                      ("neighbour_build_k_fin_1M", "k_fin_build_1M_npt_traffic.txt")):
+        r = read_rate(os.path.join(prof, name))
+        if r and r[0] and r[1]:
+            gbs = r[0] / (r[1] * 1e-6) / 1e9
+            out[key] = {"bytes_per_launch": r[0], "median_us": r[1], "hbm_GBps": round(gbs, 1),
+                        "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4), "source": f"profiles/{name}"}
+    c = read_counters(os.path.join(prof, "k_step_262k_liquid_counters.txt"))
+    if c and c[1]:
+        v, dur = c
+        f32 = v.get("SQ_INSTS_VALU_FLOPS_FP32", 0.0) * 64 / (dur * 1e-6)
+        f64 = v.get("SQ_INSTS_VALU_FLOPS_FP64", 0.0) * 64 / (dur * 1e-6)
+        out["forces_k_step_262k_liquid"] = {
+            "median_us": dur, "fp32_TFLOPs": round(f32 / 1e12, 3), "fp64_TFLOPs": round(f64 / 1e12, 3),
+            "fp32_frac_of_157.3": round(f32 / 157.3e12, 4), "fp64_frac_of_78.6": round(f64 / 78.6e12, 4),
+            "valu_wave_insts": v.get("SQ_INSTS_VALU"), "source": "profiles/k_step_262k_liquid_counters.txt"}
+    return out or None
+
+
 def force_error(f, ref):
     """Force error of f against ref (n x 3); relative figures are None when
     no atom of ref has a force (the comparison would be vacuous)."""
@@ -749,6 +776,7 @@ def main():
         "comparators": comparators,
         "force_error_vs_fp64": force_err,
         "accuracy_vs_oracle": acc,
+        "counter_evidence": counter_evidence(),
         "other_workloads": extras or None,
     }
     print(json.dumps(out, default=float), flush=True)

@@ -213,6 +213,37 @@ def counters(args):
     _emit("\n".join(lines) + "\n", args.out)
 
 
+def read_counters(path):
+    """({counter: median per dispatch}, median kernel-trace duration in us or
+    None) from a summary written by ``counters`` (None if absent)."""
+    if not os.path.exists(path):
+        return None
+    vals, dur = {}, None
+    with open(path) as fh:
+        for line in fh:
+            parts = line.split()
+            if len(parts) == 5 and parts[0].isupper() and parts[1].isdigit():
+                vals[parts[0]] = float(parts[2])
+            if line.strip().startswith("dispatches") and "median" in line:
+                dur = float(line.split("median")[1].split("us")[0])
+    return vals, dur
+
+
+def read_rate(path):
+    """(bytes per launch, median duration us) from a ``traffic`` summary with
+    dispatch durations (None if absent)."""
+    if not os.path.exists(path):
+        return None
+    b, dur = None, None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("bytes_per_launch:"):
+                b = int(float(line.split(":", 1)[1]))
+            if line.strip().startswith("median") and "dispatches" in line:
+                dur = float(line.split()[1])
+    return b, dur
+
+
 def read_traffic(path):
     """bytes_per_launch from a summary written by ``traffic`` (None if absent)."""
     if not os.path.exists(path):
