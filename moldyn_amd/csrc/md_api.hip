@@ -703,7 +703,7 @@ static hipError_t launch_once(md_ctx* c) {
     c->d.par = c->par;
     if (c->mlp) {
         if ((e = launch_mlp_desc(c->d, c->bs, false, c->st))) return e;
-        if ((e = launch_mlp_net(c->d, c->st))) return e;
+        if ((e = launch_mlp_net(c->d, c->n_cu, c->st))) return e;
         if ((e = launch_mlp_force(c->d, c->bs, false, c->st))) return e;
     } else if (c->list_mode) {
         if ((e = launch_step(c->d, c->fp32, c->bs, c->st))) return e;
@@ -861,7 +861,7 @@ static int list_force_eval(md_ctx* c) {
             HIPCHK(c, launch_build(c->d, c->fp32, c->st));
             if (c->mlp) {
                 HIPCHK(c, launch_mlp_desc(c->d, c->bs, true, c->st));
-                HIPCHK(c, launch_mlp_net(c->d, c->st));
+                HIPCHK(c, launch_mlp_net(c->d, c->n_cu, c->st));
                 HIPCHK(c, launch_mlp_force(c->d, c->bs, true, c->st));
             } else {
                 HIPCHK(c, launch_force_eval(c->d, c->fp32, c->bs, c->st));
@@ -1923,6 +1923,7 @@ int md_mlp_set_params(md_ctx* c, const float* theta, const float* g_mean, const 
         k.bas_c = (float)exp(-2.0 * a * dm * dm);
     }
     k.rc = (float)(2.5 * sigma);
+    k.pi_rc = (float)(M_PI / (2.5 * sigma));
     k.e_scale = (float)e_scale;
     k.e_shift = (float)e_shift;
     k.b3 = theta[kMlpOffB3];

@@ -191,6 +191,7 @@ struct MlpConst {
     // t_k = r - mu_k, and q_{k+1} = q_k exp(-2 a D^2)
     float bas_2ad, bas_ad2, bas_c;          // 2 a D, a D^2, exp(-2 a D^2)
     float rc;                               // basis cutoff (nm)
+    float pi_rc;                            // pi / rc (cosine cutoff argument per nm)
     float e_scale, e_shift;
     float b3;
     int32_t judge_every;                    // LJ force check interval (steps), 0 = never
@@ -425,7 +426,7 @@ hipError_t launch_wrap(const DevState& d, bool fp32, int bs, hipStream_t st);
 // surrogate step kernels (md_mlp.hip); force_only evaluates at the current
 // positions without advancing time
 hipError_t launch_mlp_desc(const DevState& d, int bs, bool force_only, hipStream_t st);
-hipError_t launch_mlp_net(const DevState& d, hipStream_t st);
+hipError_t launch_mlp_net(const DevState& d, int max_blocks, hipStream_t st);
 hipError_t launch_mlp_force(const DevState& d, int bs, bool force_only, hipStream_t st);
 hipError_t launch_mlp_convert(const DevState& d, hipStream_t st);
 

@@ -11,7 +11,7 @@
 //                (neighbour drift recomputed from the input records);
 //   k_mlp_net    per 16-atom row block of a wave: the forward pass and the
 //                backward pass to dE/dG on BF16 MFMA (FP32 accumulation),
-//                weights staged in LDS;
+//                weights staged in LDS once per block, one block per CU;
 //   k_mlp_force  pair forces from both atoms' dE/dG, the LJ fallback when a
 //                feature left the training range, the periodic LJ check,
 //                second half kick and barostat scale into buffer cur;
@@ -26,7 +26,8 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr float kPi = 3.14159265358979323846f;
 
 __device__ __forceinline__ float silu_f(float a, float& ds) {
-    const float s = 1.0f / (1.0f + __expf(-a));
+    // the hardware reciprocal (1 ulp): the activations are rounded to BF16
+    const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-a));
     ds = s * (1.0f + a * (1.0f - s));
     return a * s;
 }
@@ -88,8 +89,9 @@ __device__ __forceinline__ void mlp_features(const DevState& d, const MlpConst& 
         const double dz = min_image(zj - z1, bz);
         const double r2 = (dx * dx + dy * dy) + dz * dz;
         if (!(r2 < rc2)) continue;
-        const float r = (float)sqrt(r2);
-        const float fc = 0.5f * (__cosf(kPi * r / mc.rc) + 1.0f);
+        // FP32 distance and cutoff from the hardware square root (1 ulp)
+        const float r = __builtin_amdgcn_sqrtf((float)r2);
+        const float fc = 0.5f * (__cosf(r * mc.pi_rc) + 1.0f);
         gauss_basis(r, mc, [&](int k, float e) { g[k] = fmaf(e, fc, g[k]); });
     }
 }
@@ -231,14 +233,18 @@ __global__ __launch_bounds__(kNetBS) void k_mlp_net(DevState d) {
     const uint64_t t_start = prof_now(d);
     const MlpConst& mc = *d.mlp_c;
     const __bf16* w = d.mlp_w;
-    for (int i = threadIdx.x; i < kMlpH * kMlpK; i += kNetBS) {
-        sW1[(i / kMlpK) * kStrK + i % kMlpK] = w[kMlpBfW1 + i];
-        sW1t[(i / kMlpH) * kStrH + i % kMlpH] = w[kMlpBfW1t + i];
-    }
-    for (int i = threadIdx.x; i < kMlpH * kMlpH; i += kNetBS) {
-        sW2[(i / kMlpH) * kStrH + i % kMlpH] = w[kMlpBfW2 + i];
-        sW2t[(i / kMlpH) * kStrH + i % kMlpH] = w[kMlpBfW2t + i];
-    }
+    // rows of the BF16 weights into the padded LDS rows, 16 bytes at a time
+    auto stage = [&](__bf16* dst, int stride, const __bf16* src, int rows, int cols) {
+        const int per = cols / 8;
+        for (int i = threadIdx.x; i < rows * per; i += kNetBS) {
+            const int r = i / per, c8 = i % per;
+            *(bf16x8*)(dst + r * stride + c8 * 8) = *(const bf16x8*)(src + r * cols + c8 * 8);
+        }
+    };
+    stage(sW1, kStrK, w + kMlpBfW1, kMlpH, kMlpK);
+    stage(sW1t, kStrH, w + kMlpBfW1t, kMlpK, kMlpH);
+    stage(sW2, kStrH, w + kMlpBfW2, kMlpH, kMlpH);
+    stage(sW2t, kStrH, w + kMlpBfW2t, kMlpH, kMlpH);
     for (int i = threadIdx.x; i < kMlpH; i += kNetBS) {
         sb1[i] = d.mlp_theta[kMlpOffB1 + i];
         sb2[i] = d.mlp_theta[kMlpOffB2 + i];
@@ -419,17 +425,17 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
             const double r2 = (dx * dx + dy * dy) + dz * dz;
             if (lj && r2 <= p0.r2max) acc.add(dx, dy, dz, r2, p0);
             if (!(r2 < rc2)) continue;
-            const float rr = (float)sqrt(r2);
-            const float arg = kPi * rr / mc.rc;
+            const float rr = __builtin_amdgcn_sqrtf((float)r2);
+            const float arg = rr * mc.pi_rc;
             const float fc = 0.5f * (__cosf(arg) + 1.0f);
-            const float dfc = -0.5f * kPi / mc.rc * __sinf(arg);
+            const float dfc = -0.5f * mc.pi_rc * __sinf(arg);
             float cc = 0.f;
             gauss_basis(rr, mc, [&](int k, float e) {
                 const float tt = rr - mc.mu[k];
                 const float dg = e * fmaf(-tt * mc.inv_w2, fc, dfc);
                 cc = fmaf(gi[k] + gj[k / 4][k % 4], dg, cc);
             });
-            const float ir = 1.0f / rr;
+            const float ir = __builtin_amdgcn_rcpf(rr);
             fx = fmaf(cc * ir, (float)dx, fx);
             fy = fmaf(cc * ir, (float)dy, fy);
             fz = fmaf(cc * ir, (float)dz, fz);
@@ -525,9 +531,11 @@ hipError_t launch_mlp_desc(const DevState& d, int bs, bool force_only, hipStream
     return hipGetLastError();
 }
 
-hipError_t launch_mlp_net(const DevState& d, hipStream_t st) {
+hipError_t launch_mlp_net(const DevState& d, int max_blocks, hipStream_t st) {
+    // one block per CU (the weights fill most of its LDS): each block stages
+    // the weights once and its waves stride over the row tiles
     const int64_t rows = d.n_slots / 16;
-    int grid = (int)std::min<int64_t>((rows + kNetWaves - 1) / kNetWaves, 1024);
+    int grid = (int)std::min<int64_t>((rows + kNetWaves - 1) / kNetWaves, std::max(1, max_blocks));
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(k_mlp_net, dim3(grid), dim3(kNetBS), 0, st, d);
     return hipGetLastError();

@@ -1,6 +1,6 @@
 """Text summaries of rocprofv3 output for the measurement records.
 
-Three commands:
+Four commands:
 
   python rocprof_summary.py stats DIR --command "..." --out FILE
       per-kernel duration statistics of a ``rocprofv3 --kernel-trace`` run
@@ -20,6 +20,10 @@ Three commands:
       [--trace DIR] --out FILE
       median per dispatch of every counter of a ``--pmc`` run, with the
       durations of a kernel-trace run of the same program beside them.
+
+  python rocprof_summary.py device BENCH_JSON --command "..." --out FILE
+      per-kernel device-clock times of the surrogate workload from the JSON
+      line of ``bench.py --only mlp`` (no profiler involved).
 
 Only the standard library is used; the module reads what rocprofv3 wrote and
 never runs anything.
@@ -213,6 +217,36 @@ def counters(args):
     _emit("\n".join(lines) + "\n", args.out)
 
 
+def device_kernels(args):
+    """Per-kernel device-clock times of the surrogate workload from the JSON
+    line of ``bench.py --only mlp`` (md_kernel_time: every block's start and
+    end on the GPU wall clock), with the network kernel's MFMA figures."""
+    import json
+    with open(args.json) as fh:
+        d = json.loads(fh.read().strip().splitlines()[-1])
+    if "mlp" not in d:
+        d = d["other_workloads"]["mlp_surrogate"]
+    lines = [f"# device-clock kernel times: {args.title}", "", "command:", "", f"    {args.command}", "",
+             "times in microseconds per launch, first block start to last block end (span) and from the",
+             "previous kernel's end (stream), from per-block s_memrealtime stamps (md_kernel_time)", ""]
+    for leg in ("mlp", "lj_fp32"):
+        r = d[leg]
+        lines.append(f"{leg}: {r['ms_per_step'] * 1e3:.1f} us per step over the timed run, "
+                     f"{r.get('rebuilds', '?')} rebuilds in {r.get('launches', '?')} launches")
+        for k, v in r["kernels"].items():
+            lines.append(f"  {k:14s} span {v['span_us']:8.1f}  stream {v['stream_us']:8.1f}  launches {v['launches']}")
+        lines.append("")
+    m = d["mlp"].get("k_mlp_net_mfma")
+    if m:
+        lines += ["k_mlp_net MFMA:",
+                  f"  flops per launch {m['flops_per_launch']:.4g} (80 MFMA 16x16x32 BF16 per 16 atoms)",
+                  f"  {m['achieved_tflops']:.1f} TFLOP/s in {m['span_us']:.1f} us = {m['frac']:.4f} of "
+                  f"{m['peak_tflops_bf16_dense']:.0f} TFLOP/s dense BF16", ""]
+    lines += [f"speed_vs_lj {d['speed_vs_lj']:.4f}, held-out force RMSE {d['heldout_force_rmse_rel']:.4f}, "
+              f"largest periodic LJ check {d['mlp'].get('max_rmse_rel', float('nan')):.4f}"]
+    _emit("\n".join(lines) + "\n", args.out)
+
+
 def read_counters(path):
     """({counter: median per dispatch}, median kernel-trace duration in us or
     None) from a summary written by ``counters`` (None if absent)."""
@@ -293,8 +327,13 @@ def main(argv=None):
     c.add_argument("--trace", help="kernel-trace directory of the same program, for durations")
     c.add_argument("--trace-command")
     c.add_argument("--out")
+    k = sub.add_parser("device", help="device-clock kernel times from a bench.py --only mlp JSON line")
+    k.add_argument("json")
+    k.add_argument("--command", required=True)
+    k.add_argument("--title", default="")
+    k.add_argument("--out")
     a = ap.parse_args(argv)
-    {"stats": stats, "traffic": traffic, "counters": counters}[a.cmd](a)
+    {"stats": stats, "traffic": traffic, "counters": counters, "device": device_kernels}[a.cmd](a)
 
 
 if __name__ == "__main__":
