@@ -99,6 +99,8 @@ struct md_ctx {
     RepCtrl* h_ctrl = nullptr;
     RepStats* h_stats = nullptr;
     RepParams* h_params = nullptr;
+    RepCtrl* h_ctrl_dev = nullptr;    // device addresses of h_ctrl / h_stats (the epilogue writes there)
+    RepStats* h_stats_dev = nullptr;
     hipEvent_t params_ev = nullptr;  // the last parameter upload (h_params is reused after it)
     bool params_pending = false;
     // host copy of the control blocks the next launch reads, valid while no
@@ -247,6 +249,8 @@ static void free_all(md_ctx* c) {
     c->params_pending = false;
     if (c->h_params) (void)hipHostFree(c->h_params);
     c->h_params = nullptr;
+    c->h_ctrl_dev = nullptr;
+    c->h_stats_dev = nullptr;
     c->d.prof = nullptr;
     c->d.prof_blk = nullptr;
     c->prof = false;  // a new layout starts unprofiled
@@ -537,6 +541,10 @@ static int pinned_staging(md_ctx* c) {
     c->h_stats = (RepStats*)p;
     HIPCHK(c, hipHostMalloc(&p, sizeof(RepParams) * c->R, hipHostMallocDefault));
     c->h_params = (RepParams*)p;
+    HIPCHK(c, hipHostGetDevicePointer(&p, c->h_ctrl, 0));
+    c->h_ctrl_dev = (RepCtrl*)p;
+    HIPCHK(c, hipHostGetDevicePointer(&p, c->h_stats, 0));
+    c->h_stats_dev = (RepStats*)p;
     HIPCHK(c, hipEventCreateWithFlags(&c->params_ev, hipEventDisableTiming));
     return MD_OK;
 }
@@ -755,13 +763,19 @@ static size_t state_bytes(md_ctx* c) {
     return recs + sizeof(double) * NS * c->groups.size() + (c->d.mlp_stat ? sizeof(MlpStat) * c->R : 0);
 }
 
-static int snapshot(md_ctx* c, bool save) {
-    const size_t n = sizeof(double) * (size_t)c->n_slots;
+static int snapshot_alloc(md_ctx* c) {
     if (!c->snap) {
         c->snap_bytes = std::max<size_t>(state_bytes(c), 256);
         HIPCHK(c, hipMalloc(&c->snap, c->snap_bytes));
         HIPCHK(c, hipMalloc((void**)&c->snap_ctrl, sizeof(RepCtrl) * c->R));
     }
+    return MD_OK;
+}
+
+static int snapshot(md_ctx* c, bool save) {
+    const size_t n = sizeof(double) * (size_t)c->n_slots;
+    int rc;
+    if ((rc = snapshot_alloc(c))) return rc;
     char* s = (char*)c->snap;
     const bool recs = records_in_snapshot(c);
     void* arrs[6] = {c->d.rec[0], c->d.rec[1], c->d.pot, c->d.vir, c->d.sums, c->d.mlp_stat};
@@ -803,8 +817,9 @@ static int read_stats(md_ctx* c, RepStats& s) {
 static int read_ctrl_stats(md_ctx* c, std::vector<RepCtrl>& C, RepStats& s) {
     int rc;
     if ((rc = pinned_staging(c))) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->h_ctrl, ctrl_cur(c), sizeof(RepCtrl) * c->R, hipMemcpyDeviceToHost, c->st));
-    HIPCHK(c, hipMemcpyAsync(c->h_stats, c->d.stats, sizeof(RepStats), hipMemcpyDeviceToHost, c->st));
+    // one kernel writes both into the pinned buffers (two copies would be
+    // two device operations)
+    HIPCHK(c, launch_epilogue(c->d, c->par, c->h_ctrl_dev, c->h_stats_dev, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
     C.assign(c->h_ctrl, c->h_ctrl + c->R);
     s = *c->h_stats;
@@ -947,8 +962,18 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
     md_stats st{};
     while (done < n) {
         const int64_t cn = std::min(n - done, chunk);
+        // one replica: parameters, cleared statistics and (step-record path)
+        // the snapshot in one prologue kernel instead of several copies
+        const bool one = c->R == 1;
+        bool snap_in_prologue = false;
         if (c->list_mode) {
-            if ((rc = snapshot(c, true))) return rc;
+            if (one && !records_in_snapshot(c)) {
+                if ((rc = snapshot_alloc(c))) return rc;
+                c->snap_par = c->par;
+                snap_in_prologue = true;
+            } else if ((rc = snapshot(c, true))) {
+                return rc;
+            }
             // dense systems (long lists expected at the current densities)
             // step and build with the long-list kernel instances
             double ne = 0.0;
@@ -965,8 +990,18 @@ static int run_steps(md_ctx* c, double dt, md_thermostat* th, md_barostat* ba, i
             P[r].use_psi_in = first ? 1 : 0;
             P[r].full_dst = c->d.compact ? ((C[r].cur & 1) ^ 1) : -1;
         }
-        if ((rc = write_params(c, P))) return rc;
-        if ((rc = reset_stats(c))) return rc;
+        if (one) {
+            // the snapshot's layout without records: the group sums, then the
+            // surrogate's counters (snapshot())
+            char* sb = (char*)c->snap;
+            const size_t sums_bytes = sizeof(double) * NS * c->groups.size();
+            HIPCHK(c, launch_prologue(c->d, P[0], snap_in_prologue ? (double*)sb : nullptr, (int)c->groups.size(),
+                                      snap_in_prologue && c->d.mlp_stat ? (MlpStat*)(sb + sums_bytes) : nullptr,
+                                      snap_in_prologue ? c->snap_ctrl : nullptr, c->par, c->st));
+        } else {
+            if ((rc = write_params(c, P))) return rc;
+            if ((rc = reset_stats(c))) return rc;
+        }
         if (c->d.compact && !(c->crec_valid && c->crec_dt == dt)) {
             // the chunk steps on step records; its last step writes the full
             // ones back (and leaves the step records valid for the next call)

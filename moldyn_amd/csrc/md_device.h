@@ -411,6 +411,15 @@ hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st);
 hipError_t launch_pack(const DevState& d, bool fp32, int bs, hipStream_t st);
 hipError_t launch_build(const DevState& d, bool fp32, hipStream_t st);
 hipError_t launch_prof_init(const DevState& d, hipStream_t st);
+// Chunk prologue of a one-replica context in one launch: the parameters
+// (from the kernel argument), cleared statistics, and the snapshot a replay
+// restarts from (group sums of n_groups groups, the surrogate's counters when
+// snap_mlp is set, and the control block the next launch reads).
+hipError_t launch_prologue(const DevState& d, const RepParams& p, double* snap_sums, int n_groups,
+                           MlpStat* snap_mlp, RepCtrl* snap_ctrl, int par, hipStream_t st);
+// Chunk epilogue: the control blocks the next launch reads (n_reps) and the
+// statistics, written into host memory the device can address.
+hipError_t launch_epilogue(const DevState& d, int par, RepCtrl* h_ctrl, RepStats* h_stats, hipStream_t st);
 // finalize (grid n_reps); with `build` the grid also covers the list-build
 // blocks, which build the lists of replicas flagged F_BUILD
 hipError_t launch_finalize(const DevState& d, int mode, bool build, bool fp32, hipStream_t st);

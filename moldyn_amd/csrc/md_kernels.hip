@@ -1812,6 +1812,39 @@ hipError_t launch_build(const DevState& d, bool fp32, hipStream_t st) {
     return hipGetLastError();
 }
 
+__global__ __launch_bounds__(256) void k_prologue(DevState d, RepParams p, double* snap_sums, int n_groups,
+                                                  MlpStat* snap_mlp, RepCtrl* snap_ctrl, int par) {
+    const int t = threadIdx.x;
+    // word copies (every structure is a whole number of 32-bit words)
+    auto copy = [&](void* dst, const void* src, int bytes) {
+        for (int i = t; i < bytes / 4; i += 256) ((uint32_t*)dst)[i] = ((const uint32_t*)src)[i];
+    };
+    copy((void*)d.params, &p, (int)sizeof(RepParams));
+    for (int i = t; i < (int)(sizeof(RepStats) / 4); i += 256) ((uint32_t*)d.stats)[i] = 0u;
+    if (!snap_ctrl) return;  // no snapshot in this prologue
+    copy(snap_sums, d.sums, (int)sizeof(double) * NS * n_groups);
+    if (snap_mlp) copy(snap_mlp, d.mlp_stat, (int)sizeof(MlpStat));
+    copy(snap_ctrl, d.ctrl + (size_t)par * d.n_reps, (int)sizeof(RepCtrl));
+}
+
+__global__ __launch_bounds__(256) void k_epilogue(DevState d, int par, RepCtrl* h_ctrl, RepStats* h_stats) {
+    const int t = threadIdx.x;
+    const uint32_t* c = (const uint32_t*)(d.ctrl + (size_t)par * d.n_reps);
+    for (int i = t; i < (int)(sizeof(RepCtrl) / 4) * d.n_reps; i += 256) ((uint32_t*)h_ctrl)[i] = c[i];
+    for (int i = t; i < (int)(sizeof(RepStats) / 4); i += 256) ((uint32_t*)h_stats)[i] = ((const uint32_t*)d.stats)[i];
+}
+
+hipError_t launch_prologue(const DevState& d, const RepParams& p, double* snap_sums, int n_groups,
+                           MlpStat* snap_mlp, RepCtrl* snap_ctrl, int par, hipStream_t st) {
+    hipLaunchKernelGGL(k_prologue, dim3(1), dim3(256), 0, st, d, p, snap_sums, n_groups, snap_mlp, snap_ctrl, par);
+    return hipGetLastError();
+}
+
+hipError_t launch_epilogue(const DevState& d, int par, RepCtrl* h_ctrl, RepStats* h_stats, hipStream_t st) {
+    hipLaunchKernelGGL(k_epilogue, dim3(1), dim3(256), 0, st, d, par, h_ctrl, h_stats);
+    return hipGetLastError();
+}
+
 hipError_t launch_prof_init(const DevState& d, hipStream_t st) {
     const int n = (int)kProfCount + 3;
     hipLaunchKernelGGL(k_prof_init, dim3((n + 255) / 256), dim3(256), 0, st, d);
