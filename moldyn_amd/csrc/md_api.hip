@@ -416,6 +416,8 @@ static int alloc_mlp(md_ctx* c) {
     HIPCHK(c, hipMemset(p, 0, sizeof(__bf16) * N * kMlpK));
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(float) * N * kMlpK))) return rc;
     d.mlp_gam = (float*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(__bf16) * N * kMlpK))) return rc;
+    d.mlp_gamb = (__bf16*)p;
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(float) * N))) return rc;
     d.mlp_e = (float*)p;
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(MlpStat) * c->R))) return rc;

@@ -244,6 +244,7 @@ struct DevState {
     const __bf16* mlp_w;          // kMlpBfN
     __bf16* mlp_x;                // n_slots x kMlpK normalised features
     float* mlp_gam;               // n_slots x kMlpK  dE_i / dG_i
+    __bf16* mlp_gamb;             // the same in BF16: the copy neighbours gather
     float* mlp_e;                 // n_slots          E_i
     MlpStat* mlp_stat;            // n_reps
     double* mlp_judge;            // 2 x n_blocks per-block check sums

@@ -318,8 +318,11 @@ __global__ __launch_bounds__(kNetBS) void k_mlp_net(DevState d) {
                 acc = mfma(ah[kc], *(const bf16x8*)(sW1t + (nt * 16 + lr) * kStrH + kc * 32 + lq * 8), acc);
             const int col = nt * 16 + lr;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                d.mlp_gam[(r0 + lq * 4 + i) * kMlpK + col] = mc.e_scale * mc.g_inv_std[col] * acc[i];
+            for (int i = 0; i < 4; ++i) {
+                const float v = mc.e_scale * mc.g_inv_std[col] * acc[i];
+                d.mlp_gam[(r0 + lq * 4 + i) * kMlpK + col] = v;
+                d.mlp_gamb[(r0 + lq * 4 + i) * kMlpK + col] = (__bf16)v;
+            }
         }
         // per-row energy: sum the 16 column lanes of each row group
 #pragma unroll
@@ -396,27 +399,29 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
         const int64_t ns = d.n_slots;
         // software pipeline over the list: the index two entries ahead, the
         // record and dE/dG one entry ahead (both gathers go out together,
-        // not after the distance test)
+        // not after the distance test).  A neighbour's dE/dG is gathered in
+        // BF16 (64 bytes instead of 128: the gathers bound this kernel); the
+        // particle's own stays FP32.
         int32_t jn = nn > 1 ? d.nbr[ns + s] : 0;  // index of the entry after the next
         Rec<float> on;
-        f32x4 gn[kMlpK / 4];
+        bf16x8 gn[kMlpK / 8];
         if (nn > 0) {
             const int32_t j0 = d.nbr[s];
             on = src[j0];
-            const f32x4* gp = (const f32x4*)(d.mlp_gam + (int64_t)j0 * kMlpK);
+            const bf16x8* gp = (const bf16x8*)(d.mlp_gamb + (int64_t)j0 * kMlpK);
 #pragma unroll
-            for (int q4 = 0; q4 < kMlpK / 4; ++q4) gn[q4] = gp[q4];
+            for (int q8 = 0; q8 < kMlpK / 8; ++q8) gn[q8] = gp[q8];
         }
         for (int q = 0; q < nn; ++q) {
             const Rec<float> o = on;
-            f32x4 gj[kMlpK / 4];
+            bf16x8 gj[kMlpK / 8];
 #pragma unroll
-            for (int q4 = 0; q4 < kMlpK / 4; ++q4) gj[q4] = gn[q4];
+            for (int q8 = 0; q8 < kMlpK / 8; ++q8) gj[q8] = gn[q8];
             if (q + 1 < nn) {
                 on = src[jn];
-                const f32x4* gp = (const f32x4*)(d.mlp_gam + (int64_t)jn * kMlpK);
+                const bf16x8* gp = (const bf16x8*)(d.mlp_gamb + (int64_t)jn * kMlpK);
 #pragma unroll
-                for (int q4 = 0; q4 < kMlpK / 4; ++q4) gn[q4] = gp[q4];
+                for (int q8 = 0; q8 < kMlpK / 8; ++q8) gn[q8] = gp[q8];
                 jn = q + 2 < nn ? d.nbr[(int64_t)(q + 2) * ns + s] : 0;
             }
             const double dx = min_image(o.x - a.x, bx);
@@ -433,7 +438,7 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
             gauss_basis(rr, mc, [&](int k, float e) {
                 const float tt = rr - mc.mu[k];
                 const float dg = e * fmaf(-tt * mc.inv_w2, fc, dfc);
-                cc = fmaf(gi[k] + gj[k / 4][k % 4], dg, cc);
+                cc = fmaf(gi[k] + (float)gj[k / 8][k % 8], dg, cc);
             });
             const float ir = __builtin_amdgcn_rcpf(rr);
             fx = fmaf(cc * ir, (float)dx, fx);
