@@ -319,6 +319,7 @@ def mlp_surrogate(M, n_frames=2000, epochs=40, steps=2000, log=log, prof_steps=2
         e.step(DT, prof_steps, th, ba)
         kern = {}
         for name, key in (("k_mlp_desc" if mode == "mlp" else None, "drift"),
+                          ("k_mlp_feat" if mode == "mlp" else None, "mlp_feat"),
                           ("k_mlp_net" if mode == "mlp" else None, "mlp_net"),
                           ("k_mlp_force" if mode == "mlp" else "k_step", "step"), ("k_fin", "finalize")):
             if name is None:

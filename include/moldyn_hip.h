@@ -283,8 +283,8 @@ int md_get_stats(md_ctx* ctx, md_stats* out);
 /* Per-kernel device time of the launches of md_step: the step kernel
  * (kernel 0; the pair-force kernel on the surrogate path), the finalize /
  * list-build kernel (1), the all-pairs force kernel (2), the drift kernel
- * (3; the feature kernel on the surrogate path) and the surrogate's network
- * kernel (5); kernel 4 reports nothing.
+ * (3; the first half step on the surrogate path), the surrogate's network
+ * kernel (5) and its feature kernel (6); kernel 4 reports nothing.
  * Every block records its start and end on the GPU wall clock
  * (s_memrealtime), so the captured graphs keep running unchanged.
  *   total_ms / launches: stream time, from the end of the previous kernel of

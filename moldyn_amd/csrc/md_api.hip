@@ -29,7 +29,7 @@ static thread_local std::string g_error;
 
 namespace {
 
-enum { K_STEP = 0, K_FINALIZE = 1, K_ALLPAIRS = 2, K_DRIFT = 3, K_BUILD = 4, K_MLP_NET = 5, K_NUM = 6 };
+enum { K_STEP = 0, K_FINALIZE = 1, K_ALLPAIRS = 2, K_DRIFT = 3, K_BUILD = 4, K_MLP_NET = 5, K_MLP_FEAT = 6, K_NUM = 7 };
 
 }  // namespace
 
@@ -677,7 +677,7 @@ static int prof_collect(md_ctx* c) {
         c->kspan_count[k] += 1;
     }
     // stream time: from the end of the previous recorded kernel on the stream
-    // (in launch order drift, net, step, finalize) to this kernel's end, counted
+    // (in launch order drift, features, net, step, finalize) to this kernel's end, counted
     // where the two ran back to back (the interval a dispatch trace reports)
     const unsigned long long gap = (unsigned long long)(20e-6 * c->wall_hz);
     auto rec = [&](int l, int k, unsigned long long* a, unsigned long long* b) {
@@ -691,7 +691,7 @@ static int prof_collect(md_ctx* c) {
         const int lp = (l + kProfRing - 1) % kProfRing;
         bool have_prev = rec(lp, P_FIN, &pa, &pb);
         unsigned long long prev_end = pb;
-        for (int k : {(int)P_DRIFT, (int)P_NET, (int)P_STEP, (int)P_FIN}) {
+        for (int k : {(int)P_DRIFT, (int)P_FEAT, (int)P_NET, (int)P_STEP, (int)P_FIN}) {
             if (!rec(l, k, &a, &b)) continue;
             if (have_prev && a >= prev_end && a - prev_end < gap) {
                 c->ktime[k] += (double)(b - prev_end) / c->wall_hz * 1e3;
@@ -712,7 +712,8 @@ static hipError_t launch_once(md_ctx* c) {
     hipError_t e;
     c->d.par = c->par;
     if (c->mlp) {
-        if ((e = launch_mlp_desc(c->d, c->bs, false, c->st))) return e;
+        if ((e = launch_mlp_desc(c->d, c->bs, false, 1, c->st))) return e;
+        if ((e = launch_mlp_desc(c->d, c->bs, false, 2, c->st))) return e;
         if ((e = launch_mlp_net(c->d, c->n_cu, c->st))) return e;
         if ((e = launch_mlp_force(c->d, c->bs, false, c->st))) return e;
     } else if (c->list_mode) {
@@ -877,7 +878,7 @@ static int list_force_eval(md_ctx* c) {
             HIPCHK(c, launch_insert_all(c->d, c->fp32, c->bs, c->st));
             HIPCHK(c, launch_build(c->d, c->fp32, c->st));
             if (c->mlp) {
-                HIPCHK(c, launch_mlp_desc(c->d, c->bs, true, c->st));
+                HIPCHK(c, launch_mlp_desc(c->d, c->bs, true, 0, c->st));
                 HIPCHK(c, launch_mlp_net(c->d, c->n_cu, c->st));
                 HIPCHK(c, launch_mlp_force(c->d, c->bs, true, c->st));
             } else {
@@ -2025,6 +2026,7 @@ int md_kernel_time(md_ctx* c, int kernel, double* total_ms, int64_t* launches, d
     if (kernel == K_FINALIZE) slot = P_FIN;
     if (kernel == K_DRIFT && (!c->list_mode || c->mlp)) slot = P_DRIFT;
     if (kernel == K_MLP_NET && c->mlp) slot = P_NET;
+    if (kernel == K_MLP_FEAT && c->mlp) slot = P_FEAT;
     if (total_ms) *total_ms = slot >= 0 ? c->ktime[slot] : 0.0;
     if (launches) *launches = slot >= 0 ? c->kcount[slot] : 0;
     if (span_ms) *span_ms = slot >= 0 ? c->kspan_time[slot] : 0.0;

@@ -281,12 +281,13 @@ __host__ __device__ inline TileRange block_tiles(const DevState& d, const GroupI
 // per-slot kernels store per-block {start, end, entries} with plain stores
 // (slot P_STEP for k_step / k_allpairs / k_mlp_force, P_DRIFT for k_drift /
 // k_mlp_desc) and one block of the finalize kernel folds them into the ring;
-// k_mlp_net (grid not one block per slot range) keeps {min start, max end}
-// with atomics in the first two words of its per-block area, which the fold
-// moves to the ring and resets; the finalize kernel's own blocks (one per
+// k_mlp_net (grid not one block per slot range) and the feature phase of
+// k_mlp_desc (P_FEAT) keep {min start, max end} with atomics in the first
+// two words of their per-block areas, which the fold moves to the ring and
+// resets; the finalize kernel's own blocks (one per
 // replica plus the blocks that build lists) record with atomics.
 constexpr int kProfRing = 8192;
-enum { P_STEP = 0, P_FIN = 1, P_DRIFT = 2, P_NET = 3, P_NUM = 4 };
+enum { P_STEP = 0, P_FIN = 1, P_DRIFT = 2, P_NET = 3, P_FEAT = 4, P_NUM = 5 };
 constexpr int64_t kProfCount = (int64_t)kProfRing * P_NUM * 2;  // index of the entry counter
 
 enum FinalizeMode { M_STEP = 0, M_REDUCE = 1, M_PRIME = 2 };
@@ -435,7 +436,7 @@ hipError_t launch_wrap(const DevState& d, bool fp32, int bs, hipStream_t st);
 
 // surrogate step kernels (md_mlp.hip); force_only evaluates at the current
 // positions without advancing time
-hipError_t launch_mlp_desc(const DevState& d, int bs, bool force_only, hipStream_t st);
+hipError_t launch_mlp_desc(const DevState& d, int bs, bool force_only, int phase, hipStream_t st);
 hipError_t launch_mlp_net(const DevState& d, int max_blocks, hipStream_t st);
 hipError_t launch_mlp_force(const DevState& d, int bs, bool force_only, hipStream_t st);
 hipError_t launch_mlp_convert(const DevState& d, hipStream_t st);

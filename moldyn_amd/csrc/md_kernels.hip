@@ -1610,14 +1610,16 @@ __device__ __forceinline__ void prof_fold(const DevState& d, int64_t launch, boo
         __syncthreads();
     }
     if (d.mlp_w && threadIdx.x == 0) {
-        unsigned long long* e = d.prof_blk + (int64_t)P_NET * d.n_blocks * 3;
-        if (e[0] != ~0ull) {
-            unsigned long long* r = d.prof + ((launch % kProfRing) * P_NUM + P_NET) * 2;
-            r[0] = e[0];
-            r[1] = e[1];
+        for (int k : {(int)P_NET, (int)P_FEAT}) {
+            unsigned long long* e = d.prof_blk + (int64_t)k * d.n_blocks * 3;
+            if (e[0] != ~0ull) {
+                unsigned long long* r = d.prof + ((launch % kProfRing) * P_NUM + k) * 2;
+                r[0] = e[0];
+                r[1] = e[1];
+            }
+            e[0] = ~0ull;
+            e[1] = 0ull;
         }
-        e[0] = ~0ull;
-        e[1] = 0ull;
     }
 }
 
@@ -1740,9 +1742,11 @@ __global__ __launch_bounds__(256) void k_prof_init(DevState d) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < kProfCount) d.prof[i] = (i & 1) ? 0ull : ~0ull;
     if (i == kProfCount) d.prof[i] = 0ull;
-    // the atomic {min start, max end} pair of k_mlp_net
+    // the atomic {min start, max end} pairs of k_mlp_net and the feature phase
     if (i == kProfCount + 1 && d.prof_blk) d.prof_blk[(int64_t)P_NET * d.n_blocks * 3] = ~0ull;
     if (i == kProfCount + 2 && d.prof_blk) d.prof_blk[(int64_t)P_NET * d.n_blocks * 3 + 1] = 0ull;
+    if (i == kProfCount + 3 && d.prof_blk) d.prof_blk[(int64_t)P_FEAT * d.n_blocks * 3] = ~0ull;
+    if (i == kProfCount + 4 && d.prof_blk) d.prof_blk[(int64_t)P_FEAT * d.n_blocks * 3 + 1] = 0ull;
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1846,7 +1850,7 @@ hipError_t launch_epilogue(const DevState& d, int par, RepCtrl* h_ctrl, RepStats
 }
 
 hipError_t launch_prof_init(const DevState& d, hipStream_t st) {
-    const int n = (int)kProfCount + 3;
+    const int n = (int)kProfCount + 5;
     hipLaunchKernelGGL(k_prof_init, dim3((n + 255) / 256), dim3(256), 0, st, d);
     return hipGetLastError();
 }

@@ -113,15 +113,21 @@ __device__ __forceinline__ bool mlp_store_features(const DevState& d, const MlpC
 
 // Features of every particle (FORCE_ONLY: at the current positions), and in
 // a stepping launch the first half of the velocity-Verlet step into cur ^ 1.
-template <int BS, bool FORCE_ONLY>
+// PHASE 0 does both in one pass, recomputing each neighbour's drift from its
+// input record; a stepping launch runs PHASE 1 (the first half step, records
+// into cur ^ 1) and then PHASE 2 (the features from the drifted records),
+// which gathers 24 bytes of drifted position per neighbour instead of its
+// whole input record.
+template <int BS, bool FORCE_ONLY, int PHASE>
 __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
+    constexpr bool kDrift = PHASE != 2, kFeat = PHASE != 1;
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
     const int r = G.rep;
     const RepCtrl& C = ctrl_in(d, r);
     const uint32_t flags = FORCE_ONLY ? 0u : C.flags;
     if (flags & (F_EXIT | F_BUILD)) {
-        prof_block(d, P_DRIFT, 0, false, 0);
+        if (PHASE != 2) prof_block(d, P_DRIFT, 0, false, 0);
         return;
     }
     const uint64_t t_start = prof_now(d);
@@ -134,6 +140,7 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
     const Rec<float>* __restrict__ rin = (const Rec<float>*)d.rec[cur];
     Rec<float>* __restrict__ rnx = (Rec<float>*)d.rec[nx];
     if (flags & F_NULL) {
+        if (PHASE == 2) return;
         for (int64_t t = tr.t0; t < tr.t1; ++t) {
             const int64_t s = G.start + t * BS + lane;
             if (t * BS + lane < G.count) grid_insert(d, C, R, s, rin[s].x, rin[s].y, rin[s].z);
@@ -145,6 +152,14 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
     const bool split_a = (flags & F_SPLIT_A) != 0;
     const bool split_b = (flags & F_SPLIT_B) != 0;
     const bool drift = !FORCE_ONLY && !split_b;  // this launch moves the particles
+    if (PHASE == 2 && split_a) return;            // no features before a split step's build
+    if (PHASE == 1 && !drift) {
+        prof_block(d, P_DRIFT, t_start, true, 0);
+        return;
+    }
+    // where the positions the features use are: the drifted records once
+    // PHASE 1 has written them
+    const bool from_nx = split_b || (PHASE == 2 && !FORCE_ONLY);
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double dt = P.dt, lam = C.lambda;
     const double m = G.mass;
@@ -158,15 +173,15 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
     for (int64_t t = tr.t0; t < tr.t1; ++t) {
         const int64_t s = G.start + t * BS + lane;
         if (t * BS + lane >= G.count) {
-            if (!split_a) {
+            if (kFeat && !split_a) {
                 const float z[kMlpK] = {};
                 mlp_store_features(d, mc, s, z, false);  // padding slot
             }
             continue;
         }
-        const Rec<float> a = (split_b ? rnx : rin)[s];
+        const Rec<float> a = (from_nx ? rnx : rin)[s];
         double x1 = a.x, y1 = a.y, z1 = a.z;
-        if (drift) {
+        if (kDrift && drift) {
             // integrator.rs:28-45
             const double vkx = a.vx + (double)a.fx * c;
             const double vky = a.vy + (double)a.fy * c;
@@ -194,16 +209,20 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
                 continue;
             }
         }
+        if (!kFeat) continue;
         float g[kMlpK];
-        if (drift)
+        if (PHASE == 0 && drift)
             mlp_features<true>(d, mc, rin, s, x1, y1, z1, bx, by, bz, c, lam, dt, g);
         else
-            mlp_features<false>(d, mc, split_b ? rnx : rin, s, x1, y1, z1, bx, by, bz, c, lam, dt, g);
+            mlp_features<false>(d, mc, from_nx ? rnx : rin, s, x1, y1, z1, bx, by, bz, c, lam, dt, g);
         oor |= mlp_store_features(d, mc, s, g, true);
     }
-    if (oor) atomicOr(&d.mlp_stat[r].oor, 1u);
-    if (drift) partial_store<BS>(d, part, b, (P.th_kind == 2 ? kColsKick : 0u) | kColsDrift);
-    prof_block(d, P_DRIFT, t_start, true, 0);
+    if (kFeat && oor) atomicOr(&d.mlp_stat[r].oor, 1u);
+    if (kDrift && drift) partial_store<BS>(d, part, b, (P.th_kind == 2 ? kColsKick : 0u) | kColsDrift);
+    if (PHASE == 2)
+        prof_span_atomic(d, P_FEAT, t_start);
+    else
+        prof_block(d, P_DRIFT, t_start, true, 0);
 }
 
 // ---------------------------------------------------------------- network
@@ -521,10 +540,12 @@ __global__ __launch_bounds__(256) void k_mlp_convert(DevState d) {
 
 // ---------------------------------------------------------------- launchers
 
-hipError_t launch_mlp_desc(const DevState& d, int bs, bool force_only, hipStream_t st) {
-#define L(B)                                                                                        \
-    if (force_only) hipLaunchKernelGGL((k_mlp_desc<B, true>), dim3(d.n_blocks), dim3(B), 0, st, d); \
-    else hipLaunchKernelGGL((k_mlp_desc<B, false>), dim3(d.n_blocks), dim3(B), 0, st, d);
+hipError_t launch_mlp_desc(const DevState& d, int bs, bool force_only, int phase, hipStream_t st) {
+#define L(B)                                                                                              \
+    if (force_only) hipLaunchKernelGGL((k_mlp_desc<B, true, 0>), dim3(d.n_blocks), dim3(B), 0, st, d);   \
+    else if (phase == 1) hipLaunchKernelGGL((k_mlp_desc<B, false, 1>), dim3(d.n_blocks), dim3(B), 0, st, d); \
+    else if (phase == 2) hipLaunchKernelGGL((k_mlp_desc<B, false, 2>), dim3(d.n_blocks), dim3(B), 0, st, d); \
+    else hipLaunchKernelGGL((k_mlp_desc<B, false, 0>), dim3(d.n_blocks), dim3(B), 0, st, d);
     switch (bs) {
         case 64: L(64) break;
         case 128: L(128) break;

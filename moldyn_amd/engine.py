@@ -13,7 +13,7 @@ from . import _lib as L
 
 FORCE_MODES = {"fp64_allpairs": L.MD_FORCE_FP64_ALLPAIRS, "fp64_list": L.MD_FORCE_FP64_LIST,
                "fp32_list": L.MD_FORCE_FP32_LIST, "mlp": L.MD_FORCE_MLP}
-KERNELS = {"step": 0, "finalize": 1, "allpairs": 2, "drift": 3, "mlp_net": 5}
+KERNELS = {"step": 0, "finalize": 1, "allpairs": 2, "drift": 3, "mlp_net": 5, "mlp_feat": 6}
 
 
 def _dp(a):

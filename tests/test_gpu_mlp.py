@@ -125,7 +125,7 @@ def test_mlp_kernel_timing_leaves_results_unchanged():
         outs.append(e.download(("x", "v")))
         if prof:
             launches = e.stats()["launches"]
-            for key in ("drift", "mlp_net", "step"):
+            for key in ("drift", "mlp_feat", "mlp_net", "step"):
                 ms, n = e.kernel_time(key, span=True)
                 assert 0.95 * launches <= n <= launches, key
                 assert 0.0 < ms / n < 5.0, key
