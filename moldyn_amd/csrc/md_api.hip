@@ -460,6 +460,8 @@ static int alloc_base(md_ctx* c) {
             if ((rc = dalloc(c, c->base_allocs, &p, sizeof(CRec) * (size_t)std::max<int64_t>(N, 1)))) return rc;
             d.crec[b] = (CRec*)p;
         }
+        if ((rc = dalloc(c, c->base_allocs, &p, sizeof(double4) * (size_t)std::max<int64_t>(N, 1)))) return rc;
+        d.dpos = (double4*)p;
         std::vector<uint8_t> ty((size_t)std::max<int64_t>(N, 1), 0);
         for (const GroupInfo& G : c->groups) {
             const int64_t n = (G.count + c->bs - 1) / c->bs * c->bs;
@@ -717,6 +719,7 @@ static hipError_t launch_once(md_ctx* c) {
         if ((e = launch_mlp_net(c->d, c->n_cu, c->st))) return e;
         if ((e = launch_mlp_force(c->d, c->bs, false, c->st))) return e;
     } else if (c->list_mode) {
+        if (c->d.dense && (e = launch_dpos(c->d, c->bs, c->st))) return e;
         if ((e = launch_step(c->d, c->fp32, c->bs, c->st))) return e;
     } else {
         if ((e = launch_drift(c->d, c->bs, c->st))) return e;

@@ -210,6 +210,7 @@ struct MlpStat {
 struct DevState {
     void* rec[2];            // Rec<float> or Rec<double>, n_slots each
     CRec* crec[2];           // step records of the list path (null on the other paths)
+    double4* dpos;           // dense list path: every slot's drifted position (x, y, z, unused)
     const uint8_t* slot_type;  // type of every slot (padding slots: the group's type)
     int32_t compact;         // the list path steps on crec (k_fin builds from crec)
     int32_t dense;           // long lists expected: lanes walk their own lists, builds sort long lists in LDS
@@ -411,6 +412,10 @@ constexpr uint32_t kColsAll = (1u << NS) - 1u;
 // host-side launchers (md_kernels.hip)
 hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st);
 hipError_t launch_pack(const DevState& d, bool fp32, int bs, hipStream_t st);
+// Dense list path: the drifted positions of every slot for this launch's
+// neighbour gathers (the step kernel then reads 32 bytes per list entry and
+// recomputes no drift).
+hipError_t launch_dpos(const DevState& d, int bs, hipStream_t st);
 hipError_t launch_build(const DevState& d, bool fp32, hipStream_t st);
 hipError_t launch_prof_init(const DevState& d, hipStream_t st);
 // Chunk prologue of a one-replica context in one launch: the parameters

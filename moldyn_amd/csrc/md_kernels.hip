@@ -617,9 +617,9 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
     uint8_t* pf = (uint8_t*)(omap + W);
     const int wl = threadIdx.x & (W - 1);
     Acc<FT> acc;
-    auto pair = [&](const CRec& o, int tj, double xi, double yi, double zi, Terms& tm) -> bool {
+    auto pair = [&](const CRec& o, int tj, double xi, double yi, double zi, Terms& tm, bool drift) -> bool {
         double xj = o.x, yj = o.y, zj = o.z;
-        if (!split) {
+        if (drift) {
             xj = wrap1(xj + (o.ux * lam) * dt, bx);
             yj = wrap1(yj + (o.uy * lam) * dt, by);
             zj = wrap1(zj + (o.uz * lam) * dt, bz);
@@ -634,12 +634,24 @@ __device__ __forceinline__ Acc<FT> wave_list_forces(const DevState& d, const CRe
         return true;
     };
     auto gathered = [&](int32_t j, double xi, double yi, double zi, Terms& tm) -> bool {
-        const CRec o = src[j];  // one copy: the record loads as whole 16-byte words
-        return pair(o, MULTI ? (int)d.slot_type[j] : 0, xi, yi, zi, tm);
+        if constexpr (WALK) {
+            // dense systems: the neighbour's drifted position, written by
+            // k_dpos for this launch with the operations its own thread uses
+            const double4 q = d.dpos[j];
+            CRec o;
+            o.x = q.x;
+            o.y = q.y;
+            o.z = q.z;
+            o.ux = o.uy = o.uz = 0.0;
+            return pair(o, MULTI ? (int)d.slot_type[j] : 0, xi, yi, zi, tm, false);
+        } else {
+            const CRec o = src[j];  // one copy: the record loads as whole 16-byte words
+            return pair(o, MULTI ? (int)d.slot_type[j] : 0, xi, yi, zi, tm, !split);
+        }
     };
     if (!WALK && n_mine >= 1) {
         Terms tm;
-        if (pair(o0, tj0, x1, y1, z1, tm)) acc.add_terms(tm);
+        if (pair(o0, tj0, x1, y1, z1, tm, !split)) acc.add_terms(tm);
     }
     const int n_rest = n_mine > K0 ? n_mine - K0 : 0;
     if (!WALK && !__ballot(n_rest > 0)) return acc;
@@ -1768,6 +1780,50 @@ __global__ __launch_bounds__(256) void k_prof_init(DevState d) {
         if (multi) hipLaunchKernelGGL((K<double, B, true>), dim3(d.n_blocks), dim3(B), 0, st, d); \
         else hipLaunchKernelGGL((K<double, B, false>), dim3(d.n_blocks), dim3(B), 0, st, d);      \
     }
+
+// Drifted positions of a dense launch: the step's drift of every slot
+// (integrator.rs:40-45 with the thermostat scale), computed as k_step
+// computes its own particle's, or the record's position in the second half
+// of a split step.  Padding slots keep their NaN positions.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_dpos(DevState d) {
+    const int b = blockIdx.x;
+    const int gi = d.blk_group[b];
+    const GroupInfo G = d.groups[gi];
+    const RepCtrl& C = ctrl_in(d, G.rep);
+    const uint32_t flags = C.flags;
+    if (flags & (F_EXIT | F_BUILD | F_NULL | F_SPLIT_A)) return;
+    const RepParams& P = d.params[G.rep];
+    const TileRange tr = block_tiles(d, G, b, BS);
+    const int cur = (C.cur ^ C.cflip) & 1;
+    const bool split = (flags & F_SPLIT_B) != 0;
+    const CRec* __restrict__ src = d.crec[split ? cur ^ 1 : cur];
+    const double bx = C.box[0], by = C.box[1], bz = C.box[2];
+    const double dt = P.dt, lam = C.lambda;
+    for (int64_t t = tr.t0; t < tr.t1; ++t) {
+        const int64_t s = G.start + t * BS + threadIdx.x;
+        const CRec a = src[s];
+        double4 q;
+        if (split || t * BS + (int)threadIdx.x >= G.count) {
+            q.x = a.x;
+            q.y = a.y;
+            q.z = a.z;
+        } else {
+            q.x = wrap1(a.x + (a.ux * lam) * dt, bx);
+            q.y = wrap1(a.y + (a.uy * lam) * dt, by);
+            q.z = wrap1(a.z + (a.uz * lam) * dt, bz);
+        }
+        q.w = 0.0;
+        d.dpos[s] = q;
+    }
+}
+
+hipError_t launch_dpos(const DevState& d, int bs, hipStream_t st) {
+#define L(B) hipLaunchKernelGGL((k_dpos<B>), dim3(d.n_blocks), dim3(B), 0, st, d);
+    MD_BS_SWITCH(bs, L)
+#undef L
+    return hipGetLastError();
+}
 
 hipError_t launch_step(const DevState& d, bool fp32, int bs, hipStream_t st) {
     const bool multi = d.n_types > 1;
