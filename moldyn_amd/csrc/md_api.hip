@@ -418,6 +418,8 @@ static int alloc_mlp(md_ctx* c) {
     d.mlp_gam = (float*)p;
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(__bf16) * N * kMlpK))) return rc;
     d.mlp_gamb = (__bf16*)p;
+    if ((rc = dalloc(c, c->base_allocs, &p, sizeof(float4) * N))) return rc;
+    d.mlp_pos = (float4*)p;
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(float) * N))) return rc;
     d.mlp_e = (float*)p;
     if ((rc = dalloc(c, c->base_allocs, &p, sizeof(MlpStat) * c->R))) return rc;

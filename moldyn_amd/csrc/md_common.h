@@ -19,6 +19,14 @@ __device__ __forceinline__ double min_image(double r, double b) {
     return r;
 }
 
+// FP32 minimum image of one component (the list build's prefilter, the
+// surrogate's FP32 distances).
+__device__ __forceinline__ float min_image_f(float r, float b) {
+    // selects, not branches: the candidate loop stays one basic block
+    const float h = 0.5f * b;
+    return r < -h ? r + b : (r > h ? r - b : r);
+}
+
 __device__ __forceinline__ double wrap1(double x, double b) {
     // core/src/particle.rs:124-138
     if (x < 0.0) {

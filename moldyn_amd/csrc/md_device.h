@@ -246,6 +246,7 @@ struct DevState {
     __bf16* mlp_x;                // n_slots x kMlpK normalised features
     float* mlp_gam;               // n_slots x kMlpK  dE_i / dG_i
     __bf16* mlp_gamb;             // the same in BF16: the copy neighbours gather
+    float4* mlp_pos;              // n_slots  FP32 drifted positions of the stepping launch
     float* mlp_e;                 // n_slots          E_i
     MlpStat* mlp_stat;            // n_reps
     double* mlp_judge;            // 2 x n_blocks per-block check sums

@@ -185,13 +185,6 @@ __device__ void build_list_global(const DevState& d, const RepCtrl& C, const Rep
     list_close(d, slot, n);
 }
 
-// FP32 minimum image of one component (the prefilter of the list build).
-__device__ __forceinline__ float min_image_f(float r, float b) {
-    // selects, not branches: the candidate loop stays one basic block
-    const float h = 0.5f * b;
-    return r < -h ? r + b : (r > h ? r - b : r);
-}
-
 
 // The reference's minimum image (potential.rs:186-200) as selects, for the
 // list builder's exact tests: the same values as min_image, without a branch

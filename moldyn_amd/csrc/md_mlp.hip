@@ -55,13 +55,11 @@ __device__ __forceinline__ void gauss_basis(float r, const MlpConst& mc, F&& f) 
     }
 }
 
-// Features of one particle from its neighbour list.  Positions come from
-// `pos` records; with DRIFT the neighbours' drifted positions are recomputed
-// from their input records with the operations their own threads perform.
-template <bool DRIFT>
+// Features of one particle from its neighbour list and FP64 records (the
+// force-only evaluation, at the current positions).
 __device__ __forceinline__ void mlp_features(const DevState& d, const MlpConst& mc, const Rec<float>* __restrict__ pos,
                                              int64_t s, double x1, double y1, double z1, double bx, double by,
-                                             double bz, double c, double lam, double dt, float (&g)[kMlpK]) {
+                                             double bz, float (&g)[kMlpK]) {
 #pragma unroll
     for (int k = 0; k < kMlpK; ++k) g[k] = 0.0f;
     const int nn = d.ncount[s];
@@ -78,19 +76,48 @@ __device__ __forceinline__ void mlp_features(const DevState& d, const MlpConst& 
             on = pos[jn];
             jn = q + 2 < nn ? d.nbr[(int64_t)(q + 2) * ns + s] : 0;
         }
-        double xj = o.x, yj = o.y, zj = o.z;
-        if (DRIFT) {
-            xj = wrap1(xj + ((o.vx + (double)o.fx * c) * lam) * dt, bx);
-            yj = wrap1(yj + ((o.vy + (double)o.fy * c) * lam) * dt, by);
-            zj = wrap1(zj + ((o.vz + (double)o.fz * c) * lam) * dt, bz);
-        }
-        const double dx = min_image(xj - x1, bx);
-        const double dy = min_image(yj - y1, by);
-        const double dz = min_image(zj - z1, bz);
+        const double dx = min_image(o.x - x1, bx);
+        const double dy = min_image(o.y - y1, by);
+        const double dz = min_image(o.z - z1, bz);
         const double r2 = (dx * dx + dy * dy) + dz * dz;
         if (!(r2 < rc2)) continue;
         // FP32 distance and cutoff from the hardware square root (1 ulp)
         const float r = __builtin_amdgcn_sqrtf((float)r2);
+        const float fc = 0.5f * (__cosf(r * mc.pi_rc) + 1.0f);
+        gauss_basis(r, mc, [&](int k, float e) { g[k] = fmaf(e, fc, g[k]); });
+    }
+}
+
+// The same from the FP32 copies of the drifted positions (mlp_pos) that a
+// stepping launch writes before its features: 16 bytes per neighbour
+// instead of 24 and four particles per 64-byte line instead of one.  FP32
+// coordinates (absolute error below 5e-7 nm in a 16 nm box) perturb the
+// distances by less than 1e-5 relative, well under the BF16 rounding of the
+// features and of dE/dG that follows.
+__device__ __forceinline__ void mlp_features_f32(const DevState& d, const MlpConst& mc, int64_t s, float bx, float by,
+                                                 float bz, float (&g)[kMlpK]) {
+#pragma unroll
+    for (int k = 0; k < kMlpK; ++k) g[k] = 0.0f;
+    const float4* __restrict__ pos = d.mlp_pos;
+    const float4 a = pos[s];
+    const int nn = d.ncount[s];
+    const float rc2 = mc.rc * mc.rc;
+    const int64_t ns = d.n_slots;
+    int32_t jn = nn > 1 ? d.nbr[ns + s] : 0;
+    float4 on;
+    if (nn > 0) on = pos[d.nbr[s]];
+    for (int q = 0; q < nn; ++q) {
+        const float4 o = on;
+        if (q + 1 < nn) {
+            on = pos[jn];
+            jn = q + 2 < nn ? d.nbr[(int64_t)(q + 2) * ns + s] : 0;
+        }
+        const float dx = min_image_f(o.x - a.x, bx);
+        const float dy = min_image_f(o.y - a.y, by);
+        const float dz = min_image_f(o.z - a.z, bz);
+        const float r2 = (dx * dx + dy * dy) + dz * dz;
+        if (!(r2 < rc2)) continue;
+        const float r = __builtin_amdgcn_sqrtf(r2);
         const float fc = 0.5f * (__cosf(r * mc.pi_rc) + 1.0f);
         gauss_basis(r, mc, [&](int k, float e) { g[k] = fmaf(e, fc, g[k]); });
     }
@@ -111,16 +138,17 @@ __device__ __forceinline__ bool mlp_store_features(const DevState& d, const MlpC
     return oor;
 }
 
-// Features of every particle (FORCE_ONLY: at the current positions), and in
-// a stepping launch the first half of the velocity-Verlet step into cur ^ 1.
-// PHASE 0 does both in one pass, recomputing each neighbour's drift from its
-// input record; a stepping launch runs PHASE 1 (the first half step, records
-// into cur ^ 1) and then PHASE 2 (the features from the drifted records),
-// which gathers 24 bytes of drifted position per neighbour instead of its
-// whole input record.
+// Features of every particle, and in a stepping launch the first half of
+// the velocity-Verlet step.  PHASE 0 (FORCE_ONLY) computes the features at
+// the current positions from the FP64 records.  A stepping launch runs
+// PHASE 1, the first half step into records cur ^ 1 and the FP32 position
+// copies mlp_pos (a split step's second launch, which does not move the
+// particles, only copies its drifted positions there), and then PHASE 2, the
+// features from mlp_pos.
 template <int BS, bool FORCE_ONLY, int PHASE>
 __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
-    constexpr bool kDrift = PHASE != 2, kFeat = PHASE != 1;
+    static_assert(FORCE_ONLY == (PHASE == 0), "PHASE 0 is the force-only evaluation");
+    constexpr bool kDrift = PHASE == 1, kFeat = PHASE != 1;
     const int b = blockIdx.x;
     const GroupInfo G = d.groups[d.blk_group[b]];
     const int r = G.rep;
@@ -154,12 +182,15 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
     const bool drift = !FORCE_ONLY && !split_b;  // this launch moves the particles
     if (PHASE == 2 && split_a) return;            // no features before a split step's build
     if (PHASE == 1 && !drift) {
+        // split step, second launch: its drifted records are in cur ^ 1
+        for (int64_t t = tr.t0; t < tr.t1; ++t) {
+            const int64_t s = G.start + t * BS + lane;
+            if (t * BS + lane < G.count)
+                d.mlp_pos[s] = make_float4((float)rnx[s].x, (float)rnx[s].y, (float)rnx[s].z, 0.f);
+        }
         prof_block(d, P_DRIFT, t_start, true, 0);
         return;
     }
-    // where the positions the features use are: the drifted records once
-    // PHASE 1 has written them
-    const bool from_nx = split_b || (PHASE == 2 && !FORCE_ONLY);
     const double bx = C.box[0], by = C.box[1], bz = C.box[2];
     const double dt = P.dt, lam = C.lambda;
     const double m = G.mass;
@@ -179,7 +210,13 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
             }
             continue;
         }
-        const Rec<float> a = (from_nx ? rnx : rin)[s];
+        if (PHASE == 2) {
+            float g[kMlpK];
+            mlp_features_f32(d, mc, s, (float)bx, (float)by, (float)bz, g);
+            oor |= mlp_store_features(d, mc, s, g, true);
+            continue;
+        }
+        const Rec<float> a = rin[s];
         double x1 = a.x, y1 = a.y, z1 = a.z;
         if (kDrift && drift) {
             // integrator.rs:28-45
@@ -204,6 +241,7 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
             o.vy = v1y;
             o.vz = v1z;
             rnx[s] = o;
+            d.mlp_pos[s] = make_float4((float)x1, (float)y1, (float)z1, 0.f);
             if (split_a) {
                 grid_insert(d, C, R, s, x1, y1, z1);
                 continue;
@@ -211,10 +249,7 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
         }
         if (!kFeat) continue;
         float g[kMlpK];
-        if (PHASE == 0 && drift)
-            mlp_features<true>(d, mc, rin, s, x1, y1, z1, bx, by, bz, c, lam, dt, g);
-        else
-            mlp_features<false>(d, mc, from_nx ? rnx : rin, s, x1, y1, z1, bx, by, bz, c, lam, dt, g);
+        mlp_features(d, mc, rin, s, x1, y1, z1, bx, by, bz, g);
         oor |= mlp_store_features(d, mc, s, g, true);
     }
     if (kFeat && oor) atomicOr(&d.mlp_stat[r].oor, 1u);
@@ -416,60 +451,104 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
         Acc<float> acc;  // LJ (fallback and check)
         const int nn = d.ncount[s];
         const int64_t ns = d.n_slots;
-        // software pipeline over the list: the index two entries ahead, the
-        // record and dE/dG one entry ahead (both gathers go out together,
-        // not after the distance test).  A neighbour's dE/dG is gathered in
-        // BF16 (64 bytes instead of 128: the gathers bound this kernel); the
-        // particle's own stays FP32.
-        int32_t jn = nn > 1 ? d.nbr[ns + s] : 0;  // index of the entry after the next
-        Rec<float> on;
-        bf16x8 gn[kMlpK / 8];
-        if (nn > 0) {
-            const int32_t j0 = d.nbr[s];
-            on = src[j0];
-            const bf16x8* gp = (const bf16x8*)(d.mlp_gamb + (int64_t)j0 * kMlpK);
-#pragma unroll
-            for (int q8 = 0; q8 < kMlpK / 8; ++q8) gn[q8] = gp[q8];
-        }
-        for (int q = 0; q < nn; ++q) {
-            const Rec<float> o = on;
-            bf16x8 gj[kMlpK / 8];
-#pragma unroll
-            for (int q8 = 0; q8 < kMlpK / 8; ++q8) gj[q8] = gn[q8];
-            if (q + 1 < nn) {
-                on = src[jn];
-                const bf16x8* gp = (const bf16x8*)(d.mlp_gamb + (int64_t)jn * kMlpK);
-#pragma unroll
-                for (int q8 = 0; q8 < kMlpK / 8; ++q8) gn[q8] = gp[q8];
-                jn = q + 2 < nn ? d.nbr[(int64_t)(q + 2) * ns + s] : 0;
-            }
-            const double dx = min_image(o.x - a.x, bx);
-            const double dy = min_image(o.y - a.y, by);
-            const double dz = min_image(o.z - a.z, bz);
-            const double r2 = (dx * dx + dy * dy) + dz * dz;
-            if (lj && r2 <= p0.r2max) acc.add(dx, dy, dz, r2, p0);
-            if (!(r2 < rc2)) continue;
-            const float rr = __builtin_amdgcn_sqrtf((float)r2);
+        // radial coefficient c of F_i += c d_ij / r_ij for a pair at rr < r_c:
+        // c = sum_k G_k dg_k/dr with G = dE_i/dG + dE_j/dG and
+        // dg_k/dr = e_k (-(r - mu_k) fc / w^2 + dfc/dr).  Two sums
+        // A = sum G_k e_k and B = sum G_k e_k mu_k give
+        // c = -(r A - B) fc / w^2 + A dfc/dr (three operations per k
+        // besides the basis instead of five).
+        auto coef = [&](float rr, const bf16x8 (&gj)[kMlpK / 8]) {
             const float arg = rr * mc.pi_rc;
             const float fc = 0.5f * (__cosf(arg) + 1.0f);
             const float dfc = -0.5f * mc.pi_rc * __sinf(arg);
-            // c = sum_k G_k dg_k/dr with G = dE_i/dG + dE_j/dG and
-            // dg_k/dr = e_k (-(r - mu_k) fc / w^2 + dfc/dr): two sums
-            // A = sum G_k e_k and B = sum G_k e_k mu_k give
-            // c = -(r A - B) fc / w^2 + A dfc/dr (three operations per k
-            // besides the basis instead of five)
             float sa = 0.f, sb = 0.f;
             gauss_basis(rr, mc, [&](int k, float e) {
                 const float ge = (gi[k] + (float)gj[k / 8][k % 8]) * e;
                 sa += ge;
                 sb = fmaf(ge, mc.mu[k], sb);
             });
-            const float cc = fmaf(-mc.inv_w2 * fc, fmaf(rr, sa, -sb), dfc * sa);
-            const float ir = __builtin_amdgcn_rcpf(rr);
-            fx = fmaf(cc * ir, (float)dx, fx);
-            fy = fmaf(cc * ir, (float)dy, fy);
-            fz = fmaf(cc * ir, (float)dz, fz);
-            vir = fmaf(cc, rr, vir);
+            return fmaf(-mc.inv_w2 * fc, fmaf(rr, sa, -sb), dfc * sa);
+        };
+        // software pipeline over the list: the index two entries ahead, the
+        // position and dE/dG one entry ahead (both gathers go out together,
+        // not after the distance test).  A neighbour's dE/dG is gathered in
+        // BF16 (64 bytes instead of 128: the gathers bound this kernel); the
+        // particle's own stays FP32.
+        int32_t jn = nn > 1 ? d.nbr[ns + s] : 0;  // index of the entry after the next
+        bf16x8 gn[kMlpK / 8];
+        auto gather_gam = [&](int32_t j) {
+            const bf16x8* gp = (const bf16x8*)(d.mlp_gamb + (int64_t)j * kMlpK);
+#pragma unroll
+            for (int q8 = 0; q8 < kMlpK / 8; ++q8) gn[q8] = gp[q8];
+        };
+        if (FORCE_ONLY || lj) {
+            // FP64 positions from the records: the LJ sums here are the LJ
+            // path's own, bit for bit
+            Rec<float> on;
+            if (nn > 0) {
+                const int32_t j0 = d.nbr[s];
+                on = src[j0];
+                gather_gam(j0);
+            }
+            for (int q = 0; q < nn; ++q) {
+                const Rec<float> o = on;
+                bf16x8 gj[kMlpK / 8];
+#pragma unroll
+                for (int q8 = 0; q8 < kMlpK / 8; ++q8) gj[q8] = gn[q8];
+                if (q + 1 < nn) {
+                    on = src[jn];
+                    gather_gam(jn);
+                    jn = q + 2 < nn ? d.nbr[(int64_t)(q + 2) * ns + s] : 0;
+                }
+                const double dx = min_image(o.x - a.x, bx);
+                const double dy = min_image(o.y - a.y, by);
+                const double dz = min_image(o.z - a.z, bz);
+                const double r2 = (dx * dx + dy * dy) + dz * dz;
+                if (lj && r2 <= p0.r2max) acc.add(dx, dy, dz, r2, p0);
+                if (!(r2 < rc2)) continue;
+                const float rr = __builtin_amdgcn_sqrtf((float)r2);
+                const float cc = coef(rr, gj);
+                const float ir = __builtin_amdgcn_rcpf(rr);
+                fx = fmaf(cc * ir, (float)dx, fx);
+                fy = fmaf(cc * ir, (float)dy, fy);
+                fz = fmaf(cc * ir, (float)dz, fz);
+                vir = fmaf(cc, rr, vir);
+            }
+        } else {
+            // surrogate-only steps: the FP32 drifted positions the features
+            // used (16 bytes per neighbour)
+            const float4* __restrict__ pos = d.mlp_pos;
+            const float4 pa = pos[s];
+            const float bxf = (float)bx, byf = (float)by, bzf = (float)bz, rc2f = mc.rc * mc.rc;
+            float4 on;
+            if (nn > 0) {
+                const int32_t j0 = d.nbr[s];
+                on = pos[j0];
+                gather_gam(j0);
+            }
+            for (int q = 0; q < nn; ++q) {
+                const float4 o = on;
+                bf16x8 gj[kMlpK / 8];
+#pragma unroll
+                for (int q8 = 0; q8 < kMlpK / 8; ++q8) gj[q8] = gn[q8];
+                if (q + 1 < nn) {
+                    on = pos[jn];
+                    gather_gam(jn);
+                    jn = q + 2 < nn ? d.nbr[(int64_t)(q + 2) * ns + s] : 0;
+                }
+                const float dx = min_image_f(o.x - pa.x, bxf);
+                const float dy = min_image_f(o.y - pa.y, byf);
+                const float dz = min_image_f(o.z - pa.z, bzf);
+                const float r2 = (dx * dx + dy * dy) + dz * dz;
+                if (!(r2 < rc2f)) continue;
+                const float rr = __builtin_amdgcn_sqrtf(r2);
+                const float cc = coef(rr, gj);
+                const float ir = __builtin_amdgcn_rcpf(rr);
+                fx = fmaf(cc * ir, dx, fx);
+                fy = fmaf(cc * ir, dy, fy);
+                fz = fmaf(cc * ir, dz, fz);
+                vir = fmaf(cc, rr, vir);
+            }
         }
         float pot = 2.0f * d.mlp_e[s];  // Particle::potential counts each energy twice
         if (check) {
@@ -551,7 +630,7 @@ hipError_t launch_mlp_desc(const DevState& d, int bs, bool force_only, int phase
     if (force_only) hipLaunchKernelGGL((k_mlp_desc<B, true, 0>), dim3(d.n_blocks), dim3(B), 0, st, d);   \
     else if (phase == 1) hipLaunchKernelGGL((k_mlp_desc<B, false, 1>), dim3(d.n_blocks), dim3(B), 0, st, d); \
     else if (phase == 2) hipLaunchKernelGGL((k_mlp_desc<B, false, 2>), dim3(d.n_blocks), dim3(B), 0, st, d); \
-    else hipLaunchKernelGGL((k_mlp_desc<B, false, 0>), dim3(d.n_blocks), dim3(B), 0, st, d);
+    else return hipErrorInvalidValue;
     switch (bs) {
         case 64: L(64) break;
         case 128: L(128) break;
