@@ -784,25 +784,29 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
     Rec<FT>* __restrict__ full = (Rec<FT>*)d.rec[P.full_dst & 1];
     const PairParam p0 = wave_uniform(d.pairs[0]);
     int visits = 0;  // list entries read by this thread (profiling)
-    // A tile's records move between HBM and LDS as whole 16-byte words with
-    // consecutive lanes on consecutive words (fully coalesced); each thread
-    // then reads and writes its own record in LDS.  The next tile's words
-    // load into registers while the current tile is processed.
+    // A wave's 64 records move between HBM and LDS as whole 16-byte words
+    // with consecutive lanes on consecutive words (fully coalesced); each
+    // lane then reads and writes its own record in the wave's part of the
+    // LDS tile, so only the wave itself orders these accesses (no block
+    // barrier).  The next tile's words load into registers while the current
+    // tile is processed.
     typedef unsigned int u4 __attribute__((ext_vector_type(4)));
     constexpr int RQ = (int)(sizeof(CRec) / 16);
     constexpr int WS = wave_scratch_bytes<FT>();
     __shared__ u4 s_tile[BS * RQ];
     __shared__ u4 s_ws[(BS / 64) * WS / 16];
     char* const wreg = (char*)s_ws + (threadIdx.x >> 6) * WS;
+    const int wl = lane & 63, w64 = lane & ~63;  // lane in the wave, the wave's first lane
+    u4* const wtile = s_tile + w64 * RQ;          // the wave's 64 records
     u4 pre[RQ];
     int nn = 0;
     // the step record (and type) of the first list entry of the thread's
     // particle, gathered a tile ahead (entry 0 of every slot exists)
     int32_t e0 = 0;
     if (tr.t0 < tr.t1) {
-        const u4* g = (const u4*)(src + G.start + tr.t0 * BS);
+        const u4* g = (const u4*)(src + G.start + tr.t0 * BS + w64);
 #pragma unroll
-        for (int q = 0; q < RQ; ++q) pre[q] = g[q * BS + lane];
+        for (int q = 0; q < RQ; ++q) pre[q] = g[q * 64 + wl];
         nn = tr.t0 * BS + lane < G.count ? d.ncount[base + tr.t0 * BS] : 0;
         e0 = d.nbr[base + tr.t0 * BS];
     }
@@ -821,15 +825,15 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
             }
         }
 #pragma unroll
-        for (int q = 0; q < RQ; ++q) s_tile[q * BS + lane] = pre[q];
-        lds_barrier();  // stores of the previous tile may still be in flight
-        const CRec a = ((const CRec*)s_tile)[lane];
+        for (int q = 0; q < RQ; ++q) wtile[q * 64 + wl] = pre[q];
+        wave_lds_sync();
+        const CRec a = ((const CRec*)wtile)[wl];
         int nnn = 0;
         int32_t ne0 = 0;
         if (t + 1 < tr.t1) {
-            const u4* g = (const u4*)(src + G.start + (t + 1) * BS);
+            const u4* g = (const u4*)(src + G.start + (t + 1) * BS + w64);
 #pragma unroll
-            for (int q = 0; q < RQ; ++q) pre[q] = g[q * BS + lane];
+            for (int q = 0; q < RQ; ++q) pre[q] = g[q * 64 + wl];
             nnn = (t + 1) * BS + lane < G.count ? d.ncount[s + BS] : 0;
             ne0 = d.nbr[s + BS];
         }
@@ -915,14 +919,15 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevState d) {
             full[s] = o;
         }
         visits += valid ? nn : 0;
-        ((CRec*)s_tile)[lane] = res;
-        lds_barrier();
+        wave_lds_sync();  // every lane has read its input record
+        ((CRec*)wtile)[wl] = res;
+        wave_lds_sync();
         {
-            u4* g = (u4*)(dst + G.start + t * BS);
+            u4* g = (u4*)(dst + G.start + t * BS + w64);
 #pragma unroll
-            for (int q = 0; q < RQ; ++q) g[q * BS + lane] = s_tile[q * BS + lane];
+            for (int q = 0; q < RQ; ++q) g[q * 64 + wl] = wtile[q * 64 + wl];
         }
-        lds_barrier();  // s_tile is read out before the next tile overwrites it
+        wave_lds_sync();  // the tile is read out before the next one overwrites it
         nn = nnn;
         e0 = ne0;
     }
