@@ -402,6 +402,20 @@ def counter_evidence():
             "median_us": dur, "fp32_TFLOPs": round(f32 / 1e12, 3), "fp64_TFLOPs": round(f64 / 1e12, 3),
             "fp32_frac_of_157.3": round(f32 / 157.3e12, 4), "fp64_frac_of_78.6": round(f64 / 78.6e12, 4),
             "valu_wave_insts": v.get("SQ_INSTS_VALU"), "source": "profiles/k_step_262k_liquid_counters.txt"}
+    c = read_counters(os.path.join(prof, "k_step_1M_npt_counters.txt"))
+    if c and c[0].get("SQ_WAVE_CYCLES"):
+        v = c[0]
+        wc = v["SQ_WAVE_CYCLES"]
+        tiles = 1_000_000 / 64  # wave-tiles of 64 atoms per launch
+        out["integration_k_step_1M_sq"] = {
+            "wait_any_frac": round(v.get("SQ_WAIT_ANY", 0.0) / wc, 4),
+            "wait_inst_frac": round(v.get("SQ_WAIT_INST_ANY", 0.0) / wc, 4),
+            "valu_active_frac": round(v.get("SQ_ACTIVE_INST_VALU", 0.0) / wc, 4),
+            "valu_insts_per_wave_tile": round(v.get("SQ_INSTS_VALU", 0.0) / tiles, 1),
+            "salu_insts_per_wave_tile": round(v.get("SQ_INSTS_SALU", 0.0) / tiles, 1),
+            "vmem_rd_per_wave_tile": round(v.get("SQ_INSTS_VMEM_RD", 0.0) / tiles, 2),
+            "vmem_wr_per_wave_tile": round(v.get("SQ_INSTS_VMEM_WR", 0.0) / tiles, 2),
+            "source": "profiles/k_step_1M_npt_counters.txt"}
     return out or None
 
 

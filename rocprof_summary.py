@@ -190,9 +190,12 @@ def counters(args):
     run of the same program when --trace names one."""
     pat = re.compile(args.kernel)
     per = {}
-    files = sorted(glob.glob(os.path.join(args.dir, "**", "*counter_collection.csv"), recursive=True))
-    if not files:
-        raise SystemExit(f"no counter_collection.csv under {args.dir}")
+    files = []
+    for d in args.dir:
+        found = sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True))
+        if not found:
+            raise SystemExit(f"no counter_collection.csv under {d}")
+        files += found
     for f in files:
         with open(f, newline="") as fh:
             for r in csv.DictReader(fh):
@@ -201,8 +204,9 @@ def counters(args):
                 per.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
     if not per:
         raise SystemExit(f"no dispatch of a kernel matching {args.kernel!r}")
-    lines = [f"# rocprofv3 --pmc counters: {args.title}", "", "command:", "",
-             f"    {args.command}", "", f"kernel regex: {args.kernel}", "",
+    lines = [f"# rocprofv3 --pmc counters: {args.title}", "", "commands:" if len(args.command) > 1 else "command:", ""]
+    lines += [f"    {c}" for c in args.command]
+    lines += ["", f"kernel regex: {args.kernel}", "",
              f"{'counter':34s} {'dispatches':>10s} {'median':>16s} {'p10':>16s} {'p90':>16s}"]
     for name in sorted(per):
         vs = sorted(per[name])
@@ -320,9 +324,10 @@ def main(argv=None):
                    help="only dispatches whose FETCH_SIZE exceeds this many KiB")
     t.add_argument("--out")
     c = sub.add_parser("counters", help="per-dispatch medians of SQ counters and the dispatch durations")
-    c.add_argument("dir")
+    c.add_argument("dir", nargs="+", help="--pmc output directories (one per counter pass)")
     c.add_argument("--kernel", required=True)
-    c.add_argument("--command", required=True)
+    c.add_argument("--command", required=True, action="append",
+                   help="the command of each pass (repeat once per pass)")
     c.add_argument("--title", default="")
     c.add_argument("--trace", help="kernel-trace directory of the same program, for durations")
     c.add_argument("--trace-command")
