@@ -356,6 +356,8 @@ static int alloc_lists(md_ctx* c) {
     // theirs, so a later upload still starts past every epoch in use
     void* p;
     int rc;
+    c->d.mlp_nbr = nullptr;
+    c->d.mlp_ncount = nullptr;
     if (!c->list_mode) {
         c->d.nbr = nullptr;
         c->d.ncount = nullptr;
@@ -369,6 +371,13 @@ static int alloc_lists(md_ctx* c) {
     if ((rc = dalloc(c, c->list_allocs, &p, sizeof(int32_t) * (size_t)N))) return rc;
     c->d.ncount = (int32_t*)p;
     HIPCHK(c, hipMemsetAsync(c->d.ncount, 0, sizeof(int32_t) * (size_t)N, c->st));
+    if (c->mlp) {
+        // the surrogate's per-step lists of the entries within its cutoff
+        if ((rc = dalloc(c, c->list_allocs, &p, sizeof(int32_t) * (size_t)N * c->d.list_cap))) return rc;
+        c->d.mlp_nbr = (int32_t*)p;
+        if ((rc = dalloc(c, c->list_allocs, &p, sizeof(int32_t) * (size_t)N))) return rc;
+        c->d.mlp_ncount = (int32_t*)p;
+    }
     for (int k = 0; k < 2; ++k) {
         if ((rc = dalloc(c, c->list_allocs, &p, sizeof(unsigned long long) * (size_t)c->total_cells)))
             return rc;

@@ -247,6 +247,8 @@ struct DevState {
     float* mlp_gam;               // n_slots x kMlpK  dE_i / dG_i
     __bf16* mlp_gamb;             // the same in BF16: the copy neighbours gather
     float4* mlp_pos;              // n_slots  FP32 drifted positions of the stepping launch
+    int32_t* mlp_nbr;             // list_cap x n_slots  the list entries within r_c (column-major, as nbr)
+    int32_t* mlp_ncount;          // n_slots  their number
     float* mlp_e;                 // n_slots          E_i
     MlpStat* mlp_stat;            // n_reps
     double* mlp_judge;            // 2 x n_blocks per-block check sums

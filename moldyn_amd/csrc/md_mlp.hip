@@ -23,8 +23,6 @@ namespace md {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr float kPi = 3.14159265358979323846f;
-
 __device__ __forceinline__ float silu_f(float a, float& ds) {
     // the hardware reciprocal (1 ulp): the activations are rounded to BF16
     const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-a));
@@ -93,7 +91,8 @@ __device__ __forceinline__ void mlp_features(const DevState& d, const MlpConst& 
 // instead of 24 and four particles per 64-byte line instead of one.  FP32
 // coordinates (absolute error below 5e-7 nm in a 16 nm box) perturb the
 // distances by less than 1e-5 relative, well under the BF16 rounding of the
-// features and of dE/dG that follows.
+// features and of dE/dG that follows.  The entries within r_c, in list
+// order, go to mlp_nbr / mlp_ncount for the force kernel.
 __device__ __forceinline__ void mlp_features_f32(const DevState& d, const MlpConst& mc, int64_t s, float bx, float by,
                                                  float bz, float (&g)[kMlpK]) {
 #pragma unroll
@@ -103,12 +102,19 @@ __device__ __forceinline__ void mlp_features_f32(const DevState& d, const MlpCon
     const int nn = d.ncount[s];
     const float rc2 = mc.rc * mc.rc;
     const int64_t ns = d.n_slots;
-    int32_t jn = nn > 1 ? d.nbr[ns + s] : 0;
+    int32_t jn = nn > 1 ? d.nbr[ns + s] : 0;  // index of the entry after the next
+    int32_t j1 = 0;                           // index of the next entry (its position is in on)
     float4 on;
-    if (nn > 0) on = pos[d.nbr[s]];
+    if (nn > 0) {
+        j1 = d.nbr[s];
+        on = pos[j1];
+    }
+    int kept = 0;
     for (int q = 0; q < nn; ++q) {
         const float4 o = on;
+        const int32_t j = j1;
         if (q + 1 < nn) {
+            j1 = jn;
             on = pos[jn];
             jn = q + 2 < nn ? d.nbr[(int64_t)(q + 2) * ns + s] : 0;
         }
@@ -117,10 +123,12 @@ __device__ __forceinline__ void mlp_features_f32(const DevState& d, const MlpCon
         const float dz = min_image_f(o.z - a.z, bz);
         const float r2 = (dx * dx + dy * dy) + dz * dz;
         if (!(r2 < rc2)) continue;
+        d.mlp_nbr[(int64_t)kept++ * ns + s] = j;
         const float r = __builtin_amdgcn_sqrtf(r2);
         const float fc = 0.5f * (__cosf(r * mc.pi_rc) + 1.0f);
         gauss_basis(r, mc, [&](int k, float e) { g[k] = fmaf(e, fc, g[k]); });
     }
+    d.mlp_ncount[s] = kept;
 }
 
 __device__ __forceinline__ bool mlp_store_features(const DevState& d, const MlpConst& mc, int64_t s,
@@ -474,7 +482,7 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
         // not after the distance test).  A neighbour's dE/dG is gathered in
         // BF16 (64 bytes instead of 128: the gathers bound this kernel); the
         // particle's own stays FP32.
-        int32_t jn = nn > 1 ? d.nbr[ns + s] : 0;  // index of the entry after the next
+        int32_t jn;  // index of the entry after the next
         bf16x8 gn[kMlpK / 8];
         auto gather_gam = [&](int32_t j) {
             const bf16x8* gp = (const bf16x8*)(d.mlp_gamb + (int64_t)j * kMlpK);
@@ -484,6 +492,7 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
         if (FORCE_ONLY || lj) {
             // FP64 positions from the records: the LJ sums here are the LJ
             // path's own, bit for bit
+            jn = nn > 1 ? d.nbr[ns + s] : 0;
             Rec<float> on;
             if (nn > 0) {
                 const int32_t j0 = d.nbr[s];
@@ -515,26 +524,31 @@ __global__ __launch_bounds__(BS) void k_mlp_force(DevState d) {
                 vir = fmaf(cc, rr, vir);
             }
         } else {
-            // surrogate-only steps: the FP32 drifted positions the features
-            // used (16 bytes per neighbour)
+            // surrogate-only steps: the entries within r_c that the features
+            // kernel kept (in list order, so the sums are those of the whole
+            // list), with the FP32 drifted positions it used (16 bytes per
+            // neighbour)
             const float4* __restrict__ pos = d.mlp_pos;
+            const int32_t* __restrict__ nb = d.mlp_nbr;
             const float4 pa = pos[s];
             const float bxf = (float)bx, byf = (float)by, bzf = (float)bz, rc2f = mc.rc * mc.rc;
+            const int nc = d.mlp_ncount[s];
+            jn = nc > 1 ? nb[ns + s] : 0;
             float4 on;
-            if (nn > 0) {
-                const int32_t j0 = d.nbr[s];
+            if (nc > 0) {
+                const int32_t j0 = nb[s];
                 on = pos[j0];
                 gather_gam(j0);
             }
-            for (int q = 0; q < nn; ++q) {
+            for (int q = 0; q < nc; ++q) {
                 const float4 o = on;
                 bf16x8 gj[kMlpK / 8];
 #pragma unroll
                 for (int q8 = 0; q8 < kMlpK / 8; ++q8) gj[q8] = gn[q8];
-                if (q + 1 < nn) {
+                if (q + 1 < nc) {
                     on = pos[jn];
                     gather_gam(jn);
-                    jn = q + 2 < nn ? d.nbr[(int64_t)(q + 2) * ns + s] : 0;
+                    jn = q + 2 < nc ? nb[(int64_t)(q + 2) * ns + s] : 0;
                 }
                 const float dx = min_image_f(o.x - pa.x, bxf);
                 const float dy = min_image_f(o.y - pa.y, byf);
