@@ -317,10 +317,16 @@ __global__ __launch_bounds__(kNetBS) void k_mlp_net(DevState d) {
     const int lr = l & 15, lq = l >> 4;  // fragment row / column index, k group
     __bf16* H = sH[wv];
     const int64_t nrow = d.n_slots / 16;  // slots come in whole tiles of >= 64
-    for (int64_t rb = (int64_t)blockIdx.x * kNetWaves + wv; rb < nrow; rb += (int64_t)gridDim.x * kNetWaves) {
+    const int64_t rstep = (int64_t)gridDim.x * kNetWaves;
+    // the features of the wave's next row tile load while this one runs
+    int64_t rb = (int64_t)blockIdx.x * kNetWaves + wv;
+    bf16x8 axn = {};
+    if (rb < nrow) axn = *(const bf16x8*)(d.mlp_x + (rb * 16 + lr) * kMlpK + lq * 8);
+    for (; rb < nrow; rb += rstep) {
         const int64_t r0 = rb * 16;
         // a1 = x W1^T + b1 (K = 32: one MFMA per 16 outputs)
-        const bf16x8 ax = *(const bf16x8*)(d.mlp_x + (r0 + lr) * kMlpK + lq * 8);
+        const bf16x8 ax = axn;
+        if (rb + rstep < nrow) axn = *(const bf16x8*)(d.mlp_x + ((rb + rstep) * 16 + lr) * kMlpK + lq * 8);
         float s1[8][4];  // silu'(a1) in D layout
 #pragma unroll
         for (int nt = 0; nt < 8; ++nt) {
