@@ -284,14 +284,32 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+// The network runs transposed: every layer computes (units x atoms) =
+// W (units x k) . act^T (k x atoms), weights as the A operand from LDS and
+// the activations as the B operand in registers.  A layer's D tile then
+// leaves lane l holding atom l % 16 and units 16 t + 4 g + i (g = l / 16,
+// i = 0..3), and the B operand of the next layer wants, for a K chunk c of
+// 32 units, atom l % 16 and 8 units at K slots 8 g .. 8 g + 7.  Taking those
+// 8 units from the lane's own values of tiles 2c and 2c + 1,
+//   K slot 8 g + j  <->  unit 32 c + (j < 4 ? 4 g + j : 16 + 4 g + j - 4),
+// keeps every activation in its lane (no LDS round trip between layers); the
+// weights' K columns are staged in LDS in the same order, so each MFMA sums
+// the same products.
+__device__ __forceinline__ int net_perm(int k) {
+    const int c = k & ~31, g = (k >> 3) & 3, j = k & 7;
+    return c + (j < 4 ? 4 * g + j : 16 + 4 * g + j - 4);
+}
+
 // Forward and backward pass of 16 particles per wave and iteration.
 __global__ __launch_bounds__(kNetBS) void k_mlp_net(DevState d) {
-    __shared__ __attribute__((aligned(16))) __bf16 sW1[kMlpH * kStrK];   // W1 [n][k]: B of a1 = x W1^T
-    __shared__ __attribute__((aligned(16))) __bf16 sW1t[kMlpK * kStrH];  // W1^T [n][k]: B of gx = d1 W1
-    __shared__ __attribute__((aligned(16))) __bf16 sW2[kMlpH * kStrH];   // W2 [n][k]: B of a2 = h1 W2^T
-    __shared__ __attribute__((aligned(16))) __bf16 sW2t[kMlpH * kStrH];  // W2^T [n][k]: B of t = d2 W2
-    __shared__ __attribute__((aligned(16))) __bf16 sH[kNetWaves][16 * kStrH];
-    __shared__ float sb1[kMlpH], sb2[kMlpH], sw3[kMlpH];
+    __shared__ __attribute__((aligned(16))) __bf16 sW1[kMlpH * kStrK];   // W1 [n][k]: A of a1^T = W1 x^T
+    __shared__ __attribute__((aligned(16))) __bf16 sW1t[kMlpK * kStrH];  // W1^T [k][perm n]: A of gx^T = W1^T d1^T
+    __shared__ __attribute__((aligned(16))) __bf16 sW2[kMlpH * kStrH];   // W2 [m][perm n]: A of a2^T = W2 h1^T
+    __shared__ __attribute__((aligned(16))) __bf16 sW2t[kMlpH * kStrH];  // W2^T [n][perm m]: A of t^T = W2^T d2^T
+    __shared__ __attribute__((aligned(16))) float sb1[kMlpH];
+    __shared__ __attribute__((aligned(16))) float sb2[kMlpH];
+    __shared__ __attribute__((aligned(16))) float sw3[kMlpH];
+    __shared__ __attribute__((aligned(16))) float sgs[kMlpK];  // e_scale * inv_std
     const uint64_t t_start = prof_now(d);
     const MlpConst& mc = *d.mlp_c;
     const __bf16* w = d.mlp_w;
@@ -303,103 +321,113 @@ __global__ __launch_bounds__(kNetBS) void k_mlp_net(DevState d) {
             *(bf16x8*)(dst + r * stride + c8 * 8) = *(const bf16x8*)(src + r * cols + c8 * 8);
         }
     };
+    // the same with the K columns (128 hidden units) in net_perm order, 8
+    // bytes (4 units) at a time
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+    auto stage_perm = [&](__bf16* dst, const __bf16* src, int rows) {
+        constexpr int per = kMlpH / 4;
+        for (int i = threadIdx.x; i < rows * per; i += kNetBS) {
+            const int r = i / per, k4 = (i % per) * 4;
+            *(bf16x4*)(dst + r * kStrH + k4) = *(const bf16x4*)(src + r * kMlpH + net_perm(k4));
+        }
+    };
     stage(sW1, kStrK, w + kMlpBfW1, kMlpH, kMlpK);
-    stage(sW1t, kStrH, w + kMlpBfW1t, kMlpK, kMlpH);
-    stage(sW2, kStrH, w + kMlpBfW2, kMlpH, kMlpH);
-    stage(sW2t, kStrH, w + kMlpBfW2t, kMlpH, kMlpH);
+    stage_perm(sW1t, w + kMlpBfW1t, kMlpK);
+    stage_perm(sW2, w + kMlpBfW2, kMlpH);
+    stage_perm(sW2t, w + kMlpBfW2t, kMlpH);
     for (int i = threadIdx.x; i < kMlpH; i += kNetBS) {
         sb1[i] = d.mlp_theta[kMlpOffB1 + i];
         sb2[i] = d.mlp_theta[kMlpOffB2 + i];
         sw3[i] = d.mlp_theta[kMlpOffW3 + i];
     }
+    for (int i = threadIdx.x; i < kMlpK; i += kNetBS) sgs[i] = mc.e_scale * mc.g_inv_std[i];
     __syncthreads();
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int lr = l & 15, lq = l >> 4;  // fragment row / column index, k group
-    __bf16* H = sH[wv];
+    const int lr = l & 15, g = l >> 4;  // the lane's atom in the tile, its unit group
     const int64_t nrow = d.n_slots / 16;  // slots come in whole tiles of >= 64
     const int64_t rstep = (int64_t)gridDim.x * kNetWaves;
     // the features of the wave's next row tile load while this one runs
     int64_t rb = (int64_t)blockIdx.x * kNetWaves + wv;
     bf16x8 axn = {};
-    if (rb < nrow) axn = *(const bf16x8*)(d.mlp_x + (rb * 16 + lr) * kMlpK + lq * 8);
+    if (rb < nrow) axn = *(const bf16x8*)(d.mlp_x + (rb * 16 + lr) * kMlpK + g * 8);
     for (; rb < nrow; rb += rstep) {
+        // the weight fragments are read from LDS where they are used: kept
+        // in registers across iterations they would need 320 VGPRs
+        __asm__ volatile("" ::: "memory");
         const int64_t r0 = rb * 16;
-        // a1 = x W1^T + b1 (K = 32: one MFMA per 16 outputs)
+        // a1^T = W1 x^T + b1 (K = 32 features: one MFMA per 16 units); the
+        // lane's B operand is its atom's features 8 g .. 8 g + 7
         const bf16x8 ax = axn;
-        if (rb + rstep < nrow) axn = *(const bf16x8*)(d.mlp_x + ((rb + rstep) * 16 + lr) * kMlpK + lq * 8);
-        float s1[8][4];  // silu'(a1) in D layout
+        if (rb + rstep < nrow) axn = *(const bf16x8*)(d.mlp_x + ((rb + rstep) * 16 + lr) * kMlpK + g * 8);
+        float s1[8][4];  // silu'(a1) of units 16 t + 4 g + i
+        bf16x8 h1[4];    // h1 as the B operand of K chunk c
 #pragma unroll
-        for (int nt = 0; nt < 8; ++nt) {
-            const bf16x8 bw = *(const bf16x8*)(sW1 + (nt * 16 + lr) * kStrK + lq * 8);
+        for (int t = 0; t < 8; ++t) {
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = mfma(ax, bw, acc);
-            const int col = nt * 16 + lr;
+            acc = mfma(*(const bf16x8*)(sW1 + (t * 16 + lr) * kStrK + g * 8), ax, acc);
+            const f32x4 bb = *(const f32x4*)(sb1 + t * 16 + 4 * g);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 float ds;
-                const float h = silu_f(acc[i] + sb1[col], ds);
-                s1[nt][i] = ds;
-                H[(lq * 4 + i) * kStrH + col] = (__bf16)h;
+                const float h = silu_f(acc[i] + bb[i], ds);
+                s1[t][i] = ds;
+                h1[t / 2][(t & 1) * 4 + i] = (__bf16)h;
             }
         }
-        // a2 = h1 W2^T + b2, then y = w3 . h2 and d2 = w3 * silu'(a2)
-        bf16x8 ah[4];
+        // a2^T = W2 h1^T + b2, then y = w3 . h2 and d2 = w3 * silu'(a2)
+        float y = 0.f;
+        bf16x8 d2[4];
 #pragma unroll
-        for (int kc = 0; kc < 4; ++kc) ah[kc] = *(const bf16x8*)(H + lr * kStrH + kc * 32 + lq * 8);
-        float y[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int nt = 0; nt < 8; ++nt) {
+        for (int t = 0; t < 8; ++t) {
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kc = 0; kc < 4; ++kc)
-                acc = mfma(ah[kc], *(const bf16x8*)(sW2 + (nt * 16 + lr) * kStrH + kc * 32 + lq * 8), acc);
-            const int col = nt * 16 + lr;
+            for (int c = 0; c < 4; ++c)
+                acc = mfma(*(const bf16x8*)(sW2 + (t * 16 + lr) * kStrH + c * 32 + g * 8), h1[c], acc);
+            const f32x4 bb = *(const f32x4*)(sb2 + t * 16 + 4 * g);
+            const f32x4 ww = *(const f32x4*)(sw3 + t * 16 + 4 * g);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 float ds;
-                const float h = silu_f(acc[i] + sb2[col], ds);
-                y[i] = fmaf(sw3[col], h, y[i]);
-                H[(lq * 4 + i) * kStrH + col] = (__bf16)(sw3[col] * ds);
+                const float h = silu_f(acc[i] + bb[i], ds);
+                y = fmaf(ww[i], h, y);
+                d2[t / 2][(t & 1) * 4 + i] = (__bf16)(ww[i] * ds);
             }
         }
-        // t = d2 W2, d1 = t * silu'(a1)
+        // t^T = W2^T d2^T, d1 = t * silu'(a1)
+        bf16x8 d1[4];
 #pragma unroll
-        for (int kc = 0; kc < 4; ++kc) ah[kc] = *(const bf16x8*)(H + lr * kStrH + kc * 32 + lq * 8);
-#pragma unroll
-        for (int nt = 0; nt < 8; ++nt) {
+        for (int t = 0; t < 8; ++t) {
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kc = 0; kc < 4; ++kc)
-                acc = mfma(ah[kc], *(const bf16x8*)(sW2t + (nt * 16 + lr) * kStrH + kc * 32 + lq * 8), acc);
-            const int col = nt * 16 + lr;
+            for (int c = 0; c < 4; ++c)
+                acc = mfma(*(const bf16x8*)(sW2t + (t * 16 + lr) * kStrH + c * 32 + g * 8), d2[c], acc);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) H[(lq * 4 + i) * kStrH + col] = (__bf16)(acc[i] * s1[nt][i]);
+            for (int i = 0; i < 4; ++i) d1[t / 2][(t & 1) * 4 + i] = (__bf16)(acc[i] * s1[t][i]);
         }
-        // gx = d1 W1; dE/dG = e_scale * inv_std * gx
+        // gx^T = W1^T d1^T; dE/dG = e_scale * inv_std * gx: the lane holds
+        // features 16 t + 4 g .. + 3 of its atom
+        const int64_t row = (r0 + lr) * kMlpK;
 #pragma unroll
-        for (int kc = 0; kc < 4; ++kc) ah[kc] = *(const bf16x8*)(H + lr * kStrH + kc * 32 + lq * 8);
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
+        for (int t = 0; t < 2; ++t) {
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kc = 0; kc < 4; ++kc)
-                acc = mfma(ah[kc], *(const bf16x8*)(sW1t + (nt * 16 + lr) * kStrH + kc * 32 + lq * 8), acc);
-            const int col = nt * 16 + lr;
+            for (int c = 0; c < 4; ++c)
+                acc = mfma(*(const bf16x8*)(sW1t + (t * 16 + lr) * kStrH + c * 32 + g * 8), d1[c], acc);
+            const f32x4 gs = *(const f32x4*)(sgs + t * 16 + 4 * g);
+            f32x4 v;
+            bf16x4 vb;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float v = mc.e_scale * mc.g_inv_std[col] * acc[i];
-                d.mlp_gam[(r0 + lq * 4 + i) * kMlpK + col] = v;
-                d.mlp_gamb[(r0 + lq * 4 + i) * kMlpK + col] = (__bf16)v;
+                v[i] = gs[i] * acc[i];
+                vb[i] = (__bf16)v[i];
             }
+            *(f32x4*)(d.mlp_gam + row + t * 16 + 4 * g) = v;
+            *(bf16x4*)(d.mlp_gamb + row + t * 16 + 4 * g) = vb;
         }
-        // per-row energy: sum the 16 column lanes of each row group
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            float v = y[i];
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
-            if (lr == 0) d.mlp_e[r0 + lq * 4 + i] = mc.e_scale * (v + mc.b3) + mc.e_shift;
-        }
+        // per-atom energy: the four unit groups of each atom
+        y += __shfl_xor(y, 16, 64);
+        y += __shfl_xor(y, 32, 64);
+        if (g == 0) d.mlp_e[r0 + lr] = mc.e_scale * (y + mc.b3) + mc.e_shift;
     }
     prof_span_atomic(d, P_NET, t_start);
 }
