@@ -5,13 +5,16 @@
 // cosine cutoff f_c) and MLP = 32-128-128-1 with SiLU.  Forces are the exact
 // negative gradient of sum_i E_i:
 //   F_i = sum_j (c_i(r_ij) + c_j(r_ij)) d_ij / r_ij,  c_i(r) = sum_k dE_i/dG_ik dg_k/dr.
-// A surrogate step launch is four kernels:
-//   k_mlp_desc   half kick, thermostat scale, drift, wrap (integrator.rs:28-45)
-//                into buffer cur ^ 1 and the features of the drifted positions
-//                (neighbour drift recomputed from the input records);
-//   k_mlp_net    per 16-atom row block of a wave: the forward pass and the
+// A surrogate step launch is five kernels:
+//   k_mlp_desc   (PHASE 1) half kick, thermostat scale, drift, wrap
+//                (integrator.rs:28-45) into buffer cur ^ 1, and FP32 copies
+//                of the drifted positions;
+//   k_mlp_desc   (PHASE 2) the features of the drifted positions, and the
+//                list entries within the cutoff;
+//   k_mlp_net    per 16-atom row tile of a wave: the forward pass and the
 //                backward pass to dE/dG on BF16 MFMA (FP32 accumulation),
-//                weights staged in LDS once per block, one block per CU;
+//                transposed so that activations stay in registers, weights
+//                staged in LDS once per block, one 16-wave block per CU;
 //   k_mlp_force  pair forces from both atoms' dE/dG, the LJ fallback when a
 //                feature left the training range, the periodic LJ check,
 //                second half kick and barostat scale into buffer cur;
@@ -274,7 +277,7 @@ __global__ __launch_bounds__(BS) void k_mlp_desc(DevState d) {
 // fragment load touches fall on distinct banks.
 constexpr int kStrK = kMlpK + 8;  // rows of 32
 constexpr int kStrH = kMlpH + 8;  // rows of 128
-constexpr int kNetWaves = 8;
+constexpr int kNetWaves = 16;
 constexpr int kNetBS = 64 * kNetWaves;
 
 // D (16 x 16, FP32) of v_mfma_f32_16x16x32_bf16 in lane l: rows 4 (l / 16) + i,
