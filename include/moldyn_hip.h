@@ -78,7 +78,8 @@ typedef struct {
 typedef struct {
     int force_mode;        /* MD_FORCE_*; default MD_FORCE_FP32_LIST */
     double skin;           /* Verlet skin, nm; default 0.3 * sigma(0,0) */
-    int block_size;        /* threads per block of the per-atom kernels: 64, 128, 256 or 512; default 256 */
+    int block_size;        /* threads per block of the per-atom kernels: 64, 128, 256 or 512; 0 = automatic
+                              (256, or 512 for MD_FORCE_MLP); default 256 */
     int list_capacity;     /* initial neighbours per atom; 0 = automatic; grown on overflow */
     int cell_capacity;     /* initial atoms per cell slot array; 0 = automatic; grown on overflow */
     double cells_per_atom; /* cell-grid budget per replica, cells per atom; default 2 */

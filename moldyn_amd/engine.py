@@ -34,7 +34,7 @@ class Engine:
     box: 3 values, or n_replicas x 3.
     """
 
-    def __init__(self, counts, masses, box, force_mode="fp32_list", skin=None, block_size=256,
+    def __init__(self, counts, masses, box, force_mode="fp32_list", skin=None, block_size=None,
                  cells_per_atom=2.0, graph_chunk=32, list_capacity=0, cell_capacity=0,
                  tiles_per_block=0):
         self._lib = L.load()
@@ -59,7 +59,7 @@ class Engine:
         self._chk(self._lib.md_get_options(self._h, C.byref(opt)))
         opt.force_mode = FORCE_MODES[force_mode]
         opt.skin = float(skin) if skin is not None else 0.0
-        opt.block_size = int(block_size)
+        opt.block_size = int(block_size) if block_size is not None else 0  # 0: the library's choice
         opt.cells_per_atom = float(cells_per_atom)
         opt.graph_chunk = int(graph_chunk)
         opt.list_capacity = int(list_capacity)
