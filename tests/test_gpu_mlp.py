@@ -98,6 +98,33 @@ def test_mlp_fallback_is_the_lj_path(thermo):
     assert st["fallbacks"] == 300
 
 
+@pytest.mark.parametrize("npt", [False, True])
+def test_mlp_schedule_invariance(npt):
+    """Surrogate trajectories do not depend on the skin, on split steps (a
+    skin thinner than one step's drift: the second launch copies its drifted
+    positions for the features), on capacity-overflow replays, on the block
+    size or on graph capture: features and forces sum the pairs within the
+    cutoff in slot order, whatever list holds them."""
+    import moldyn_amd as M
+    s = liquid_state(4, temperature=150.0)
+    p = oracle_params(s, seed=11)
+    outs = []
+    for kw in [dict(), dict(skin=0.002), dict(skin=0.4), dict(list_capacity=2, cell_capacity=1),
+               dict(graph_chunk=1), dict(block_size=64)]:
+        e = mlp_engine(s, p, judge_every=0, **kw)
+        e.update_force()
+        th = M.thermostat("berendsen", tau=1.0, target_t=200.0) if npt else None
+        ba = M.barostat("berendsen", beta=1.0, tau=5.0, target_p=5.0) if npt else None
+        e.step(0.002, 60, th, ba)
+        outs.append((e.download(), e.stats()))
+        e.close()
+    assert outs[1][1]["split_steps"] >= 10 and outs[0][1]["split_steps"] == 0
+    assert outs[3][1]["grows"] >= 1
+    for o, st in outs[1:]:
+        for key in ("x", "v", "force", "box"):
+            assert np.array_equal(o[key], outs[0][0][key]), (key, st)
+
+
 def test_mlp_periodic_lj_check():
     s = liquid_state(4, temperature=100.0)
     p = oracle_params(s, seed=8)
