@@ -63,12 +63,12 @@ __device__ __forceinline__ void list_close(const DevState& d, int64_t slot, int 
 // longer list continues in global memory (sorted insertion); store() writes
 // the register part.  With SEG and a column the builder hands over (room for
 // the list capacity, at a stride of seg_stride; dense systems), the scan
-// appends the staging index of every candidate the FP32 test accepts, and
-// the builder then runs the FP64 test, compacts and sorts the column once,
-// in loops every lane runs at the same time: in a dense brick some lane of
-// the wave accepts a candidate at nearly every step of the scan, and work
-// per accepted entry inside the scan would make the whole wave execute it
-// each time.
+// lanes append the staging index of every candidate the FP32 test accepts
+// (brick_build), and one lane then runs the FP64 test, compacts and sorts
+// the column once (store_seg), in loops every lane runs at the same time:
+// in a dense brick some lane of the wave accepts a candidate at nearly every
+// step of the scan, and work per accepted entry inside the scan would make
+// the whole wave execute it each time.
 template <bool SEG>
 struct RegList {
     int32_t v[kListReg];
@@ -80,12 +80,6 @@ struct RegList {
     int seg_stride = 0;
     __device__ __forceinline__ RegList(int32_t* c, int64_t stride, int capacity)
         : n(0), col(c), N(stride), cap(capacity) {}
-    // SEG: append a candidate's staging index to the column (past the
-    // capacity a list is only counted: list_close reports the overflow)
-    __device__ __forceinline__ void push_raw(int32_t b2) {
-        if (n < cap) seg[n * seg_stride] = b2;
-        ++n;
-    }
     // SEG: sort the column's first m entries (slot ids) and store them
     __device__ __forceinline__ void store_seg(int m) {
         for (int q = 1; q < m; ++q) {
@@ -438,22 +432,22 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
         const int dl = d.list_cap > 1 ? d.list_cap : 1;
         const int per_pass = SEG ? (n_own < kBrickSeg / dl ? n_own : (kBrickSeg / dl > 1 ? kBrickSeg / dl : 1))
                                  : n_own;
-        for (int p0 = 0; p0 < n_own; p0 += per_pass) {
-        const int p1 = n_own - p0 < per_pass ? n_own : p0 + per_pass;
-        const int seg_cap = kBrickSeg / (p1 - p0);
-        for (int k = p0 + threadIdx.x; k < p1; k += BS) {
-            const int a = s_cidx[k];
+        // the stencil of staged centre particle a: its halo cell hc, the x and
+        // y offsets [lo, hi], and its z cells as at most two runs of
+        // consecutive halo cells, whose particles are consecutive in the
+        // staging order
+        auto stencil = [&](int a, int (&hc)[3], int (&lo)[2], int (&hi)[2], int& za0, int& za1, int& zb0,
+                           int& zb1) {
             const int h = s_hc[a];
-            const int hc[3] = {h / (hd[1] * hd[2]), (h / hd[2]) % hd[1], h % hd[2]};
-            int lo[2], hi[2];
+            hc[0] = h / (hd[1] * hd[2]);
+            hc[1] = (h / hd[2]) % hd[1];
+            hc[2] = h % hd[2];
             for (int q = 0; q < 2; ++q) {
                 const bool w = (whole >> q) & 1;
                 lo[q] = w ? (gd[q] >= 3 ? -1 : 0) : -1;
                 hi[q] = w ? (gd[q] >= 2 ? 1 : 0) : 1;
             }
-            // the stencil's z cells as at most two runs of consecutive halo
-            // cells, whose particles are consecutive in the staging order
-            int za0, za1, zb0 = 0, zb1 = 0;
+            zb0 = zb1 = 0;
             if (!((whole >> 2) & 1)) {
                 za0 = hc[2] - 1;
                 za1 = hc[2] + 2;
@@ -474,16 +468,106 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                 za0 = hc[2] - 1;
                 za1 = hc[2] + 2;
             }
+        };
+        for (int p0 = 0; p0 < n_own; p0 += per_pass) {
+        const int p1 = n_own - p0 < per_pass ? n_own : p0 + per_pass;
+        const int seg_cap = kBrickSeg / (p1 - p0);
+        if (SEG && seg_cap >= d.list_cap) {
+            // Dense bricks, every list in an LDS column: T lanes per centre
+            // particle share its nine stencil rows (a brick has fewer centre
+            // particles than the block has threads) and append the
+            // FP32-accepted candidates to the particle's column through a
+            // counter in s_cen (free once the staging is done).  One lane per
+            // particle then runs the FP64 test, compacts and sorts the column
+            // by slot id, so the list does not depend on the order of the
+            // appends.
+            const int npp = p1 - p0;
+            const int T = BS / npp < 1 ? 1 : (BS / npp > 4 ? 4 : BS / npp);
+            int* const s_cnt = s_cen;
+            for (int k = threadIdx.x; k < npp; k += BS) s_cnt[k] = 0;
+            lds_barrier();
+            for (int kk = threadIdx.x; kk < npp * T; kk += BS) {
+                const int k = p0 + kk / T, sub = kk % T;
+                const int a = s_cidx[k];
+                int hc[3], lo[2], hi[2], za0, za1, zb0, zb1;
+                stencil(a, hc, lo, hi, za0, za1, zb0, zb1);
+                const int ti = MULTI ? (int)s_ty[a] : 0;
+                const float4 pi = s_pf[a];
+                const float xf = pi.x, yf = pi.y, zf = pi.z;
+                int32_t* const col = L.seg + (k - p0);
+                int* const cnt = s_cnt + (k - p0);
+                const int cap = d.list_cap;
+                auto scan = [&](int b0, int b1) {
+                    // the next candidate's position is read while this one is
+                    // tested (the index stays in the run)
+                    float4 pn = s_pf[b0 < b1 ? b0 : 0];
+                    for (int b2 = b0; b2 < b1; ++b2) {
+                        // FP32 test first: it never rejects a pair the FP64 test keeps
+                        const float4 pj = pn;
+                        pn = s_pf[b2 + 1 < b1 ? b2 + 1 : b2];
+                        const float ex = min_image_f(pj.x - xf, bxf);
+                        const float ey = min_image_f(pj.y - yf, byf);
+                        const float ez = min_image_f(pj.z - zf, bzf);
+                        if (!((ex * ex + ey * ey) + ez * ez <= rl2f)) continue;
+                        // a particle is staged once per region: b2 == a is j == i
+                        if (b2 == a) continue;
+                        if (MULTI && (int)s_ty[b2] < ti) continue;
+                        // past the capacity a candidate is only counted (an
+                        // overflow: the chunk is replayed with a larger one)
+                        const int pos = atomicAdd(cnt, 1);
+                        if (pos < cap) col[pos * npp] = b2;
+                    }
+                };
+                int row_i = 0;  // the particle's stencil rows dealt out over its T lanes
+                for (int o0 = lo[0]; o0 <= hi[0]; ++o0) {
+                    const int g0 = ((whole & 1) ? (hc[0] + o0 + gd[0]) % gd[0] : hc[0] + o0);
+                    for (int o1 = lo[1]; o1 <= hi[1]; ++o1) {
+                        if (row_i++ % T != sub) continue;
+                        const int g1 = ((whole & 2) ? (hc[1] + o1 + gd[1]) % gd[1] : hc[1] + o1);
+                        const int row = (g0 * hd[1] + g1) * hd[2];
+                        scan(s_off[row + za0], s_off[row + za1]);
+                        if (zb1 > zb0) scan(s_off[row + zb0], s_off[row + zb1]);
+                    }
+                }
+            }
+            lds_barrier();
+            for (int k = p0 + threadIdx.x; k < p1; k += BS) {
+                const int a = s_cidx[k];
+                const int32_t i = s_id[a];
+                const double xi = s_x[a], yi = s_y[a], zi = s_z[a];
+                const int n = s_cnt[k - p0];
+                RegList<SEG> lst(d.nbr + i, N, d.list_cap);
+                lst.seg = L.seg + (k - p0);
+                lst.seg_stride = npp;
+                // the FP64 test (the reference's minimum image) over the
+                // FP32-accepted candidates, in a loop every lane runs at once
+                int kept = 0;
+                const int m = n < d.list_cap ? n : d.list_cap;
+                // the next entry is read before this one is kept (kept <= q)
+                int bn = m > 0 ? lst.seg[0] : 0;
+                for (int q = 0; q < m; ++q) {
+                    const int b2 = bn;
+                    if (q + 1 < m) bn = lst.seg[(q + 1) * lst.seg_stride];
+                    const double dx = min_image_sel(s_x[b2] - xi, bx);
+                    const double dy = min_image_sel(s_y[b2] - yi, by);
+                    const double dz = min_image_sel(s_z[b2] - zi, bz);
+                    const double r2 = (dx * dx + dy * dy) + dz * dz;
+                    if (r2 <= rl2) lst.seg[kept++ * lst.seg_stride] = s_id[b2];
+                }
+                lst.store_seg(kept);
+                list_close(d, i, n > d.list_cap ? n : kept);
+            }
+        } else
+        for (int k = p0 + threadIdx.x; k < p1; k += BS) {
+            const int a = s_cidx[k];
+            int hc[3], lo[2], hi[2], za0, za1, zb0, zb1;
+            stencil(a, hc, lo, hi, za0, za1, zb0, zb1);
             const int32_t i = s_id[a];
             const int ti = MULTI ? (int)s_ty[a] : 0;
             const double xi = s_x[a], yi = s_y[a], zi = s_z[a];
             const float4 pi = s_pf[a];
             const float xf = pi.x, yf = pi.y, zf = pi.z;
             RegList<SEG> lst(d.nbr + i, N, d.list_cap);
-            if (SEG && seg_cap >= d.list_cap) {
-                lst.seg = L.seg + (k - p0);
-                lst.seg_stride = p1 - p0;
-            }
             auto scan = [&](int b0, int b1) {
                 // the next candidate's position is read while this one is
                 // tested (the index stays in the run)
@@ -499,10 +583,6 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                     // a particle is staged once per region: b2 == a is j == i
                     if (b2 == a) continue;
                     if (MULTI && (int)s_ty[b2] < ti) continue;
-                    if (SEG && lst.seg) {
-                        lst.push_raw(b2);  // the FP64 test runs once the scan is done
-                        continue;
-                    }
                     const int32_t j = s_id[b2];
                     const double dx = min_image_sel(s_x[b2] - xi, bx);
                     const double dy = min_image_sel(s_y[b2] - yi, by);
@@ -521,30 +601,8 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                     if (zb1 > zb0) scan(s_off[row + zb0], s_off[row + zb1]);
                 }
             }
-            if (SEG && lst.seg) {
-                // the FP64 test (the reference's minimum image) over the
-                // FP32-accepted candidates, in a loop every lane runs at once
-                int kept = 0;
-                const int m = lst.n < d.list_cap ? lst.n : d.list_cap;
-                // the next entry is read before this one is kept (kept <= q)
-                int bn = m > 0 ? lst.seg[0] : 0;
-                for (int q = 0; q < m; ++q) {
-                    const int b2 = bn;
-                    if (q + 1 < m) bn = lst.seg[(q + 1) * lst.seg_stride];
-                    const double dx = min_image_sel(s_x[b2] - xi, bx);
-                    const double dy = min_image_sel(s_y[b2] - yi, by);
-                    const double dz = min_image_sel(s_z[b2] - zi, bz);
-                    const double r2 = (dx * dx + dy * dy) + dz * dz;
-                    if (r2 <= rl2) lst.seg[kept++ * lst.seg_stride] = s_id[b2];
-                }
-                lst.store_seg(kept);
-                // more FP32-accepted candidates than the capacity counts as
-                // an overflow (the chunk is replayed with a larger one)
-                list_close(d, i, lst.n > d.list_cap ? lst.n : kept);
-            } else {
-                lst.store();
-                list_close(d, i, lst.n);
-            }
+            lst.store();
+            list_close(d, i, lst.n);
         }
         if (p1 < n_own) lds_barrier();  // the next pass reuses the columns
         }  // passes
