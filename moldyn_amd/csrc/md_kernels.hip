@@ -477,10 +477,11 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             // particle share its nine stencil rows (a brick has fewer centre
             // particles than the block has threads) and append the
             // FP32-accepted candidates to the particle's column through a
-            // counter in s_cen (free once the staging is done).  One lane per
-            // particle then runs the FP64 test, compacts and sorts the column
-            // by slot id, so the list does not depend on the order of the
-            // appends.
+            // counter in s_cen (free once the staging is done).  Then the
+            // FP64 test and the ordering by slot id, so the list does not
+            // depend on the order of the appends: a wave per particle (lists
+            // up to 128 entries), else one lane per particle sorting its
+            // column.
             const int npp = p1 - p0;
             const int T = BS / npp < 1 ? 1 : (BS / npp > 4 ? 4 : BS / npp);
             int* const s_cnt = s_cen;
@@ -531,6 +532,51 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                 }
             }
             lds_barrier();
+            if (d.list_cap <= 128) {
+                // one wave per centre particle: lane q runs the FP64 test of
+                // column entries q and q + 64, and each kept slot id goes
+                // straight to its sorted position, its rank among the kept
+                // ids (distinct), counted over the wave's lanes
+                constexpr int NW = BS / 64;
+                const int wv = threadIdx.x >> 6, wl = threadIdx.x & 63;
+                for (int k = p0 + wv; k < p1; k += NW) {
+                    const int a = s_cidx[k];
+                    const int32_t i = s_id[a];
+                    const double xi = s_x[a], yi = s_y[a], zi = s_z[a];
+                    const int n = s_cnt[k - p0];
+                    const int m = n < d.list_cap ? n : d.list_cap;
+                    const int32_t* const colp = L.seg + (k - p0);
+                    auto kept_id = [&](int q) -> uint32_t {  // ~0u: rejected
+                        const int b2 = colp[q * npp];
+                        const double dx = min_image_sel(s_x[b2] - xi, bx);
+                        const double dy = min_image_sel(s_y[b2] - yi, by);
+                        const double dz = min_image_sel(s_z[b2] - zi, bz);
+                        const double r2 = (dx * dx + dy * dy) + dz * dz;
+                        return r2 <= rl2 ? (uint32_t)s_id[b2] : ~0u;
+                    };
+                    const uint32_t v0 = wl < m ? kept_id(wl) : ~0u;
+                    const uint32_t v1 = wl + 64 < m ? kept_id(wl + 64) : ~0u;
+                    int r0 = 0, r1 = 0;
+                    const int m0 = m < 64 ? m : 64;
+                    for (int l = 0; l < m0; ++l) {
+                        const uint32_t x = __builtin_amdgcn_readlane(v0, l);
+                        r0 += x < v0 ? 1 : 0;
+                        r1 += x < v1 ? 1 : 0;
+                    }
+                    for (int l = 64; l < m; ++l) {
+                        const uint32_t x = __builtin_amdgcn_readlane(v1, l - 64);
+                        r0 += x < v0 ? 1 : 0;
+                        r1 += x < v1 ? 1 : 0;
+                    }
+                    int32_t* const col = d.nbr + i;
+                    if (v0 != ~0u) col[(int64_t)r0 * N] = (int32_t)v0;
+                    if (v1 != ~0u) col[(int64_t)r1 * N] = (int32_t)v1;
+                    const int kept = __popcll(__ballot(v0 != ~0u)) + __popcll(__ballot(v1 != ~0u));
+                    // more FP32-accepted candidates than the capacity counts as
+                    // an overflow (the chunk is replayed with a larger one)
+                    if (wl == 0) list_close(d, i, n > d.list_cap ? n : kept);
+                }
+            } else
             for (int k = p0 + threadIdx.x; k < p1; k += BS) {
                 const int a = s_cidx[k];
                 const int32_t i = s_id[a];
