@@ -24,6 +24,8 @@
 // the reference (Rust) never forms.  The FP32 pair kernel uses explicit fmaf.
 #include <math.h>
 
+#include <type_traits>
+
 #include "md_common.h"
 
 namespace md {
@@ -245,6 +247,15 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
     // 2 sqrt(3) eps (a branch taken the other way near L/2 doubles it).  The
     // squared sum adds 3 roundings; 2^-20 covers them.
     const float bxf = (float)bx, byf = (float)by, bzf = (float)bz;
+    // With every axis tiled into at least four cells, the FP32 copies hold
+    // each staged particle's periodic image next to its halo cell, relative
+    // to the region's first cell (|coordinate| <= 1.25 L): two particles of
+    // adjacent cells (|unwrapped difference| < 2 cells) within r_list
+    // (<= 1 cell) then have their minimum-image difference as the plain
+    // difference, since the two differ by a multiple of L smaller than
+    // 3 cells <= 0.75 L.  The prefilter needs no minimum image, and the
+    // roundings stay inside eps below (1.25 + 1.25 + 0.5 < 5.05).
+    const bool unwrap = whole == 0 && gd[0] >= 4 && gd[1] >= 4 && gd[2] >= 4;
     const double eps = 5.05 * 0x1p-24 * fmax(bx, fmax(by, bz));
     const double rlm = P.rlist + 2.0 * 1.7320508075688772 * eps;
     const float rl2f = __double2float_ru(rlm * rlm * (1.0 + 0x1p-20));
@@ -315,6 +326,15 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             const int c1 = ((org[1] + h1) % gd[1] + gd[1]) % gd[1];
             const int c2 = ((org[2] + h2) % gd[2] + gd[2]) % gd[2];
             return R.cell_base + ((int64_t)c0 * gd[1] + c1) * gd[2] + c2;
+        };
+        // the unwrapping shift of halo cell h along axis k (unwrap): the
+        // image of the cell's particles next to the region, less the
+        // region's first cell
+        auto shift = [&](int h, int k, double b) -> double {
+            const int hk = k == 0 ? h / (hd[1] * hd[2]) : (k == 1 ? (h / hd[2]) % hd[1] : h % hd[2]);
+            const int g = org[k] + hk;
+            const int w = g < 0 ? -1 : (g >= gd[k] ? 1 : 0);
+            return (double)w * b - (double)org[k] * (b / (double)gd[k]);
         };
         // all header loads of a thread in flight at once
         constexpr int HP = (kBrickCells + BS - 1) / BS;
@@ -420,7 +440,11 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             s_x[a] = o.x;
             s_y[a] = o.y;
             s_z[a] = o.z;
-            s_pf[a] = make_float4((float)o.x, (float)o.y, (float)o.z, 0.0f);
+            if (unwrap)
+                s_pf[a] = make_float4((float)(o.x + shift(h, 0, bx)), (float)(o.y + shift(h, 1, by)),
+                                      (float)(o.z + shift(h, 2, bz)), 0.0f);
+            else
+                s_pf[a] = make_float4((float)o.x, (float)o.y, (float)o.z, 0.0f);
             s_id[a] = j;
             if (MULTI) s_ty[a] = (uint8_t)rec_type(d, recs, j);
         }
@@ -498,7 +522,9 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                 int32_t* const col = L.seg + (k - p0);
                 int* const cnt = s_cnt + (k - p0);
                 const int cap = d.list_cap;
-                auto scan = [&](int b0, int b1) {
+                // uw: std::true_type when the FP32 copies are unwrapped
+                auto scan = [&](auto uw, int b0, int b1) {
+                    constexpr bool UW = decltype(uw)::value;
                     // the next candidate's position is read while this one is
                     // tested (the index stays in the run)
                     float4 pn = s_pf[b0 < b1 ? b0 : 0];
@@ -506,9 +532,9 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                         // FP32 test first: it never rejects a pair the FP64 test keeps
                         const float4 pj = pn;
                         pn = s_pf[b2 + 1 < b1 ? b2 + 1 : b2];
-                        const float ex = min_image_f(pj.x - xf, bxf);
-                        const float ey = min_image_f(pj.y - yf, byf);
-                        const float ez = min_image_f(pj.z - zf, bzf);
+                        const float ex = UW ? pj.x - xf : min_image_f(pj.x - xf, bxf);
+                        const float ey = UW ? pj.y - yf : min_image_f(pj.y - yf, byf);
+                        const float ez = UW ? pj.z - zf : min_image_f(pj.z - zf, bzf);
                         if (!((ex * ex + ey * ey) + ez * ez <= rl2f)) continue;
                         // a particle is staged once per region: b2 == a is j == i
                         if (b2 == a) continue;
@@ -519,17 +545,23 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                         if (pos < cap) col[pos * npp] = b2;
                     }
                 };
-                int row_i = 0;  // the particle's stencil rows dealt out over its T lanes
-                for (int o0 = lo[0]; o0 <= hi[0]; ++o0) {
-                    const int g0 = ((whole & 1) ? (hc[0] + o0 + gd[0]) % gd[0] : hc[0] + o0);
-                    for (int o1 = lo[1]; o1 <= hi[1]; ++o1) {
-                        if (row_i++ % T != sub) continue;
-                        const int g1 = ((whole & 2) ? (hc[1] + o1 + gd[1]) % gd[1] : hc[1] + o1);
-                        const int row = (g0 * hd[1] + g1) * hd[2];
-                        scan(s_off[row + za0], s_off[row + za1]);
-                        if (zb1 > zb0) scan(s_off[row + zb0], s_off[row + zb1]);
+                auto rows = [&](auto uw) {
+                    int row_i = 0;  // the particle's stencil rows dealt out over its T lanes
+                    for (int o0 = lo[0]; o0 <= hi[0]; ++o0) {
+                        const int g0 = ((whole & 1) ? (hc[0] + o0 + gd[0]) % gd[0] : hc[0] + o0);
+                        for (int o1 = lo[1]; o1 <= hi[1]; ++o1) {
+                            if (row_i++ % T != sub) continue;
+                            const int g1 = ((whole & 2) ? (hc[1] + o1 + gd[1]) % gd[1] : hc[1] + o1);
+                            const int row = (g0 * hd[1] + g1) * hd[2];
+                            scan(uw, s_off[row + za0], s_off[row + za1]);
+                            if (zb1 > zb0) scan(uw, s_off[row + zb0], s_off[row + zb1]);
+                        }
                     }
-                }
+                };
+                if (unwrap)
+                    rows(std::true_type{});
+                else
+                    rows(std::false_type{});
             }
             lds_barrier();
             if (d.list_cap <= 128) {
@@ -614,7 +646,9 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             const float4 pi = s_pf[a];
             const float xf = pi.x, yf = pi.y, zf = pi.z;
             RegList<SEG> lst(d.nbr + i, N, d.list_cap);
-            auto scan = [&](int b0, int b1) {
+            // uw: std::true_type when the FP32 copies are unwrapped
+            auto scan = [&](auto uw, int b0, int b1) {
+                constexpr bool UW = decltype(uw)::value;
                 // the next candidate's position is read while this one is
                 // tested (the index stays in the run)
                 float4 pn = s_pf[b0 < b1 ? b0 : 0];
@@ -622,9 +656,9 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                     // FP32 test first: it never rejects a pair the FP64 test keeps
                     const float4 pj = pn;
                     pn = s_pf[b2 + 1 < b1 ? b2 + 1 : b2];
-                    const float ex = min_image_f(pj.x - xf, bxf);
-                    const float ey = min_image_f(pj.y - yf, byf);
-                    const float ez = min_image_f(pj.z - zf, bzf);
+                    const float ex = UW ? pj.x - xf : min_image_f(pj.x - xf, bxf);
+                    const float ey = UW ? pj.y - yf : min_image_f(pj.y - yf, byf);
+                    const float ez = UW ? pj.z - zf : min_image_f(pj.z - zf, bzf);
                     if (!((ex * ex + ey * ey) + ez * ez <= rl2f)) continue;
                     // a particle is staged once per region: b2 == a is j == i
                     if (b2 == a) continue;
@@ -638,15 +672,21 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                     lst.push(j);
                 }
             };
-            for (int o0 = lo[0]; o0 <= hi[0]; ++o0) {
-                const int g0 = ((whole & 1) ? (hc[0] + o0 + gd[0]) % gd[0] : hc[0] + o0);
-                for (int o1 = lo[1]; o1 <= hi[1]; ++o1) {
-                    const int g1 = ((whole & 2) ? (hc[1] + o1 + gd[1]) % gd[1] : hc[1] + o1);
-                    const int row = (g0 * hd[1] + g1) * hd[2];
-                    scan(s_off[row + za0], s_off[row + za1]);
-                    if (zb1 > zb0) scan(s_off[row + zb0], s_off[row + zb1]);
+            auto rows = [&](auto uw) {
+                for (int o0 = lo[0]; o0 <= hi[0]; ++o0) {
+                    const int g0 = ((whole & 1) ? (hc[0] + o0 + gd[0]) % gd[0] : hc[0] + o0);
+                    for (int o1 = lo[1]; o1 <= hi[1]; ++o1) {
+                        const int g1 = ((whole & 2) ? (hc[1] + o1 + gd[1]) % gd[1] : hc[1] + o1);
+                        const int row = (g0 * hd[1] + g1) * hd[2];
+                        scan(uw, s_off[row + za0], s_off[row + za1]);
+                        if (zb1 > zb0) scan(uw, s_off[row + zb0], s_off[row + zb1]);
+                    }
                 }
-            }
+            };
+            if (unwrap)
+                rows(std::true_type{});
+            else
+                rows(std::false_type{});
             lst.store();
             list_close(d, i, lst.n);
         }
