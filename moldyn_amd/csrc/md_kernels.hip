@@ -564,11 +564,13 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                     rows(std::false_type{});
             }
             lds_barrier();
-            if (d.list_cap <= 128) {
+            if (d.dense && d.list_cap <= 128) {
                 // one wave per centre particle: lane q runs the FP64 test of
                 // column entries q and q + 64, and each kept slot id goes
                 // straight to its sorted position, its rank among the kept
-                // ids (distinct), counted over the wave's lanes
+                // ids (distinct), counted over the wave's lanes (sparse
+                // systems, whose columns hold a few entries, keep one lane
+                // per particle: a wave per particle would serialize them)
                 constexpr int NW = BS / 64;
                 const int wv = threadIdx.x >> 6, wl = threadIdx.x & 63;
                 for (int k = p0 + wv; k < p1; k += NW) {
