@@ -319,19 +319,44 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             hd[k] = w ? gd[k] : ncen[k] + 2;
         }
         const int H = hd[0] * hd[1] * hd[2];
+        // halo cell h = (h0 hd1 + h1) hd2 + h2, split with float reciprocals
+        // (no integer division): for h < 2^13 the estimate is off by at most
+        // one, which the remainder corrects
+        const int hd12 = hd[1] * hd[2];
+        const float rhd12 = 1.0f / (float)hd12, rhd2 = 1.0f / (float)hd[2];
+        auto divmod = [](int n, int dv, float rdv, int& q) -> int {
+            int qq = (int)((float)n * rdv);
+            int rr = n - qq * dv;
+            if (rr < 0) {
+                --qq;
+                rr += dv;
+            } else if (rr >= dv) {
+                ++qq;
+                rr -= dv;
+            }
+            q = qq;
+            return rr;
+        };
+        auto split = [&](int h, int& h0, int& h1, int& h2) {
+            const int rem = divmod(h, hd12, rhd12, h0);
+            h2 = divmod(rem, hd[2], rhd2, h1);
+        };
+        // a halo coordinate's grid cell along axis k (org + h in [-1, 2 gd))
+        auto wrapc = [&](int g, int k) -> int {
+            g += g < 0 ? gd[k] : 0;
+            return g >= gd[k] ? g - gd[k] : g;
+        };
         // halo cell h -> grid cell (halo coordinates wrap periodically)
         auto cell_of = [&](int h) -> int64_t {
-            const int h0 = h / (hd[1] * hd[2]), h1 = (h / hd[2]) % hd[1], h2 = h % hd[2];
-            const int c0 = ((org[0] + h0) % gd[0] + gd[0]) % gd[0];
-            const int c1 = ((org[1] + h1) % gd[1] + gd[1]) % gd[1];
-            const int c2 = ((org[2] + h2) % gd[2] + gd[2]) % gd[2];
+            int h0, h1, h2;
+            split(h, h0, h1, h2);
+            const int c0 = wrapc(org[0] + h0, 0), c1 = wrapc(org[1] + h1, 1), c2 = wrapc(org[2] + h2, 2);
             return R.cell_base + ((int64_t)c0 * gd[1] + c1) * gd[2] + c2;
         };
-        // the unwrapping shift of halo cell h along axis k (unwrap): the
+        // the unwrapping shift of axis k for halo coordinate hk (unwrap): the
         // image of the cell's particles next to the region, less the
         // region's first cell
-        auto shift = [&](int h, int k, double b) -> double {
-            const int hk = k == 0 ? h / (hd[1] * hd[2]) : (k == 1 ? (h / hd[2]) % hd[1] : h % hd[2]);
+        auto shift = [&](int hk, int k, double b) -> double {
             const int g = org[k] + hk;
             const int w = g < 0 ? -1 : (g >= gd[k] ? 1 : 0);
             return (double)w * b - (double)org[k] * (b / (double)gd[k]);
@@ -349,7 +374,8 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             const int h = threadIdx.x + q * BS;
             if (h < H) {
                 const int cnt = (uint32_t)(hw[q] >> 32) == ep ? (int)(uint32_t)hw[q] : 0;
-                const int h0 = h / (hd[1] * hd[2]), h1 = (h / hd[2]) % hd[1], h2 = h % hd[2];
+                int h0, h1, h2;
+                split(h, h0, h1, h2);
                 const bool own = ((whole & 1) || (h0 >= 1 && h0 <= ncen[0])) &&
                                  ((whole & 2) || (h1 >= 1 && h1 <= ncen[1])) &&
                                  ((whole & 4) || (h2 >= 1 && h2 <= ncen[2]));
@@ -440,11 +466,14 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
             s_x[a] = o.x;
             s_y[a] = o.y;
             s_z[a] = o.z;
-            if (unwrap)
-                s_pf[a] = make_float4((float)(o.x + shift(h, 0, bx)), (float)(o.y + shift(h, 1, by)),
-                                      (float)(o.z + shift(h, 2, bz)), 0.0f);
-            else
+            if (unwrap) {
+                int h0, h1, h2;
+                split(h, h0, h1, h2);
+                s_pf[a] = make_float4((float)(o.x + shift(h0, 0, bx)), (float)(o.y + shift(h1, 1, by)),
+                                      (float)(o.z + shift(h2, 2, bz)), 0.0f);
+            } else {
                 s_pf[a] = make_float4((float)o.x, (float)o.y, (float)o.z, 0.0f);
+            }
             s_id[a] = j;
             if (MULTI) s_ty[a] = (uint8_t)rec_type(d, recs, j);
         }
@@ -462,10 +491,7 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
         // staging order
         auto stencil = [&](int a, int (&hc)[3], int (&lo)[2], int (&hi)[2], int& za0, int& za1, int& zb0,
                            int& zb1) {
-            const int h = s_hc[a];
-            hc[0] = h / (hd[1] * hd[2]);
-            hc[1] = (h / hd[2]) % hd[1];
-            hc[2] = h % hd[2];
+            split(s_hc[a], hc[0], hc[1], hc[2]);
             for (int q = 0; q < 2; ++q) {
                 const bool w = (whole >> q) & 1;
                 lo[q] = w ? (gd[q] >= 3 ? -1 : 0) : -1;
