@@ -597,44 +597,76 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                 // ids (distinct), counted over the wave's lanes (sparse
                 // systems, whose columns hold a few entries, keep one lane
                 // per particle: a wave per particle would serialize them)
+                // Two particles per wave and iteration, their loads and
+                // tests interleaved (each particle's chain of dependent LDS
+                // reads would otherwise run alone).
                 constexpr int NW = BS / 64;
                 const int wv = threadIdx.x >> 6, wl = threadIdx.x & 63;
-                for (int k = p0 + wv; k < p1; k += NW) {
-                    const int a = s_cidx[k];
-                    const int32_t i = s_id[a];
-                    const double xi = s_x[a], yi = s_y[a], zi = s_z[a];
-                    const int n = s_cnt[k - p0];
-                    const int m = n < d.list_cap ? n : d.list_cap;
-                    const int32_t* const colp = L.seg + (k - p0);
-                    auto kept_id = [&](int q) -> uint32_t {  // ~0u: rejected
-                        const int b2 = colp[q * npp];
-                        const double dx = min_image_sel(s_x[b2] - xi, bx);
-                        const double dy = min_image_sel(s_y[b2] - yi, by);
-                        const double dz = min_image_sel(s_z[b2] - zi, bz);
+                const int cap = d.list_cap;
+                for (int k0 = p0 + wv; k0 < p1; k0 += 2 * NW) {
+                    const bool has1 = k0 + NW < p1;
+                    int n[2], m[2];
+                    int32_t ii[2];
+                    double xi[2], yi[2], zi[2];
+                    const int32_t* colp[2];
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        const int k = p == 0 || has1 ? k0 + p * NW : k0;
+                        const int a = s_cidx[k];
+                        ii[p] = s_id[a];
+                        xi[p] = s_x[a];
+                        yi[p] = s_y[a];
+                        zi[p] = s_z[a];
+                        n[p] = p == 0 || has1 ? s_cnt[k - p0] : 0;
+                        m[p] = n[p] < cap ? n[p] : cap;
+                        colp[p] = L.seg + (k - p0);
+                    }
+                    auto kept_id = [&](int p, int q) -> uint32_t {  // ~0u: rejected
+                        const int b2 = colp[p][q * npp];
+                        const double dx = min_image_sel(s_x[b2] - xi[p], bx);
+                        const double dy = min_image_sel(s_y[b2] - yi[p], by);
+                        const double dz = min_image_sel(s_z[b2] - zi[p], bz);
                         const double r2 = (dx * dx + dy * dy) + dz * dz;
                         return r2 <= rl2 ? (uint32_t)s_id[b2] : ~0u;
                     };
-                    const uint32_t v0 = wl < m ? kept_id(wl) : ~0u;
-                    const uint32_t v1 = wl + 64 < m ? kept_id(wl + 64) : ~0u;
-                    int r0 = 0, r1 = 0;
-                    const int m0 = m < 64 ? m : 64;
-                    for (int l = 0; l < m0; ++l) {
-                        const uint32_t x = __builtin_amdgcn_readlane(v0, l);
-                        r0 += x < v0 ? 1 : 0;
-                        r1 += x < v1 ? 1 : 0;
+                    uint32_t v0[2], v1[2];
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        v0[p] = wl < m[p] ? kept_id(p, wl) : ~0u;
+                        v1[p] = wl + 64 < m[p] ? kept_id(p, wl + 64) : ~0u;
                     }
-                    for (int l = 64; l < m; ++l) {
-                        const uint32_t x = __builtin_amdgcn_readlane(v1, l - 64);
-                        r0 += x < v0 ? 1 : 0;
-                        r1 += x < v1 ? 1 : 0;
+                    // ranks: lanes past a particle's m hold ~0u, below no id
+                    int r0[2] = {0, 0}, r1[2] = {0, 0};
+                    const int mx = m[0] > m[1] ? m[0] : m[1];
+                    const int mx0 = mx < 64 ? mx : 64;
+                    for (int l = 0; l < mx0; ++l) {
+#pragma unroll
+                        for (int p = 0; p < 2; ++p) {
+                            const uint32_t x = __builtin_amdgcn_readlane(v0[p], l);
+                            r0[p] += x < v0[p] ? 1 : 0;
+                            r1[p] += x < v1[p] ? 1 : 0;
+                        }
                     }
-                    int32_t* const col = d.nbr + i;
-                    if (v0 != ~0u) col[(int64_t)r0 * N] = (int32_t)v0;
-                    if (v1 != ~0u) col[(int64_t)r1 * N] = (int32_t)v1;
-                    const int kept = __popcll(__ballot(v0 != ~0u)) + __popcll(__ballot(v1 != ~0u));
-                    // more FP32-accepted candidates than the capacity counts as
-                    // an overflow (the chunk is replayed with a larger one)
-                    if (wl == 0) list_close(d, i, n > d.list_cap ? n : kept);
+                    for (int l = 64; l < mx; ++l) {
+#pragma unroll
+                        for (int p = 0; p < 2; ++p) {
+                            const uint32_t x = __builtin_amdgcn_readlane(v1[p], l - 64);
+                            r0[p] += x < v0[p] ? 1 : 0;
+                            r1[p] += x < v1[p] ? 1 : 0;
+                        }
+                    }
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        if (p == 1 && !has1) break;
+                        int32_t* const col = d.nbr + ii[p];
+                        if (v0[p] != ~0u) col[(int64_t)r0[p] * N] = (int32_t)v0[p];
+                        if (v1[p] != ~0u) col[(int64_t)r1[p] * N] = (int32_t)v1[p];
+                        const int kept = __popcll(__ballot(v0[p] != ~0u)) + __popcll(__ballot(v1[p] != ~0u));
+                        // more FP32-accepted candidates than the capacity
+                        // counts as an overflow (the chunk is replayed with
+                        // a larger one)
+                        if (wl == 0) list_close(d, ii[p], n[p] > cap ? n[p] : kept);
+                    }
                 }
             } else
             for (int k = p0 + threadIdx.x; k < p1; k += BS) {
