@@ -597,27 +597,27 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                 // ids (distinct), counted over the wave's lanes (sparse
                 // systems, whose columns hold a few entries, keep one lane
                 // per particle: a wave per particle would serialize them)
-                // Two particles per wave and iteration, their loads and
+                // PP particles per wave and iteration, their loads and
                 // tests interleaved (each particle's chain of dependent LDS
                 // reads would otherwise run alone).
-                constexpr int NW = BS / 64;
+                constexpr int NW = BS / 64, PP = 4;
                 const int wv = threadIdx.x >> 6, wl = threadIdx.x & 63;
                 const int cap = d.list_cap;
-                for (int k0 = p0 + wv; k0 < p1; k0 += 2 * NW) {
-                    const bool has1 = k0 + NW < p1;
-                    int n[2], m[2];
-                    int32_t ii[2];
-                    double xi[2], yi[2], zi[2];
-                    const int32_t* colp[2];
+                for (int k0 = p0 + wv; k0 < p1; k0 += PP * NW) {
+                    int n[PP], m[PP];
+                    int32_t ii[PP];
+                    double xi[PP], yi[PP], zi[PP];
+                    const int32_t* colp[PP];
 #pragma unroll
-                    for (int p = 0; p < 2; ++p) {
-                        const int k = p == 0 || has1 ? k0 + p * NW : k0;
+                    for (int p = 0; p < PP; ++p) {
+                        const bool has = k0 + p * NW < p1;
+                        const int k = has ? k0 + p * NW : k0;
                         const int a = s_cidx[k];
                         ii[p] = s_id[a];
                         xi[p] = s_x[a];
                         yi[p] = s_y[a];
                         zi[p] = s_z[a];
-                        n[p] = p == 0 || has1 ? s_cnt[k - p0] : 0;
+                        n[p] = has ? s_cnt[k - p0] : 0;
                         m[p] = n[p] < cap ? n[p] : cap;
                         colp[p] = L.seg + (k - p0);
                     }
@@ -629,19 +629,24 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                         const double r2 = (dx * dx + dy * dy) + dz * dz;
                         return r2 <= rl2 ? (uint32_t)s_id[b2] : ~0u;
                     };
-                    uint32_t v0[2], v1[2];
+                    uint32_t v0[PP], v1[PP];
 #pragma unroll
-                    for (int p = 0; p < 2; ++p) {
+                    for (int p = 0; p < PP; ++p) {
                         v0[p] = wl < m[p] ? kept_id(p, wl) : ~0u;
                         v1[p] = wl + 64 < m[p] ? kept_id(p, wl + 64) : ~0u;
                     }
                     // ranks: lanes past a particle's m hold ~0u, below no id
-                    int r0[2] = {0, 0}, r1[2] = {0, 0};
-                    const int mx = m[0] > m[1] ? m[0] : m[1];
+                    int r0[PP], r1[PP];
+                    int mx = 0;
+#pragma unroll
+                    for (int p = 0; p < PP; ++p) {
+                        r0[p] = r1[p] = 0;
+                        mx = m[p] > mx ? m[p] : mx;
+                    }
                     const int mx0 = mx < 64 ? mx : 64;
                     for (int l = 0; l < mx0; ++l) {
 #pragma unroll
-                        for (int p = 0; p < 2; ++p) {
+                        for (int p = 0; p < PP; ++p) {
                             const uint32_t x = __builtin_amdgcn_readlane(v0[p], l);
                             r0[p] += x < v0[p] ? 1 : 0;
                             r1[p] += x < v1[p] ? 1 : 0;
@@ -649,15 +654,15 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                     }
                     for (int l = 64; l < mx; ++l) {
 #pragma unroll
-                        for (int p = 0; p < 2; ++p) {
+                        for (int p = 0; p < PP; ++p) {
                             const uint32_t x = __builtin_amdgcn_readlane(v1[p], l - 64);
                             r0[p] += x < v0[p] ? 1 : 0;
                             r1[p] += x < v1[p] ? 1 : 0;
                         }
                     }
 #pragma unroll
-                    for (int p = 0; p < 2; ++p) {
-                        if (p == 1 && !has1) break;
+                    for (int p = 0; p < PP; ++p) {
+                        if (k0 + p * NW >= p1) break;
                         int32_t* const col = d.nbr + ii[p];
                         if (v0[p] != ~0u) col[(int64_t)r0[p] * N] = (int32_t)v0[p];
                         if (v1[p] != ~0u) col[(int64_t)r1[p] * N] = (int32_t)v1[p];
