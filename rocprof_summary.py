@@ -196,17 +196,29 @@ def counters(args):
         if not found:
             raise SystemExit(f"no counter_collection.csv under {d}")
         files += found
+    rows = []
     for f in files:
         with open(f, newline="") as fh:
             for r in csv.DictReader(fh):
-                if not pat.search(r["Kernel_Name"]):
-                    continue
-                per.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+                if pat.search(r["Kernel_Name"]):
+                    rows.append((int(r["Dispatch_Id"]), r["Counter_Name"], float(r["Counter_Value"])))
+    keep = None
+    if args.select:
+        # dispatches whose counter reaches the threshold; the passes run the
+        # same program, so a dispatch id names the same launch in each
+        name, lo = args.select.split(":")
+        keep = {i for i, c, v in rows if c == name and v >= float(lo)}
+    for i, c, v in rows:
+        if keep is None or i in keep:
+            per.setdefault(c, []).append(v)
     if not per:
         raise SystemExit(f"no dispatch of a kernel matching {args.kernel!r}")
     lines = [f"# rocprofv3 --pmc counters: {args.title}", "", "commands:" if len(args.command) > 1 else "command:", ""]
     lines += [f"    {c}" for c in args.command]
-    lines += ["", f"kernel regex: {args.kernel}", "",
+    lines += ["", f"kernel regex: {args.kernel}"]
+    if args.select:
+        lines += [f"dispatches with {args.select.replace(':', ' >= ')} only ({len(keep)})"]
+    lines += ["",
              f"{'counter':34s} {'dispatches':>10s} {'median':>16s} {'p10':>16s} {'p90':>16s}"]
     for name in sorted(per):
         vs = sorted(per[name])
@@ -329,6 +341,7 @@ def main(argv=None):
     c.add_argument("--command", required=True, action="append",
                    help="the command of each pass (repeat once per pass)")
     c.add_argument("--title", default="")
+    c.add_argument("--select", help="COUNTER:MIN: only dispatches whose COUNTER reaches MIN")
     c.add_argument("--trace", help="kernel-trace directory of the same program, for durations")
     c.add_argument("--trace-command")
     c.add_argument("--out")
