@@ -635,7 +635,8 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                         v0[p] = wl < m[p] ? kept_id(p, wl) : ~0u;
                         v1[p] = wl + 64 < m[p] ? kept_id(p, wl + 64) : ~0u;
                     }
-                    // ranks: lanes past a particle's m hold ~0u, below no id
+                    // ranks of the kept ids (lanes past a particle's m hold
+                    // ~0u, below no id)
                     int r0[PP], r1[PP];
                     int mx = 0;
 #pragma unroll
@@ -643,13 +644,33 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                         r0[p] = r1[p] = 0;
                         mx = m[p] > mx ? m[p] : mx;
                     }
-                    const int mx0 = mx < 64 ? mx : 64;
-                    for (int l = 0; l < mx0; ++l) {
+                    if (mx <= 64) {
+                        // one entry per lane: a bitonic sort over the wave
+                        // (ids ascending, the ~0u of rejected entries last),
+                        // after which lane r holds the entry of rank r
 #pragma unroll
-                        for (int p = 0; p < PP; ++p) {
-                            const uint32_t x = __builtin_amdgcn_readlane(v0[p], l);
-                            r0[p] += x < v0[p] ? 1 : 0;
-                            r1[p] += x < v1[p] ? 1 : 0;
+                        for (int kb = 2; kb <= 64; kb <<= 1) {
+#pragma unroll
+                            for (int jb = kb >> 1; jb > 0; jb >>= 1) {
+                                const bool keep_min = ((wl & kb) == 0) == ((wl & jb) == 0);
+#pragma unroll
+                                for (int p = 0; p < PP; ++p) {
+                                    const uint32_t o = (uint32_t)__shfl_xor((int)v0[p], jb, 64);
+                                    v0[p] = keep_min ? (o < v0[p] ? o : v0[p]) : (o > v0[p] ? o : v0[p]);
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int p = 0; p < PP; ++p) r0[p] = wl;
+                    } else {
+                        const int mx0 = mx < 64 ? mx : 64;
+                        for (int l = 0; l < mx0; ++l) {
+#pragma unroll
+                            for (int p = 0; p < PP; ++p) {
+                                const uint32_t x = __builtin_amdgcn_readlane(v0[p], l);
+                                r0[p] += x < v0[p] ? 1 : 0;
+                                r1[p] += x < v1[p] ? 1 : 0;
+                            }
                         }
                     }
                     for (int l = 64; l < mx; ++l) {
