@@ -604,7 +604,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-launch device timestamps (no roofline)")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--tune-steps", type=int, default=300)
+    ap.add_argument("--tune-steps", type=int, default=1000)
+    ap.add_argument("--tune-repeats", type=int, default=2)
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the 64k NVE, 256-replica and surrogate measurements")
     ap.add_argument("--mlp-frames", type=int, default=2000)
@@ -654,8 +655,12 @@ def main():
         mid = e0.download(("x", "v", "box"))
         e0.close()
         log(f"[bench] tuning at step {pre}: box {mid['box'][0]:.4f}")
+        # 1000-step windows hold about ten list builds at this state, and
+        # the faster of two windows counts: settings within a few per cent
+        # of each other (the cell budget, tiles per block) are otherwise
+        # ranked by noise
         opts, _ = autotune([n], [MASS], mid["box"], mid["x"], mid["v"], DT, lambda: controls(M),
-                           steps=args.tune_steps, warmup=20, log=log)
+                           steps=args.tune_steps, warmup=20, log=log, repeats=args.tune_repeats)
         log(f"[bench] autotuned options: {opts}")
     e = M.Engine([n], [MASS], box, force_mode="fp32_list", graph_chunk=args.graph_chunk, **opts)
 

@@ -23,16 +23,21 @@ TILES = (0, 1, 2, 4, 8)
 CELLS = (0.5, 1.0, 2.0, 4.0)
 
 
-def _trial(counts, masses, box, x, v, dt, controls, steps, warmup, force_mode, opts):
+def _trial(counts, masses, box, x, v, dt, controls, steps, warmup, force_mode, opts, repeats=1):
     e = Engine(counts, masses, box, force_mode=force_mode, **opts)
     try:
         e.upload(x, v, box)
         e.update_force()
         th, ba = controls()
         e.step(dt, warmup, th, ba)
-        t0 = time.perf_counter()
-        e.step(dt, steps, th, ba)
-        t = (time.perf_counter() - t0) / steps
+        # the fastest of `repeats` consecutive windows: a one-off stall of the
+        # host or the GPU should not decide between settings a few per cent
+        # apart
+        t = float("inf")
+        for _ in range(max(1, repeats)):
+            t0 = time.perf_counter()
+            e.step(dt, steps, th, ba)
+            t = min(t, (time.perf_counter() - t0) / steps)
         return t, e.stats()
     finally:
         e.close()
@@ -40,15 +45,17 @@ def _trial(counts, masses, box, x, v, dt, controls, steps, warmup, force_mode, o
 
 def autotune(counts, masses, box, x, v, dt, controls, steps=300, warmup=20,
              force_mode="fp32_list", skins=SKINS, blocks=BLOCKS, tiles=TILES, cells=CELLS,
-             log=None):
+             log=None, repeats=1):
     """controls() -> (thermostat, barostat) as passed to Engine.step (fresh objects).
+    Each setting is timed over `repeats` windows of `steps` steps (the fastest
+    counts).
 
     Returns (best_options, table) where table lists (options, seconds_per_step, stats)."""
     best = {"skin": 0.3 * 0.3418, "block_size": 256, "tiles_per_block": 0, "cells_per_atom": 2.0}
     table = []
 
     def run(opts):
-        t, st = _trial(counts, masses, box, x, v, dt, controls, steps, warmup, force_mode, opts)
+        t, st = _trial(counts, masses, box, x, v, dt, controls, steps, warmup, force_mode, opts, repeats)
         table.append((dict(opts), t, st))
         if log:
             log(f"[autotune] {opts} -> {t * 1e6:.2f} us/step (rebuilds {st['rebuilds']}, "
