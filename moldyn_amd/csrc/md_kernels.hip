@@ -1504,6 +1504,15 @@ __device__ __forceinline__ void advance(RepCtrl& C, const RepParams& P, const Re
     const int T = R.n_types;
     const double dt = P.dt;
     const bool baro = P.ba_kind == 1;
+    // the list terms that depend on the sums alone, computed first so that
+    // their square roots overlap the control chain below (the same
+    // operations as where they are used: the values are bitwise the same)
+    double vmax = 0.0, wmax = 0.0;
+    for (int t = 0; t < T; ++t) {
+        vmax = fmax(vmax, S[t * NS + S_VMAX]);
+        wmax = fmax(wmax, (sqrt(S[t * NS + S_WV]) + (dt / 2.0) * sqrt(S[t * NS + S_WA])) * (1.0 + 1e-12));
+    }
+    const double vmax_root = sqrt(vmax);
     // lists built in this launch describe the current positions exactly
     const bool built = mode == M_STEP && (C.flags & F_BUILD);
     bool controls = true, decide = true;
@@ -1563,9 +1572,7 @@ __device__ __forceinline__ void advance(RepCtrl& C, const RepParams& P, const Re
             if (P.list_mode) {
                 if (!(f & F_SPLIT_B)) {
                     // the list of a split step was built after this step's drift
-                    double vmax = 0.0;
-                    for (int t = 0; t < T; ++t) vmax = fmax(vmax, S[t * NS + S_VMAX]);
-                    C.D += sqrt(vmax) * (1.0 + 1e-12) * dt / C.s;
+                    C.D += vmax_root * (1.0 + 1e-12) * dt / C.s;
                 }
                 if (baro) C.s *= C.myu;
             }
@@ -1618,10 +1625,7 @@ __device__ __forceinline__ void advance(RepCtrl& C, const RepParams& P, const Re
     }
     if (P.list_mode && decide) {
         // |v + F dt / 2m| <= |v| + (dt / 2) |F| / m for every particle
-        double wmax = 0.0;
-        for (int t = 0; t < T; ++t)
-            wmax = fmax(wmax, (sqrt(S[t * NS + S_WV]) + (dt / 2.0) * sqrt(S[t * NS + S_WA])) *
-                                  (1.0 + 1e-12));
+        // (wmax above)
         const double w = C.lambda * wmax * dt / C.s;
         const double thr = P.rlist - P.rc_max / C.s - P.margin;
         const bool need = !(C.flags & F_HAVE_LIST) || (P.rebuild_every_step && !built) ||
