@@ -378,6 +378,7 @@ static int alloc_lists(md_ctx* c) {
         if ((rc = dalloc(c, c->list_allocs, &p, sizeof(int32_t) * (size_t)N))) return rc;
         c->d.mlp_ncount = (int32_t*)p;
     }
+    c->d.cell_plane = c->total_cells;  // slot k of cell c at k * total_cells + c
     for (int k = 0; k < 2; ++k) {
         if ((rc = dalloc(c, c->list_allocs, &p, sizeof(unsigned long long) * (size_t)c->total_cells)))
             return rc;

@@ -501,7 +501,7 @@ __device__ __forceinline__ void grid_insert(const DevState& d, const RepCtrl& C,
         old = atomicCAS(h, assumed, nw);
     } while (old != assumed);
     if ((int)k < d.cell_cap) {
-        d.cell_slots[buf][cell * d.cell_cap + k] = (int32_t)slot;
+        d.cell_slots[buf][(int64_t)k * d.cell_plane + cell] = (int32_t)slot;
         // within a quarter of the capacity: report, the host grows it between chunks
         if (4 * ((int)k + 1) > 3 * d.cell_cap) atomicMax(&d.stats->max_cell, (int)k + 1);
     } else {

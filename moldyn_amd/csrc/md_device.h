@@ -220,8 +220,13 @@ struct DevState {
     int32_t* ncount;
     int32_t list_cap;
     unsigned long long* cell_hdr[2];
+    // slot k of cell c at cell_slots[buf][k * cell_plane + c] (k < cell_cap):
+    // a plane per slot rank, so that the first slots of a run of cells are
+    // contiguous (a sparse grid holds about one particle per cell, and a
+    // brick reads every slot plane of its halo in runs of cells)
     int32_t* cell_slots[2];
     int32_t cell_cap;
+    int64_t cell_plane;
     const int32_t* blk_group;
     const int32_t* bld_rep;  // k_build block -> replica
     const GroupInfo* groups;

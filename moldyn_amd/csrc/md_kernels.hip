@@ -163,7 +163,7 @@ __device__ void build_list_global(const DevState& d, const RepCtrl& C, const Rep
                 int cnt = (int)(uint32_t)h;
                 cnt = cnt < cc ? cnt : cc;
                 for (int k = 0; k < cnt; ++k) {
-                    const int32_t j = cs[cell * cc + k];
+                    const int32_t j = cs[(int64_t)k * d.cell_plane + cell];
                     if ((int64_t)j == slot) continue;
                     const RT o = recs[j];
                     if (MULTI && rec_type(d, recs, j) < ti) continue;
@@ -444,7 +444,8 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
                     if (s_cen[mid] <= k) lo = mid;
                     else hi = mid;
                 }
-                build_list_global<RT, MULTI>(d, C, P, R, recs, cs[cell_of(lo) * cc + (k - s_cen[lo])]);
+                build_list_global<RT, MULTI>(d, C, P, R, recs,
+                                               cs[(int64_t)(k - s_cen[lo]) * d.cell_plane + cell_of(lo)]);
             }
         } else {
         // staged particle -> its halo cell, centre particle -> staging index
@@ -461,7 +462,7 @@ __device__ __forceinline__ void brick_build(const DevState& d, const RepCtrl& C,
         lds_barrier();
         for (int a = threadIdx.x; a < total; a += BS) {
             const int h = s_hc[a];
-            const int32_t j = cs[cell_of(h) * cc + (a - s_off[h])];
+            const int32_t j = cs[(int64_t)(a - s_off[h]) * d.cell_plane + cell_of(h)];
             const RT& o = recs[j];
             s_x[a] = o.x;
             s_y[a] = o.y;
